@@ -1,0 +1,301 @@
+"""Python host binding of the MI355X push-relabel graph cut.
+
+A thin ctypes layer over the C ABI of libcugip_hip.so (include/cugip_hip.h), mirroring the
+reference's cugip::Graph<TFlow> interface (include/cugip/advanced_operations/graph_cut.hpp:47-146):
+same method names, same argument meaning, errors raised as exceptions. The C++ header
+include/cugip/graph_cut.hpp is the drop-in for C++ callers; this module serves Python
+callers, the tests and bench.py.
+
+The library must have been built (python -c "import __graft_entry__ as g; g.build()"); there is
+no fallback: without it, importing this module raises ImportError.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcugip_hip.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"{LIB_PATH} is missing: build the HIP library first (__graft_entry__.build())")
+
+_lib = ctypes.CDLL(LIB_PATH)
+
+CGP_OK = 0
+CGP_ERR_INVALID = 1
+CGP_ERR_NOMEM = 2
+CGP_ERR_NOT_CONVERGED = 3
+CGP_ERR_OVERFLOW = 4
+CGP_ERR_DEVICE = 5
+CGP_ERR_STATE = 6
+LABEL_UNREACHABLE = 0x3FFFFFFF
+
+TRACE_STARTED, TRACE_BEGIN_ITERATION, TRACE_AFTER_RELABEL, TRACE_AFTER_PUSH, TRACE_FINISHED = range(5)
+
+
+class Edge(ctypes.Structure):
+    """cgp_edge / cugip::EdgeRecord layout: second at offset 0, first at offset 4."""
+    _fields_ = [("second", ctypes.c_uint32), ("first", ctypes.c_uint32)]
+
+
+class Policy(ctypes.Structure):
+    """cgp_policy; zero fields take the library default (cugip::GraphCutPolicy counterpart)."""
+    _fields_ = [
+        ("tile_x", ctypes.c_int32), ("tile_y", ctypes.c_int32), ("tile_z", ctypes.c_int32),
+        ("discharges_per_relabel", ctypes.c_int32),
+        ("max_tile_iterations", ctypes.c_int32),
+        ("max_rounds", ctypes.c_int32),
+        ("minimal_residual", ctypes.c_float),
+        ("reserved", ctypes.c_int32 * 8),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("flow", ctypes.c_int64),
+        ("rounds", ctypes.c_int32), ("bfs_passes", ctypes.c_int32),
+        ("discharge_phases", ctypes.c_int32), ("cut_passes", ctypes.c_int32),
+        ("bfs_tile_visits", ctypes.c_int64), ("discharge_tile_visits", ctypes.c_int64),
+        ("cut_tile_visits", ctypes.c_int64),
+        ("init_ms", ctypes.c_double), ("relabel_ms", ctypes.c_double), ("discharge_ms", ctypes.c_double),
+        ("cut_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
+        ("voxels", ctypes.c_int64),
+        ("tile_x", ctypes.c_int32), ("tile_y", ctypes.c_int32), ("tile_z", ctypes.c_int32), ("conn", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 8),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+
+
+TraceFn = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64,
+                           ctypes.POINTER(ctypes.c_int32), ctypes.c_int32)
+
+_vp = ctypes.c_void_p
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+
+_lib.cgp_create.argtypes = [ctypes.POINTER(_vp)]
+_lib.cgp_destroy.argtypes = [_vp]
+_lib.cgp_last_error.argtypes = [_vp]
+_lib.cgp_last_error.restype = ctypes.c_char_p
+_lib.cgp_set_vertex_count.argtypes = [_vp, ctypes.c_int32]
+_lib.cgp_set_nweights.argtypes = [_vp, ctypes.c_int32, ctypes.POINTER(Edge), _i32p, _i32p]
+_lib.cgp_set_tweights.argtypes = [_vp, _i32p, _i32p]
+_lib.cgp_set_grid.argtypes = [_vp] + [ctypes.c_int32] * 4 + [_i32p] * 4
+_lib.cgp_set_grid_device.argtypes = [_vp] + [ctypes.c_int32] * 4 + [_vp] * 4
+_lib.cgp_set_policy.argtypes = [_vp, ctypes.POINTER(Policy)]
+_lib.cgp_get_policy.argtypes = [_vp, ctypes.POINTER(Policy)]
+_lib.cgp_set_trace.argtypes = [_vp, TraceFn, _vp]
+_lib.cgp_max_flow.argtypes = [_vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(Stats)]
+_lib.cgp_fill_segments.argtypes = [_vp, _u8p, ctypes.c_uint8, ctypes.c_uint8]
+_lib.cgp_source_set.argtypes = [_vp, _u8p]
+_lib.cgp_labels.argtypes = [_vp, _i32p]
+_lib.cgp_generate_volume.argtypes = [ctypes.c_int32] * 3 + [ctypes.c_uint64, ctypes.c_uint64, _u8p]
+_lib.cgp_caps_from_volume.argtypes = [ctypes.c_int32] * 4 + [_u8p, _i32p, _i32p, _i32p]
+_lib.cgp_device_count.argtypes = [_i32p]
+_lib.cgp_abi_version.restype = ctypes.c_int32
+
+
+class CgpError(RuntimeError):
+    """Error status of the C ABI (the reference raises cugip::ExceptionBase, exception.hpp:20-46)."""
+
+    def __init__(self, code, message):
+        super().__init__(f"[code {code}] {message}")
+        self.code = code
+
+
+def library():
+    return _lib
+
+
+def abi_version():
+    return int(_lib.cgp_abi_version())
+
+
+def device_count():
+    n = ctypes.c_int32(0)
+    rc = _lib.cgp_device_count(ctypes.byref(n))
+    return int(n.value) if rc == CGP_OK else 0
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _p32(a):
+    return a.ctypes.data_as(_i32p)
+
+
+class _Handle:
+    def __init__(self):
+        h = _vp()
+        rc = _lib.cgp_create(ctypes.byref(h))
+        if rc != CGP_OK:
+            raise CgpError(rc, _lib.cgp_last_error(None).decode())
+        self._h = h
+        self._trace_cb = None
+        self.stats = None
+        self._n = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.cgp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc):
+        if rc != CGP_OK:
+            raise CgpError(rc, _lib.cgp_last_error(self._h).decode())
+
+    def set_policy(self, policy=None, **fields):
+        if policy is None:
+            policy = Policy()
+        for k, v in fields.items():
+            setattr(policy, k, v)
+        self._check(_lib.cgp_set_policy(self._h, ctypes.byref(policy)))
+
+    def get_policy(self):
+        p = Policy()
+        self._check(_lib.cgp_get_policy(self._h, ctypes.byref(p)))
+        return p
+
+    def set_trace(self, fn):
+        """fn(event, iteration, done, flow, level_starts) or None."""
+        if fn is None:
+            self._trace_cb = None
+            self._check(_lib.cgp_set_trace(self._h, TraceFn(), None))
+            return
+
+        def _cb(_user, event, iteration, done, flow, levels, count):
+            starts = [levels[i] for i in range(count)] if levels and count > 0 else []
+            fn(event, iteration, done, flow, starts)
+
+        self._trace_cb = TraceFn(_cb)
+        self._check(_lib.cgp_set_trace(self._h, self._trace_cb, None))
+
+    def max_flow(self, policy=None):
+        """Exact max-flow value (Python int). Stats of the solve land in self.stats."""
+        if policy is not None:
+            self.set_policy(policy)
+        flow = ctypes.c_int64(0)
+        st = Stats()
+        self._check(_lib.cgp_max_flow(self._h, ctypes.byref(flow), ctypes.byref(st)))
+        self.stats = st
+        return int(flow.value)
+
+    def fill_segments(self, mask=255, background=0):
+        """Reference segmentation (T_min): mask where the vertex still reaches t."""
+        out = np.empty(self._n, np.uint8)
+        self._check(_lib.cgp_fill_segments(self._h, out.ctypes.data_as(_u8p), mask, background))
+        return out
+
+    def source_set(self):
+        """Canonical minimum cut S_min as 0/1 bytes."""
+        out = np.empty(self._n, np.uint8)
+        self._check(_lib.cgp_source_set(self._h, out.ctypes.data_as(_u8p)))
+        return out
+
+    def labels(self):
+        out = np.empty(self._n, np.int32)
+        self._check(_lib.cgp_labels(self._h, _p32(out)))
+        return out
+
+
+class Graph(_Handle):
+    """Generic graph: cugip::Graph<TFlow> (set_vertex_count / set_nweights / set_tweights / max_flow /
+    fill_segments, graph_cut.hpp:54-107)."""
+
+    def set_vertex_count(self, n):
+        self._check(_lib.cgp_set_vertex_count(self._h, int(n)))
+        self._n = int(n)
+
+    def set_nweights(self, edges, fwd, bwd):
+        """edges: (m, 2) vertex pairs (normalised to first = min, second = max as EdgeRecord does);
+        fwd: capacity first -> second of the normalised record, bwd: second -> first."""
+        edges = np.asarray(edges, dtype=np.int64).reshape(-1, 2)
+        m = edges.shape[0]
+        rec = (Edge * max(m, 1))()
+        if m:
+            lo = np.minimum(edges[:, 0], edges[:, 1]).astype(np.uint32)
+            hi = np.maximum(edges[:, 0], edges[:, 1]).astype(np.uint32)
+            arr = np.frombuffer(rec, dtype=np.uint32, count=2 * m).reshape(m, 2)
+            arr[:, 0] = hi
+            arr[:, 1] = lo
+        fwd = _i32(fwd)
+        bwd = _i32(bwd)
+        self._check(_lib.cgp_set_nweights(self._h, m, rec, _p32(fwd), _p32(bwd)))
+
+    def set_tweights(self, source, sink):
+        source = _i32(source)
+        sink = _i32(sink)
+        if source.size != self._n or sink.size != self._n:
+            raise CgpError(CGP_ERR_INVALID, "t-link arrays must have one entry per vertex")
+        self._check(_lib.cgp_set_tweights(self._h, _p32(source), _p32(sink)))
+
+
+class GridGraph(_Handle):
+    """3D grid graph with implicit 6- or 26-connected edges (cgp_set_grid)."""
+
+    def set_grid(self, cap_source, cap_sink, cap_n, cap_n_back=None, conn=6):
+        cs = _i32(cap_source)
+        if cs.ndim != 3:
+            raise CgpError(CGP_ERR_INVALID, "cap_source must be a (nz, ny, nx) volume")
+        nz, ny, nx = cs.shape
+        ct = _i32(cap_sink)
+        cn = _i32(cap_n)
+        if ct.shape != cs.shape or cn.shape != (conn // 2, nz, ny, nx):
+            raise CgpError(CGP_ERR_INVALID, "capacity shapes do not match the volume")
+        cb = _i32(cap_n_back) if cap_n_back is not None else None
+        if cb is not None and cb.shape != cn.shape:
+            raise CgpError(CGP_ERR_INVALID, "cap_n_back shape does not match cap_n")
+        self._check(_lib.cgp_set_grid(self._h, nx, ny, nz, conn, _p32(cs), _p32(ct), _p32(cn),
+                                      _p32(cb) if cb is not None else None))
+        self._n = nx * ny * nz
+        self.shape = (nz, ny, nx)
+
+    def set_grid_device(self, shape, conn, cs_ptr, ct_ptr, cn_ptr, cnb_ptr=None):
+        """Capacities already resident in device memory (raw device pointers, e.g. tensor.data_ptr())."""
+        nz, ny, nx = shape
+        self._check(_lib.cgp_set_grid_device(self._h, nx, ny, nz, conn, _vp(cs_ptr), _vp(ct_ptr), _vp(cn_ptr),
+                                             _vp(cnb_ptr) if cnb_ptr else None))
+        self._n = nx * ny * nz
+        self.shape = (nz, ny, nx)
+
+
+def generate_volume(nx, ny, nz, seed, instance=0):
+    """Synthetic blob volume, uint8 (nz, ny, nx)."""
+    out = np.empty((nz, ny, nx), np.uint8)
+    rc = _lib.cgp_generate_volume(nx, ny, nz, seed, instance, out.ctypes.data_as(_u8p))
+    if rc != CGP_OK:
+        raise CgpError(rc, "cgp_generate_volume failed")
+    return out
+
+
+def caps_from_volume(volume, conn=6):
+    """Integer capacities of a volume: (cap_source, cap_sink, cap_n[conn/2])."""
+    vol = np.ascontiguousarray(volume, dtype=np.uint8)
+    nz, ny, nx = vol.shape
+    cs = np.empty(vol.shape, np.int32)
+    ct = np.empty(vol.shape, np.int32)
+    cn = np.empty((conn // 2,) + vol.shape, np.int32)
+    rc = _lib.cgp_caps_from_volume(nx, ny, nz, conn, vol.ctypes.data_as(_u8p), _p32(cs), _p32(ct), _p32(cn))
+    if rc != CGP_OK:
+        raise CgpError(rc, "cgp_caps_from_volume failed")
+    return cs, ct, cn
+
+
+def make_instance(nx, ny, nz, seed, instance=0, conn=6):
+    """Volume and capacities of the synthetic instance (seed, instance)."""
+    vol = generate_volume(nx, ny, nz, seed, instance)
+    cs, ct, cn = caps_from_volume(vol, conn)
+    return vol, cs, ct, cn

@@ -1,0 +1,166 @@
+// Shared host/device helpers of the HIP graph cut: error handling, owned device
+// buffers, wave64 reductions.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+
+#include "cugip_hip.h"
+
+namespace cgp {
+
+constexpr int32_t kLabelInf = CGP_LABEL_UNREACHABLE;
+
+struct Error : std::runtime_error {
+	int code;
+	Error(int aCode, const std::string &aMessage) : std::runtime_error(aMessage), code(aCode) {}
+};
+
+[[noreturn]] inline void fail(int code, const std::string &message) { throw Error(code, message); }
+
+inline void hip_check(hipError_t err, const char *expr, const char *file, int line)
+{
+	if (err != hipSuccess) {
+		fail(err == hipErrorOutOfMemory ? CGP_ERR_NOMEM : CGP_ERR_DEVICE,
+		     std::string(expr) + " failed: " + hipGetErrorString(err) + " (" + file + ":" + std::to_string(line) + ")");
+	}
+}
+
+#define CGP_HIP(expr) ::cgp::hip_check((expr), #expr, __FILE__, __LINE__)
+#define CGP_LAUNCH_CHECK() ::cgp::hip_check(hipGetLastError(), "kernel launch", __FILE__, __LINE__)
+
+// Owned device allocation (hipMalloc / hipFree), movable, not copyable.
+template <typename T>
+class DeviceBuffer {
+public:
+	DeviceBuffer() = default;
+	explicit DeviceBuffer(size_t count) { allocate(count); }
+	DeviceBuffer(const DeviceBuffer &) = delete;
+	DeviceBuffer &operator=(const DeviceBuffer &) = delete;
+	DeviceBuffer(DeviceBuffer &&other) noexcept : mPtr(other.mPtr), mCount(other.mCount)
+	{
+		other.mPtr = nullptr;
+		other.mCount = 0;
+	}
+	DeviceBuffer &operator=(DeviceBuffer &&other) noexcept
+	{
+		if (this != &other) {
+			release();
+			mPtr = other.mPtr;
+			mCount = other.mCount;
+			other.mPtr = nullptr;
+			other.mCount = 0;
+		}
+		return *this;
+	}
+	~DeviceBuffer() { release(); }
+
+	void allocate(size_t count)
+	{
+		if (count == mCount && mPtr) {
+			return;
+		}
+		release();
+		if (count == 0) {
+			return;
+		}
+		void *p = nullptr;
+		hipError_t err = hipMalloc(&p, count * sizeof(T));
+		if (err != hipSuccess) {
+			(void)hipGetLastError();
+			fail(CGP_ERR_NOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed: " +
+			                        hipGetErrorString(err));
+		}
+		mPtr = static_cast<T *>(p);
+		mCount = count;
+	}
+	void release()
+	{
+		if (mPtr) {
+			(void)hipFree(mPtr);
+		}
+		mPtr = nullptr;
+		mCount = 0;
+	}
+	T *get() const { return mPtr; }
+	size_t size() const { return mCount; }
+	size_t bytes() const { return mCount * sizeof(T); }
+
+private:
+	T *mPtr = nullptr;
+	size_t mCount = 0;
+};
+
+// Pinned host staging buffer (hipHostMalloc).
+template <typename T>
+class PinnedBuffer {
+public:
+	PinnedBuffer() = default;
+	PinnedBuffer(const PinnedBuffer &) = delete;
+	PinnedBuffer &operator=(const PinnedBuffer &) = delete;
+	~PinnedBuffer()
+	{
+		if (mPtr) {
+			(void)hipHostFree(mPtr);
+		}
+	}
+	void allocate(size_t count)
+	{
+		if (count <= mCount && mPtr) {
+			return;
+		}
+		if (mPtr) {
+			(void)hipHostFree(mPtr);
+			mPtr = nullptr;
+		}
+		void *p = nullptr;
+		CGP_HIP(hipHostMalloc(&p, count * sizeof(T), hipHostMallocDefault));
+		mPtr = static_cast<T *>(p);
+		mCount = count;
+	}
+	T *get() const { return mPtr; }
+	T &operator[](size_t i) const { return mPtr[i]; }
+
+private:
+	T *mPtr = nullptr;
+	size_t mCount = 0;
+};
+
+// ---- device helpers (wave64) ----------------------------------------------------------------
+
+__device__ inline long long wave_sum_i64(long long v)
+{
+#pragma unroll
+	for (int off = 32; off > 0; off >>= 1) {
+		v += __shfl_xor(v, off, 64);
+	}
+	return v;
+}
+
+// Block-wide int64 sum, result valid in thread 0. `scratch` holds one slot per wave.
+template <int kThreads>
+__device__ inline long long block_sum_i64(long long v, long long *scratch)
+{
+	static_assert(kThreads % 64 == 0, "block size must be a multiple of the wave size");
+	v = wave_sum_i64(v);
+	const int lane = threadIdx.x & 63;
+	const int wave = threadIdx.x >> 6;
+	if (lane == 0) {
+		scratch[wave] = v;
+	}
+	__syncthreads();
+	long long total = 0;
+	if (threadIdx.x == 0) {
+#pragma unroll
+		for (int w = 0; w < kThreads / 64; ++w) {
+			total += scratch[w];
+		}
+	}
+	return total;
+}
+
+}  // namespace cgp

@@ -1,0 +1,284 @@
+// C ABI of the graph cut (include/cugip_hip.h): handle management, argument
+// checks, error capture, and dispatch to the grid or the generic (CSR) solver.
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+
+#include "common.h"
+#include "solver.h"
+
+struct cgp_graph_s {
+	enum class Mode { None, Generic, Grid };
+	std::string last_error;
+	hipStream_t stream = nullptr;
+	int device = 0;
+	Mode mode = Mode::None;
+	cgp_policy policy = cgp::default_policy();
+	cgp::TraceHook trace;
+	std::unique_ptr<cgp::CsrSolver> csr;
+	std::unique_ptr<cgp::GridSolver> grid;
+};
+
+namespace {
+
+std::mutex g_create_error_mutex;
+std::string g_create_error;
+
+constexpr int32_t kAbiVersion = 1;
+
+template <typename Fn>
+int guarded(cgp_graph g, Fn &&fn)
+{
+	if (!g) {
+		return CGP_ERR_INVALID;
+	}
+	try {
+		CGP_HIP(hipSetDevice(g->device));
+		fn();
+		g->last_error.clear();
+		return CGP_OK;
+	} catch (const cgp::Error &e) {
+		g->last_error = e.what();
+		return e.code;
+	} catch (const std::bad_alloc &) {
+		g->last_error = "host allocation failed";
+		return CGP_ERR_NOMEM;
+	} catch (const std::exception &e) {
+		g->last_error = e.what();
+		return CGP_ERR_DEVICE;
+	}
+}
+
+cgp::CsrSolver &generic(cgp_graph g)
+{
+	if (g->mode != cgp_graph_s::Mode::Generic || !g->csr) {
+		cgp::fail(CGP_ERR_STATE, "no generic graph: call cgp_set_vertex_count first");
+	}
+	return *g->csr;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t cgp_abi_version(void) { return kAbiVersion; }
+
+int cgp_device_count(int32_t *count)
+{
+	if (!count) {
+		return CGP_ERR_INVALID;
+	}
+	int n = 0;
+	const hipError_t err = hipGetDeviceCount(&n);
+	if (err != hipSuccess) {
+		(void)hipGetLastError();
+		*count = 0;
+		return CGP_ERR_DEVICE;
+	}
+	*count = n;
+	return CGP_OK;
+}
+
+int cgp_create(cgp_graph *out)
+{
+	if (!out) {
+		return CGP_ERR_INVALID;
+	}
+	*out = nullptr;
+	std::unique_ptr<cgp_graph_s> g;
+	try {
+		g.reset(new cgp_graph_s());
+		int n = 0;
+		CGP_HIP(hipGetDeviceCount(&n));
+		if (n <= 0) {
+			cgp::fail(CGP_ERR_DEVICE, "no HIP device available");
+		}
+		CGP_HIP(hipGetDevice(&g->device));
+		CGP_HIP(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+	} catch (const cgp::Error &e) {
+		std::lock_guard<std::mutex> lock(g_create_error_mutex);
+		g_create_error = e.what();
+		return e.code;
+	} catch (const std::bad_alloc &) {
+		return CGP_ERR_NOMEM;
+	}
+	*out = g.release();
+	return CGP_OK;
+}
+
+int cgp_destroy(cgp_graph g)
+{
+	if (!g) {
+		return CGP_OK;
+	}
+	(void)hipSetDevice(g->device);
+	g->csr.reset();
+	g->grid.reset();
+	if (g->stream) {
+		(void)hipStreamSynchronize(g->stream);
+		(void)hipStreamDestroy(g->stream);
+	}
+	delete g;
+	return CGP_OK;
+}
+
+const char *cgp_last_error(cgp_graph g)
+{
+	if (!g) {
+		std::lock_guard<std::mutex> lock(g_create_error_mutex);
+		return g_create_error.c_str();
+	}
+	return g->last_error.c_str();
+}
+
+int cgp_set_vertex_count(cgp_graph g, int32_t vertex_count)
+{
+	return guarded(g, [&] {
+		if (vertex_count <= 0) {
+			cgp::fail(CGP_ERR_INVALID, "vertex count must be positive");
+		}
+		g->grid.reset();
+		if (!g->csr) {
+			g->csr.reset(new cgp::CsrSolver(g->stream));
+		}
+		g->mode = cgp_graph_s::Mode::Generic;
+		g->csr->set_vertex_count(vertex_count);
+	});
+}
+
+int cgp_set_nweights(cgp_graph g, int32_t edge_count, const cgp_edge *edges, const int32_t *fwd, const int32_t *bwd)
+{
+	return guarded(g, [&] { generic(g).set_nweights(edge_count, edges, fwd, bwd); });
+}
+
+int cgp_set_tweights(cgp_graph g, const int32_t *source, const int32_t *sink)
+{
+	return guarded(g, [&] { generic(g).set_tweights(source, sink); });
+}
+
+static int set_grid_impl(cgp_graph g, int32_t nx, int32_t ny, int32_t nz, int32_t conn, const int32_t *cs,
+                         const int32_t *ct, const int32_t *cn, const int32_t *cnb, bool device)
+{
+	return guarded(g, [&] {
+		g->csr.reset();
+		if (!g->grid) {
+			g->grid.reset(new cgp::GridSolver(g->stream));
+		}
+		g->mode = cgp_graph_s::Mode::Grid;
+		g->grid->set_grid(nx, ny, nz, conn, cs, ct, cn, cnb, device);
+	});
+}
+
+int cgp_set_grid(cgp_graph g, int32_t nx, int32_t ny, int32_t nz, int32_t conn, const int32_t *cap_source,
+                 const int32_t *cap_sink, const int32_t *cap_n, const int32_t *cap_n_back)
+{
+	return set_grid_impl(g, nx, ny, nz, conn, cap_source, cap_sink, cap_n, cap_n_back, false);
+}
+
+int cgp_set_grid_device(cgp_graph g, int32_t nx, int32_t ny, int32_t nz, int32_t conn, const int32_t *cap_source,
+                        const int32_t *cap_sink, const int32_t *cap_n, const int32_t *cap_n_back)
+{
+	return set_grid_impl(g, nx, ny, nz, conn, cap_source, cap_sink, cap_n, cap_n_back, true);
+}
+
+int cgp_set_policy(cgp_graph g, const cgp_policy *policy)
+{
+	return guarded(g, [&] {
+		if (!policy) {
+			g->policy = cgp::default_policy();
+			return;
+		}
+		if (policy->minimal_residual < 0.0f) {
+			cgp::fail(CGP_ERR_INVALID, "minimal_residual must be non-negative");
+		}
+		if (policy->tile_x < 0 || policy->tile_y < 0 || policy->tile_z < 0 || policy->discharges_per_relabel < 0 ||
+		    policy->max_tile_iterations < 0 || policy->max_rounds < 0) {
+			cgp::fail(CGP_ERR_INVALID, "policy fields must be non-negative");
+		}
+		g->policy = *policy;
+	});
+}
+
+int cgp_get_policy(cgp_graph g, cgp_policy *policy)
+{
+	if (!g || !policy) {
+		return CGP_ERR_INVALID;
+	}
+	*policy = g->policy;
+	return CGP_OK;
+}
+
+int cgp_set_trace(cgp_graph g, cgp_trace_fn fn, void *user)
+{
+	if (!g) {
+		return CGP_ERR_INVALID;
+	}
+	g->trace.fn = fn;
+	g->trace.user = user;
+	return CGP_OK;
+}
+
+int cgp_max_flow(cgp_graph g, int64_t *flow, cgp_stats *stats)
+{
+	return guarded(g, [&] {
+		int64_t f = 0;
+		switch (g->mode) {
+		case cgp_graph_s::Mode::Generic:
+			f = g->csr->solve(g->policy, stats, g->trace);
+			break;
+		case cgp_graph_s::Mode::Grid:
+			f = g->grid->solve(g->policy, stats, g->trace);
+			break;
+		default:
+			cgp::fail(CGP_ERR_STATE, "no graph set");
+		}
+		if (flow) {
+			*flow = f;
+		}
+	});
+}
+
+int cgp_fill_segments(cgp_graph g, uint8_t *out, uint8_t mask, uint8_t background)
+{
+	return guarded(g, [&] {
+		if (!out) {
+			cgp::fail(CGP_ERR_INVALID, "output pointer is null");
+		}
+		if (g->mode == cgp_graph_s::Mode::Grid) {
+			g->grid->sink_side(out, mask, background);
+		} else {
+			generic(g).sink_side(out, mask, background);
+		}
+	});
+}
+
+int cgp_source_set(cgp_graph g, uint8_t *out)
+{
+	return guarded(g, [&] {
+		if (!out) {
+			cgp::fail(CGP_ERR_INVALID, "output pointer is null");
+		}
+		if (g->mode == cgp_graph_s::Mode::Grid) {
+			g->grid->source_side(out);
+		} else {
+			generic(g).source_side(out);
+		}
+	});
+}
+
+int cgp_labels(cgp_graph g, int32_t *out)
+{
+	return guarded(g, [&] {
+		if (!out) {
+			cgp::fail(CGP_ERR_INVALID, "output pointer is null");
+		}
+		if (g->mode == cgp_graph_s::Mode::Grid) {
+			g->grid->labels(out);
+		} else {
+			generic(g).labels(out);
+		}
+	});
+}
+
+}  // extern "C"

@@ -1,0 +1,661 @@
+// HIP kernels of the 3D grid push-relabel (gfx950, wave64). Included by
+// grid_solver.hip only.
+//
+// Algorithm (the reference's relabel/push loop, AOD = include/cugip/advanced_operations/detail):
+//   * init:     residuals from capacities (initResidualsKernel, AOD/graph_cut_implementation.hpp:28-40),
+//               source t-links saturated into excess (pushThroughTLinksFromSourceKernel,
+//               AOD/graph_cut_push.hpp:9-24), plus the per-voxel cancellation min(cs, ct) sent
+//               straight to t.
+//   * relabel:  exact BFS distance to t over edges with residual > threshold, level 1 = voxels with a
+//               residual sink t-link (initBFSKernel + bfsPropagation*, AOD/graph_cut_relabeling.hpp:47-112),
+//               computed by tile-local fixpoint relaxation in LDS and passes over dirty tiles.
+//   * push:     one top-down level sweep, every voxel of level L pushing min(excess, residual) to any
+//               neighbour of lower label and, at level 1, to t (Push::compute + pushImplementation,
+//               AOD/graph_cut_push.hpp:160-186, 395-482). Here it runs inside a tile staged in LDS,
+//               alternated with a region relabel (exact distances inside the tile given the halo
+//               labels), so that flow crosses a whole tile per HBM round trip. Neighbouring tiles
+//               never run together (colour phases), so the only cross-tile write that needs an
+//               atomic is the excess of a halo voxel.
+//   * cut:      T_min = finite labels of the last exact relabel (fill_segments, graph_cut.hpp:304-333);
+//               S_min = closure of the excess voxels under residual edges (see DESIGN.md proof).
+#pragma once
+
+#include "common.h"
+#include "grid_tile.h"
+
+namespace cgp {
+
+// ---------------------------------------------------------------------------------------------
+// init: residual planes, cancellation, excess, residual sink link; validates int32 headroom.
+template <int kConn>
+__global__ __launch_bounds__(256) void k_grid_init(GridView g)
+{
+	using D = Dirs<kConn>;
+	constexpr int K = D::K, H = D::H;
+	__shared__ long long red[4];
+	long long cancelled = 0;
+	bool bad = false;
+	const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+	for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.nvox; v += stride) {
+		const int x = (int)(v % g.nx);
+		const int y = (int)((v / g.nx) % g.ny);
+		const int z = (int)(v / ((int64_t)g.nx * g.ny));
+		const int32_t cs = g.cs[v], ct = g.ct[v];
+		bad |= (cs < 0) | (ct < 0);
+		const int32_t m = cs < ct ? cs : ct;
+		cancelled += m;
+		long long incoming = cs - m;
+#pragma unroll
+		for (int d = 0; d < K; ++d) {
+			const int ux = x + D::dx(d), uy = y + D::dy(d), uz = z + D::dz(d);
+			int32_t out = 0, in = 0;
+			if (inside(g, ux, uy, uz)) {
+				if (d < H) {
+					out = g.cn[(int64_t)d * g.nvox + v];
+					in = g.cnb ? g.cnb[(int64_t)d * g.nvox + v] : out;
+				} else {
+					const int64_t u = voxel_index(g, ux, uy, uz);
+					in = g.cn[(int64_t)(d - H) * g.nvox + u];
+					out = g.cnb ? g.cnb[(int64_t)(d - H) * g.nvox + u] : in;
+				}
+				bad |= (out < 0) | (in < 0);
+				bad |= ((long long)out + (long long)in) > 0x7fffffffLL;
+				incoming += in;
+			}
+			g.r[(int64_t)d * g.nvox + v] = out;
+		}
+		bad |= incoming > 0x7fffffffLL;
+		g.e[v] = cs - m;
+		g.rt[v] = ct - m;
+	}
+	if (bad) {
+		atomicOr(g.status, 1);
+	}
+	const long long total = block_sum_i64<256>(cancelled, red);
+	if (threadIdx.x == 0 && total) {
+		atomicAdd(g.flow, (unsigned long long)total);
+	}
+}
+
+// ---------------------------------------------------------------------------------------------
+// Stage the label tile with its halo into LDS; voxels outside the volume read as kLabelInf.
+template <typename T>
+__device__ inline void load_label_tile(const GridView &g, int32_t *hl, int ox, int oy, int oz)
+{
+	for (int p = threadIdx.x; p < T::NP; p += T::NT) {
+		const int px = p % T::PX;
+		const int py = (p / T::PX) % T::PY;
+		const int pz = p / (T::PX * T::PY);
+		const int x = ox + px - 1, y = oy + py - 1, z = oz + pz - 1;
+		int32_t v = kLabelInf;
+		if (inside(g, x, y, z)) {
+			v = g.h[voxel_index(g, x, y, z)];
+		}
+		hl[p] = v;
+	}
+}
+
+// Mark the neighbouring tile in direction (dx, dy, dz) of tile (tx, ty, tz), if it exists.
+__device__ inline void mark_neighbour_tile(const GridView &g, uint8_t *flags, int tx, int ty, int tz, int slot)
+{
+	const int dx = slot % 3 - 1, dy = (slot / 3) % 3 - 1, dz = slot / 9 - 1;
+	const int nx = tx + dx, ny = ty + dy, nz = tz + dz;
+	if (nx >= 0 && nx < g.ntx && ny >= 0 && ny < g.nty && nz >= 0 && nz < g.ntz) {
+		flags[(nz * g.nty + ny) * g.ntx + nx] = 1;
+	}
+}
+
+// Relax one own voxel: label = min(label, 1 + label of any neighbour reachable over an open edge).
+template <typename T>
+__device__ inline bool relax_voxel(int32_t *hl, int p, uint32_t open)
+{
+	const int32_t cur = hl[p];
+	int32_t best = cur;
+#pragma unroll
+	for (int d = 0; d < T::K; ++d) {
+		if (open & (1u << d)) {
+			const int32_t nb = hl[p + T::poff(d)] + 1;
+			best = nb < best ? nb : best;
+		}
+	}
+	if (best < cur) {
+		hl[p] = best;
+		return true;
+	}
+	return false;
+}
+
+// Fixpoint of relax_voxel over the tile (own column up then down per sweep).
+template <typename T>
+__device__ inline void relax_tile(int32_t *hl, const uint32_t (&open)[T::TZ], const bool (&valid)[T::TZ], int lx, int ly)
+{
+	for (;;) {
+		int changed = 0;
+#pragma unroll
+		for (int k = 0; k < T::TZ; ++k) {
+			if (valid[k]) {
+				changed |= relax_voxel<T>(hl, T::pidx(lx, ly, k), open[k]);
+			}
+		}
+#pragma unroll
+		for (int k = T::TZ - 2; k >= 0; --k) {
+			if (valid[k]) {
+				changed |= relax_voxel<T>(hl, T::pidx(lx, ly, k), open[k]);
+			}
+		}
+		if (!__syncthreads_or(changed)) {
+			break;
+		}
+	}
+}
+
+// Neighbour-tile slot (0..26) of the voxel beyond own voxel (lx, ly, lz) in direction d,
+// or -1 when that voxel lies inside the tile.
+template <typename T>
+__device__ inline int leaving_slot(int lx, int ly, int lz, int d)
+{
+	using D = typename T::D;
+	const int x = lx + D::dx(d), y = ly + D::dy(d), z = lz + D::dz(d);
+	const int sx = x < 0 ? 0 : (x >= T::TX ? 2 : 1);
+	const int sy = y < 0 ? 0 : (y >= T::TY ? 2 : 1);
+	const int sz = z < 0 ? 0 : (z >= T::TZ ? 2 : 1);
+	const int slot = (sz * 3 + sy) * 3 + sx;
+	return slot == 13 ? -1 : slot;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Global relabel pass over a list of tiles. Labels start at kLabelInf (a fill before the first
+// pass) and only fall: level 1 = residual sink link > threshold, then 1 + the lowest neighbour
+// reachable over an edge with residual > threshold. Tiles run concurrently and may read stale
+// halo labels; a tile whose boundary label fell below what a neighbouring voxel could use marks
+// that neighbour's tile in dirty_next, so a pass that marks nothing ends the relabel. Every
+// processed tile refreshes its active flag (some voxel with excess and a finite label).
+template <int kConn, int kTX, int kTY, int kTZ>
+__global__ __launch_bounds__(kTX *kTY) void k_grid_bfs(GridView g, const int32_t *__restrict__ list,
+                                                       const int32_t *__restrict__ count, int32_t threshold,
+                                                       uint8_t *__restrict__ dirty_next,
+                                                       uint8_t *__restrict__ tile_active)
+{
+	using T = Tile<kConn, kTX, kTY, kTZ>;
+	if ((int)blockIdx.x >= *count) {
+		return;
+	}
+	const int tile = list[blockIdx.x];
+	int tx, ty, tz;
+	tile_coords(g, tile, tx, ty, tz);
+	const int ox = tx * T::TX, oy = ty * T::TY, oz = tz * T::TZ;
+
+	__shared__ int32_t hl[T::NP];
+	__shared__ uint8_t marks[27];
+	const int tid = threadIdx.x;
+	const int lx = tid % T::TX, ly = tid / T::TX;
+	const int gx = ox + lx, gy = oy + ly;
+	if (tid < 27) {
+		marks[tid] = 0;
+	}
+	load_label_tile<T>(g, hl, ox, oy, oz);
+	__syncthreads();
+
+	bool valid[T::TZ];
+	uint32_t open[T::TZ];
+	int32_t before[T::TZ];
+	bool has_excess[T::TZ];
+#pragma unroll
+	for (int k = 0; k < T::TZ; ++k) {
+		valid[k] = gx < g.nx && gy < g.ny && oz + k < g.nz;
+		open[k] = 0;
+		has_excess[k] = false;
+		const int p = T::pidx(lx, ly, k);
+		before[k] = hl[p];
+		if (valid[k]) {
+			const int64_t v = voxel_index(g, gx, gy, oz + k);
+#pragma unroll
+			for (int d = 0; d < T::K; ++d) {
+				open[k] |= (g.r[(int64_t)d * g.nvox + v] > threshold) ? (1u << d) : 0u;
+			}
+			has_excess[k] = g.e[v] > 0;
+			if (g.rt[v] > threshold && hl[p] > 1) {
+				hl[p] = 1;
+			}
+		}
+	}
+	__syncthreads();
+	relax_tile<T>(hl, open, valid, lx, ly);
+
+	int active = 0;
+#pragma unroll
+	for (int k = 0; k < T::TZ; ++k) {
+		if (!valid[k]) {
+			continue;
+		}
+		const int p = T::pidx(lx, ly, k);
+		const int32_t nl = hl[p];
+		active |= has_excess[k] && nl < kLabelInf;
+		if (nl != before[k]) {
+			g.h[voxel_index(g, gx, gy, oz + k)] = nl;
+#pragma unroll
+			for (int d = 0; d < T::K; ++d) {
+				// a voxel beyond the tile pulls this label; it must be relaxed again if it could improve
+				const int slot = leaving_slot<T>(lx, ly, k, d);
+				if (slot >= 0 && hl[p + T::poff(d)] > nl + 1) {
+					marks[slot] = 1;
+				}
+			}
+		}
+	}
+	active = __syncthreads_or(active);
+	if (tid < 27 && marks[tid]) {
+		mark_neighbour_tile(g, dirty_next, tx, ty, tz, tid);
+	}
+	if (tid == 0) {
+		tile_active[tile] = active ? 1 : 0;
+		atomicAdd(&g.counters[1], 1ull);
+	}
+}
+
+// ---------------------------------------------------------------------------------------------
+// Discharge one tile: repeated {region relabel, top-down level push} in LDS, then write back.
+// Runs only on tiles of one colour at a time, so no neighbouring tile is running.
+template <int kConn, int kTX, int kTY, int kTZ>
+__global__ __launch_bounds__(kTX *kTY) void k_grid_discharge(GridView g, const int32_t *__restrict__ list,
+                                                             const int32_t *__restrict__ count, int max_iters,
+                                                             uint8_t *__restrict__ tile_active)
+{
+	using T = Tile<kConn, kTX, kTY, kTZ>;
+	using D = typename T::D;
+	constexpr int K = T::K;
+	if ((int)blockIdx.x >= *count) {
+		return;
+	}
+	const int tile = list[blockIdx.x];
+	int tx, ty, tz;
+	tile_coords(g, tile, tx, ty, tz);
+	const int ox = tx * T::TX, oy = ty * T::TY, oz = tz * T::TZ;
+
+	__shared__ int32_t hl[T::NP];
+	__shared__ int32_t el[T::NV];
+	__shared__ int32_t rl[K][T::NV];
+	__shared__ uint32_t dm[T::NV];   // residual directions written during this visit
+	__shared__ unsigned long long key;
+	__shared__ long long red[T::NT / 64];
+	__shared__ uint8_t marks[27];
+
+	const int tid = threadIdx.x;
+	const int lx = tid % T::TX, ly = tid / T::TX;
+	const int gx = ox + lx, gy = oy + ly;
+	if (tid < 27) {
+		marks[tid] = 0;
+	}
+	if (tid == 0) {
+		key = 0;
+	}
+	load_label_tile<T>(g, hl, ox, oy, oz);
+
+	bool valid[T::TZ];
+	int32_t rtr[T::TZ];
+	int32_t e0[T::TZ], rt0[T::TZ], h0[T::TZ];
+#pragma unroll
+	for (int k = 0; k < T::TZ; ++k) {
+		valid[k] = gx < g.nx && gy < g.ny && oz + k < g.nz;
+		const int i = T::vidx(lx, ly, k);
+		dm[i] = 0;
+		int32_t ev = 0, rv = 0;
+		if (valid[k]) {
+			const int64_t v = voxel_index(g, gx, gy, oz + k);
+			ev = g.e[v];
+			rv = g.rt[v];
+#pragma unroll
+			for (int d = 0; d < K; ++d) {
+				rl[d][i] = g.r[(int64_t)d * g.nvox + v];
+			}
+		} else {
+#pragma unroll
+			for (int d = 0; d < K; ++d) {
+				rl[d][i] = 0;
+			}
+		}
+		el[i] = ev;
+		rtr[k] = rv;
+		e0[k] = ev;
+		rt0[k] = rv;
+	}
+	__syncthreads();
+#pragma unroll
+	for (int k = 0; k < T::TZ; ++k) {
+		h0[k] = hl[T::pidx(lx, ly, k)];
+	}
+
+	unsigned int step = 0;
+	long long sink = 0;
+	for (int it = 0; it < max_iters; ++it) {
+		// ---- region relabel: exact distances to t inside the tile, halo labels fixed
+		uint32_t open[T::TZ];
+#pragma unroll
+		for (int k = 0; k < T::TZ; ++k) {
+			const int i = T::vidx(lx, ly, k);
+			uint32_t o = 0;
+#pragma unroll
+			for (int d = 0; d < K; ++d) {
+				o |= rl[d][i] > 0 ? (1u << d) : 0u;
+			}
+			open[k] = o;
+		}
+		__syncthreads();  // all reads of the previous sweep's labels are done
+#pragma unroll
+		for (int k = 0; k < T::TZ; ++k) {
+			if (valid[k]) {
+				hl[T::pidx(lx, ly, k)] = rtr[k] > 0 ? 1 : kLabelInf;
+			}
+		}
+		__syncthreads();
+		relax_tile<T>(hl, open, valid, lx, ly);
+
+		// ---- active voxels and the top level
+		int top = 0;
+#pragma unroll
+		for (int k = 0; k < T::TZ; ++k) {
+			const int32_t lab = hl[T::pidx(lx, ly, k)];
+			if (valid[k] && el[T::vidx(lx, ly, k)] > 0 && lab < kLabelInf && lab > top) {
+				top = lab;
+			}
+		}
+		++step;
+		atomicMax(&key, ((unsigned long long)step << 32) | (unsigned)top);
+		__syncthreads();
+		int level = (int)(key & 0xffffffffull);
+		if (level == 0) {
+			break;
+		}
+		// ---- top-down push sweep over the levels present in the tile
+		while (level >= 1) {
+			int next = 0;
+#pragma unroll
+			for (int k = 0; k < T::TZ; ++k) {
+				if (!valid[k]) {
+					continue;
+				}
+				const int p = T::pidx(lx, ly, k);
+				const int i = T::vidx(lx, ly, k);
+				const int32_t lab = hl[p];
+				if (lab < level && lab > next) {
+					next = lab;
+				}
+				if (lab != level) {
+					continue;
+				}
+				int32_t ex = el[i];
+				if (ex <= 0) {
+					continue;
+				}
+				if (rtr[k] > 0) {
+					const int32_t f = ex < rtr[k] ? ex : rtr[k];
+					rtr[k] -= f;
+					ex -= f;
+					sink += f;
+				}
+#pragma unroll
+				for (int d = 0; d < K; ++d) {
+					if (ex > 0) {
+						const int32_t rv = rl[d][i];
+						if (rv > 0 && hl[p + T::poff(d)] < level) {
+							const int32_t f = ex < rv ? ex : rv;
+							rl[d][i] = rv - f;
+							ex -= f;
+							atomicOr(&dm[i], 1u << d);
+							const int nx = lx + D::dx(d), ny = ly + D::dy(d), nz = k + D::dz(d);
+							if (nx >= 0 && nx < T::TX && ny >= 0 && ny < T::TY && nz >= 0 && nz < T::TZ) {
+								const int j = i + T::voff(d);
+								rl[opposite<kConn>(d)][j] += f;
+								atomicOr(&dm[j], 1u << opposite<kConn>(d));
+								atomicAdd(&el[j], f);
+							}
+						}
+					}
+				}
+				el[i] = ex;
+			}
+			++step;
+			atomicMax(&key, ((unsigned long long)step << 32) | (unsigned)next);
+			__syncthreads();
+			level = (int)(key & 0xffffffffull);
+		}
+	}
+
+	// ---- write back (only what changed); pushes that left the tile go to the halo voxels
+	int active = 0;
+#pragma unroll
+	for (int k = 0; k < T::TZ; ++k) {
+		if (!valid[k]) {
+			continue;
+		}
+		const int p = T::pidx(lx, ly, k);
+		const int i = T::vidx(lx, ly, k);
+		const int64_t v = voxel_index(g, gx, gy, oz + k);
+		const int32_t ev = el[i], lab = hl[p];
+		active |= ev > 0 && lab < kLabelInf;
+		if (ev != e0[k]) {
+			g.e[v] = ev;
+		}
+		if (rtr[k] != rt0[k]) {
+			g.rt[v] = rtr[k];
+		}
+		if (lab != h0[k]) {
+			g.h[v] = lab;
+		}
+		const uint32_t dirty = dm[i];
+		if (dirty) {
+#pragma unroll
+			for (int d = 0; d < K; ++d) {
+				if (!(dirty & (1u << d))) {
+					continue;
+				}
+				const int32_t nr = rl[d][i];
+				const int slot = leaving_slot<T>(lx, ly, k, d);
+				if (slot >= 0) {
+					// only this voxel pushes along this edge during the phase: the drop is the flow sent
+					const int32_t old = g.r[(int64_t)d * g.nvox + v];
+					const int32_t f = old - nr;
+					if (f > 0) {
+						const int64_t u = voxel_index(g, gx + D::dx(d), gy + D::dy(d), oz + k + D::dz(d));
+						atomicAdd(&g.e[u], f);
+						g.r[(int64_t)opposite<kConn>(d) * g.nvox + u] += f;
+						marks[slot] = 1;
+					}
+				}
+				g.r[(int64_t)d * g.nvox + v] = nr;
+			}
+		}
+	}
+	const long long total = block_sum_i64<T::NT>(sink, red);
+	active = __syncthreads_or(active);
+	if (tid == 0) {
+		if (total) {
+			atomicAdd(g.flow, (unsigned long long)total);
+		}
+		tile_active[tile] = active ? 1 : 0;
+		atomicAdd(&g.counters[0], 1ull);
+	}
+	if (tid < 27 && marks[tid]) {
+		mark_neighbour_tile(g, tile_active, tx, ty, tz, tid);
+	}
+}
+
+// ---------------------------------------------------------------------------------------------
+// S_min closure pass: s(u) = 1 if u has excess or some v with s(v) = 1 has residual v -> u > 0.
+// s starts at 0 (a fill before the first pass) and bits only rise, so concurrently running tiles
+// may set bits in each other's voxels. first_pass seeds from the excess and records the
+// open-direction bits of every voxel.
+template <int kConn, int kTX, int kTY, int kTZ>
+__global__ __launch_bounds__(kTX *kTY) void k_grid_source_closure(GridView g, const int32_t *__restrict__ list,
+                                                                  const int32_t *__restrict__ count, int first_pass,
+                                                                  uint8_t *__restrict__ dirty_next)
+{
+	using T = Tile<kConn, kTX, kTY, kTZ>;
+	using D = typename T::D;
+	if ((int)blockIdx.x >= *count) {
+		return;
+	}
+	const int tile = list[blockIdx.x];
+	int tx, ty, tz;
+	tile_coords(g, tile, tx, ty, tz);
+	const int ox = tx * T::TX, oy = ty * T::TY, oz = tz * T::TZ;
+
+	__shared__ uint8_t sl[T::NV];
+	__shared__ uint8_t marks[27];
+	const int tid = threadIdx.x;
+	const int lx = tid % T::TX, ly = tid / T::TX;
+	const int gx = ox + lx, gy = oy + ly;
+	if (tid < 27) {
+		marks[tid] = 0;
+	}
+	bool valid[T::TZ];
+	uint32_t open[T::TZ];
+	uint8_t before[T::TZ];
+#pragma unroll
+	for (int k = 0; k < T::TZ; ++k) {
+		valid[k] = gx < g.nx && gy < g.ny && oz + k < g.nz;
+		const int i = T::vidx(lx, ly, k);
+		uint8_t sv = 0;
+		before[k] = 0;
+		uint32_t o = 0;
+		if (valid[k]) {
+			const int64_t v = voxel_index(g, gx, gy, oz + k);
+			const uint8_t stored = g.s[v];
+			before[k] = stored;
+			if (first_pass) {
+#pragma unroll
+				for (int d = 0; d < T::K; ++d) {
+					o |= g.r[(int64_t)d * g.nvox + v] > 0 ? (1u << d) : 0u;
+				}
+				g.omask[v] = o;
+				sv = (g.e[v] > 0 || stored) ? 1 : 0;
+			} else {
+				o = g.omask[v];
+				sv = stored;
+			}
+		}
+		open[k] = o;
+		sl[i] = sv;
+	}
+	__syncthreads();
+	for (;;) {
+		int changed = 0;
+#pragma unroll
+		for (int k = 0; k < T::TZ; ++k) {
+			const int i = T::vidx(lx, ly, k);
+			if (!valid[k] || !sl[i]) {
+				continue;
+			}
+#pragma unroll
+			for (int d = 0; d < T::K; ++d) {
+				if (!(open[k] & (1u << d))) {
+					continue;
+				}
+				const int nx = lx + D::dx(d), ny = ly + D::dy(d), nz = k + D::dz(d);
+				if (nx >= 0 && nx < T::TX && ny >= 0 && ny < T::TY && nz >= 0 && nz < T::TZ) {
+					const int j = i + T::voff(d);
+					if (!sl[j]) {
+						sl[j] = 1;
+						changed = 1;
+					}
+				}
+			}
+		}
+		if (!__syncthreads_or(changed)) {
+			break;
+		}
+	}
+	// write back own voxels; push the closure across the tile faces
+#pragma unroll
+	for (int k = 0; k < T::TZ; ++k) {
+		const int i = T::vidx(lx, ly, k);
+		if (!valid[k]) {
+			continue;
+		}
+		const int64_t v = voxel_index(g, gx, gy, oz + k);
+		if (sl[i] != before[k]) {
+			g.s[v] = sl[i];
+		}
+		if (!sl[i]) {
+			continue;
+		}
+#pragma unroll
+		for (int d = 0; d < T::K; ++d) {
+			if (!(open[k] & (1u << d))) {
+				continue;
+			}
+			const int slot = leaving_slot<T>(lx, ly, k, d);
+			if (slot < 0) {
+				continue;
+			}
+			const int64_t u = voxel_index(g, gx + D::dx(d), gy + D::dy(d), oz + k + D::dz(d));
+			if (!g.s[u]) {
+				g.s[u] = 1;
+				marks[slot] = 1;
+			}
+		}
+	}
+	__syncthreads();
+	if (tid < 27 && marks[tid]) {
+		mark_neighbour_tile(g, dirty_next, tx, ty, tz, tid);
+	}
+	if (tid == 0) {
+		atomicAdd(&g.counters[2], 1ull);
+	}
+}
+
+// ---------------------------------------------------------------------------------------------
+// Compact the indices of flagged tiles (optionally of one colour) with a wave64 ballot and one
+// atomic per wave. colour_mode 2: (tx+ty+tz) & 1; 8: (tx&1) | (ty&1)<<1 | (tz&1)<<2.
+__global__ __launch_bounds__(256) void k_compact_tiles(const uint8_t *__restrict__ flags, int ntiles, int ntx,
+                                                       int nty, int colour_mode, int colour,
+                                                       int32_t *__restrict__ list, int32_t *__restrict__ count)
+{
+	const int t = blockIdx.x * blockDim.x + threadIdx.x;
+	bool pred = false;
+	if (t < ntiles && flags[t]) {
+		pred = true;
+		if (colour >= 0) {
+			const int tx = t % ntx, ty = (t / ntx) % nty, tz = t / (ntx * nty);
+			const int c = colour_mode == 2 ? ((tx + ty + tz) & 1) : ((tx & 1) | ((ty & 1) << 1) | ((tz & 1) << 2));
+			pred = c == colour;
+		}
+	}
+	const unsigned long long ballot = __ballot(pred);
+	const int lane = threadIdx.x & 63;
+	int base = 0;
+	if (lane == 0 && ballot) {
+		base = atomicAdd(count, (int)__popcll(ballot));
+	}
+	base = __shfl(base, 0, 64);
+	if (pred) {
+		list[base + (int)__popcll(ballot & ((1ull << lane) - 1ull))] = t;
+	}
+}
+
+__global__ __launch_bounds__(256) void k_fill_u8(uint8_t *__restrict__ p, int64_t n, uint8_t value)
+{
+	const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+	for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+		p[i] = value;
+	}
+}
+
+__global__ __launch_bounds__(256) void k_fill_i32(int32_t *__restrict__ p, int64_t n, int32_t value)
+{
+	const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+	for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+		p[i] = value;
+	}
+}
+
+__global__ __launch_bounds__(256) void k_sink_side_mask(const int32_t *__restrict__ h, int64_t n,
+                                                        uint8_t *__restrict__ out, uint8_t mask, uint8_t background)
+{
+	const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+	for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+		out[i] = h[i] < kLabelInf ? mask : background;
+	}
+}
+
+}  // namespace cgp

@@ -1,0 +1,442 @@
+// Host driver of the 3D grid push-relabel: buffers, tile geometry and the
+// relabel / discharge / cut phase loop (MinCut::compute of the reference,
+// include/cugip/advanced_operations/detail/graph_cut_implementation.hpp:122-198).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "grid_kernels.h"
+#include "solver.h"
+
+namespace cgp {
+
+namespace {
+
+template <int kConn, int kTX, int kTY, int kTZ>
+struct ShapeTag {
+	static constexpr int conn = kConn, tx = kTX, ty = kTY, tz = kTZ;
+};
+
+// Tile shapes compiled into the library (connectivity, x, y, z).
+#define CGP_GRID_SHAPES(F) \
+	F(6, 32, 8, 4)         \
+	F(6, 32, 8, 8)         \
+	F(6, 64, 4, 4)         \
+	F(6, 32, 4, 4)         \
+	F(6, 64, 4, 8)         \
+	F(26, 32, 4, 4)        \
+	F(26, 16, 8, 4)        \
+	F(26, 16, 4, 4)
+
+template <typename Fn>
+void dispatch_shape(int conn, int tx, int ty, int tz, Fn &&fn)
+{
+#define CGP_TRY_SHAPE(C, X, Y, Z)                          \
+	if (conn == C && tx == X && ty == Y && tz == Z) {      \
+		fn(ShapeTag<C, X, Y, Z>{});                        \
+		return;                                            \
+	}
+	CGP_GRID_SHAPES(CGP_TRY_SHAPE)
+#undef CGP_TRY_SHAPE
+	fail(CGP_ERR_INVALID, "unsupported tile shape " + std::to_string(tx) + "x" + std::to_string(ty) + "x" +
+	                          std::to_string(tz) + " for " + std::to_string(conn) + "-connectivity");
+}
+
+bool shape_supported(int conn, int tx, int ty, int tz)
+{
+#define CGP_MATCH_SHAPE(C, X, Y, Z) \
+	if (conn == C && tx == X && ty == Y && tz == Z) return true;
+	CGP_GRID_SHAPES(CGP_MATCH_SHAPE)
+#undef CGP_MATCH_SHAPE
+	return false;
+}
+
+int blocks_for(int64_t n, int threads, int cap = 4096)
+{
+	int64_t b = (n + threads - 1) / threads;
+	if (b < 1) {
+		b = 1;
+	}
+	return (int)std::min<int64_t>(b, cap);
+}
+
+double ms_since(std::chrono::steady_clock::time_point t)
+{
+	return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
+}
+
+}  // namespace
+
+GridSolver::Shape GridSolver::pick_shape(const cgp_policy &policy) const
+{
+	if (policy.tile_x || policy.tile_y || policy.tile_z) {
+		if (!shape_supported(mConn, policy.tile_x, policy.tile_y, policy.tile_z)) {
+			fail(CGP_ERR_INVALID, "unsupported tile shape " + std::to_string(policy.tile_x) + "x" +
+			                          std::to_string(policy.tile_y) + "x" + std::to_string(policy.tile_z));
+		}
+		return {policy.tile_x, policy.tile_y, policy.tile_z};
+	}
+	return mConn == 6 ? Shape{32, 8, 4} : Shape{32, 4, 4};
+}
+
+void GridSolver::set_grid(int nx, int ny, int nz, int conn, const int32_t *cs, const int32_t *ct, const int32_t *cn,
+                          const int32_t *cnb, bool device_pointers)
+{
+	if (nx <= 0 || ny <= 0 || nz <= 0) {
+		fail(CGP_ERR_INVALID, "grid dimensions must be positive");
+	}
+	if (conn != 6 && conn != 26) {
+		fail(CGP_ERR_INVALID, "connectivity must be 6 or 26");
+	}
+	if (!cs || !ct || !cn) {
+		fail(CGP_ERR_INVALID, "capacity pointers must not be null");
+	}
+	const int64_t vox = (int64_t)nx * ny * nz;
+	if (vox >= (int64_t)1 << 31) {
+		fail(CGP_ERR_INVALID, "grids of 2^31 voxels or more are not supported");
+	}
+	const int half = conn / 2;
+	mSolved = false;
+	mNx = nx;
+	mNy = ny;
+	mNz = nz;
+	mConn = conn;
+	mVox = vox;
+	mHasBack = cnb != nullptr;
+	mCs.allocate(vox);
+	mCt.allocate(vox);
+	mCn.allocate((size_t)vox * half);
+	if (mHasBack) {
+		mCnb.allocate((size_t)vox * half);
+	} else {
+		mCnb.release();
+	}
+	const hipMemcpyKind kind = device_pointers ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+	CGP_HIP(hipMemcpyAsync(mCs.get(), cs, mCs.bytes(), kind, mStream));
+	CGP_HIP(hipMemcpyAsync(mCt.get(), ct, mCt.bytes(), kind, mStream));
+	CGP_HIP(hipMemcpyAsync(mCn.get(), cn, mCn.bytes(), kind, mStream));
+	if (mHasBack) {
+		CGP_HIP(hipMemcpyAsync(mCnb.get(), cnb, mCnb.bytes(), kind, mStream));
+	}
+	mE.allocate(vox);
+	mRt.allocate(vox);
+	mH.allocate(vox);
+	mR.allocate((size_t)vox * conn);
+	mS.allocate(vox);
+	mOmask.allocate(vox);
+	mFlow.allocate(1);
+	mCounters.allocate(4);
+	mStatus.allocate(1);
+	mCount.allocate(1);
+	mHostScratch.allocate(8);
+	CGP_HIP(hipStreamSynchronize(mStream));
+}
+
+void GridSolver::configure_tiles(const Shape &shape)
+{
+	mShape = shape;
+	mNtx = (mNx + shape.tx - 1) / shape.tx;
+	mNty = (mNy + shape.ty - 1) / shape.ty;
+	mNtz = (mNz + shape.tz - 1) / shape.tz;
+	mTiles = mNtx * mNty * mNtz;
+	mActive.allocate(mTiles);
+	mDirtyA.allocate(mTiles);
+	mDirtyB.allocate(mTiles);
+	mList.allocate(mTiles);
+	mAllTiles.allocate(mTiles + 1);
+	std::vector<int32_t> all(mTiles + 1);
+	for (int t = 0; t < mTiles; ++t) {
+		all[t] = t;
+	}
+	all[mTiles] = mTiles;  // count slot for the all-tiles list
+	// largest colour class: the grid of a discharge phase must cover every tile of one colour
+	const int ncolours = mConn == 6 ? 2 : 8;
+	std::vector<int> per_colour(ncolours, 0);
+	for (int t = 0; t < mTiles; ++t) {
+		const int tx = t % mNtx, ty = (t / mNtx) % mNty, tz = t / (mNtx * mNty);
+		const int c = mConn == 6 ? ((tx + ty + tz) & 1) : ((tx & 1) | ((ty & 1) << 1) | ((tz & 1) << 2));
+		++per_colour[c];
+	}
+	mMaxColourTiles = *std::max_element(per_colour.begin(), per_colour.end());
+	CGP_HIP(hipMemcpyAsync(mAllTiles.get(), all.data(), all.size() * sizeof(int32_t), hipMemcpyHostToDevice, mStream));
+	CGP_HIP(hipStreamSynchronize(mStream));
+}
+
+static GridView make_view(int nx, int ny, int nz, int ntx, int nty, int ntz, int64_t vox, DeviceBuffer<int32_t> &e,
+                          DeviceBuffer<int32_t> &rt, DeviceBuffer<int32_t> &h, DeviceBuffer<int32_t> &r,
+                          DeviceBuffer<uint8_t> &s, DeviceBuffer<uint32_t> &omask, DeviceBuffer<int32_t> &cs,
+                          DeviceBuffer<int32_t> &ct, DeviceBuffer<int32_t> &cn, DeviceBuffer<int32_t> &cnb,
+                          DeviceBuffer<unsigned long long> &flow, DeviceBuffer<int32_t> &status,
+                          DeviceBuffer<unsigned long long> &counters)
+{
+	GridView g;
+	g.nx = nx;
+	g.ny = ny;
+	g.nz = nz;
+	g.ntx = ntx;
+	g.nty = nty;
+	g.ntz = ntz;
+	g.nvox = vox;
+	g.e = e.get();
+	g.rt = rt.get();
+	g.h = h.get();
+	g.r = r.get();
+	g.s = s.get();
+	g.omask = omask.get();
+	g.cs = cs.get();
+	g.ct = ct.get();
+	g.cn = cn.get();
+	g.cnb = cnb.get();
+	g.flow = flow.get();
+	g.status = status.get();
+	g.counters = counters.get();
+	return g;
+}
+
+#define CGP_VIEW()                                                                                                 \
+	make_view(mNx, mNy, mNz, mNtx, mNty, mNtz, mVox, mE, mRt, mH, mR, mS, mOmask, mCs, mCt, mCn, mCnb, mFlow, mStatus, \
+	          mCounters)
+
+void GridSolver::launch_init()
+{
+	GridView g = CGP_VIEW();
+	const int blocks = blocks_for(mVox, 256, 8192);
+	if (mConn == 6) {
+		hipLaunchKernelGGL(k_grid_init<6>, dim3(blocks), dim3(256), 0, mStream, g);
+	} else {
+		hipLaunchKernelGGL(k_grid_init<26>, dim3(blocks), dim3(256), 0, mStream, g);
+	}
+	CGP_LAUNCH_CHECK();
+}
+
+void GridSolver::launch_compact(const uint8_t *flags, int colour)
+{
+	hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, mStream, mCount.get(), (int64_t)1, 0);
+	const int colour_mode = mConn == 6 ? 2 : 8;
+	hipLaunchKernelGGL(k_compact_tiles, dim3(blocks_for(mTiles, 256, 1 << 20)), dim3(256), 0, mStream, flags, mTiles,
+	                   mNtx, mNty, colour_mode, colour, mList.get(), mCount.get());
+	CGP_LAUNCH_CHECK();
+}
+
+int GridSolver::count_flags(const uint8_t *flags, int colour)
+{
+	launch_compact(flags, colour);
+	int32_t *pinned = reinterpret_cast<int32_t *>(mHostScratch.get());
+	CGP_HIP(hipMemcpyAsync(pinned, mCount.get(), sizeof(int32_t), hipMemcpyDeviceToHost, mStream));
+	CGP_HIP(hipStreamSynchronize(mStream));
+	return pinned[0];
+}
+
+int GridSolver::run_global_relabel(int32_t threshold, cgp_stats &stats)
+{
+	GridView g = CGP_VIEW();
+	hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(mVox, 256, 8192)), dim3(256), 0, mStream, mH.get(), mVox, kLabelInf);
+	const int32_t *list = mAllTiles.get();
+	const int32_t *count = mAllTiles.get() + mTiles;
+	int blocks = mTiles;
+	for (;;) {
+		hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(mTiles, 256)), dim3(256), 0, mStream, mDirtyA.get(), (int64_t)mTiles,
+		                   (uint8_t)0);
+		dispatch_shape(mConn, mShape.tx, mShape.ty, mShape.tz, [&](auto tag) {
+			using S = decltype(tag);
+			hipLaunchKernelGGL((k_grid_bfs<S::conn, S::tx, S::ty, S::tz>), dim3(blocks), dim3(S::tx * S::ty), 0, mStream,
+			                   g, list, count, threshold, mDirtyA.get(), mActive.get());
+		});
+		CGP_LAUNCH_CHECK();
+		++stats.bfs_passes;
+		const int dirty = count_flags(mDirtyA.get(), -1);
+		if (dirty == 0) {
+			break;
+		}
+		list = mList.get();
+		count = mCount.get();
+		blocks = dirty;
+	}
+	return count_flags(mActive.get(), -1);
+}
+
+void GridSolver::run_discharge_phase(int colour, int max_iters)
+{
+	GridView g = CGP_VIEW();
+	launch_compact(mActive.get(), colour);
+	// every tile of one colour; blocks beyond the compacted count exit at once
+	const int blocks = mMaxColourTiles;
+	dispatch_shape(mConn, mShape.tx, mShape.ty, mShape.tz, [&](auto tag) {
+		using S = decltype(tag);
+		hipLaunchKernelGGL((k_grid_discharge<S::conn, S::tx, S::ty, S::tz>), dim3(blocks), dim3(S::tx * S::ty), 0,
+		                   mStream, g, mList.get(), mCount.get(), max_iters, mActive.get());
+	});
+	CGP_LAUNCH_CHECK();
+}
+
+void GridSolver::run_source_closure(cgp_stats &stats)
+{
+	GridView g = CGP_VIEW();
+	hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(mVox, 256, 8192)), dim3(256), 0, mStream, mS.get(), mVox, (uint8_t)0);
+	const int32_t *list = mAllTiles.get();
+	const int32_t *count = mAllTiles.get() + mTiles;
+	int blocks = mTiles;
+	int first = 1;
+	for (;;) {
+		hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(mTiles, 256)), dim3(256), 0, mStream, mDirtyB.get(), (int64_t)mTiles,
+		                   (uint8_t)0);
+		dispatch_shape(mConn, mShape.tx, mShape.ty, mShape.tz, [&](auto tag) {
+			using S = decltype(tag);
+			hipLaunchKernelGGL((k_grid_source_closure<S::conn, S::tx, S::ty, S::tz>), dim3(blocks), dim3(S::tx * S::ty), 0,
+			                   mStream, g, list, count, first, mDirtyB.get());
+		});
+		CGP_LAUNCH_CHECK();
+		++stats.cut_passes;
+		first = 0;
+		const int dirty = count_flags(mDirtyB.get(), -1);
+		if (dirty == 0) {
+			break;
+		}
+		list = mList.get();
+		count = mCount.get();
+		blocks = dirty;
+	}
+}
+
+int64_t GridSolver::read_flow()
+{
+	unsigned long long *pinned = reinterpret_cast<unsigned long long *>(mHostScratch.get());
+	CGP_HIP(hipMemcpyAsync(pinned, mFlow.get(), sizeof(unsigned long long), hipMemcpyDeviceToHost, mStream));
+	CGP_HIP(hipStreamSynchronize(mStream));
+	return (int64_t)pinned[0];
+}
+
+cgp_policy default_policy()
+{
+	cgp_policy p;
+	std::memset(&p, 0, sizeof(p));
+	return p;
+}
+
+int64_t GridSolver::solve(const cgp_policy &policy, cgp_stats *stats_out, const TraceHook &trace)
+{
+	if (mVox == 0) {
+		fail(CGP_ERR_STATE, "no grid set");
+	}
+	cgp_stats stats;
+	std::memset(&stats, 0, sizeof(stats));
+	const auto t_start = std::chrono::steady_clock::now();
+	mSolved = false;
+	configure_tiles(pick_shape(policy));
+	const int ncolours = mConn == 6 ? 2 : 8;
+	const int phases = policy.discharges_per_relabel > 0 ? policy.discharges_per_relabel : 2 * ncolours;
+	const int max_iters = policy.max_tile_iterations > 0 ? policy.max_tile_iterations : 16;
+	const int max_rounds = policy.max_rounds > 0 ? policy.max_rounds : 1 << 20;
+	int32_t threshold = policy.minimal_residual > 0.0f ? (int32_t)std::floor(policy.minimal_residual) : 0;
+	if (policy.minimal_residual < 0.0f || !std::isfinite(policy.minimal_residual)) {
+		fail(CGP_ERR_INVALID, "minimal_residual must be finite and non-negative");
+	}
+
+	hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, mStream, mStatus.get(), (int64_t)1, 0);
+	hipLaunchKernelGGL(k_fill_u8, dim3(1), dim3(64), 0, mStream, reinterpret_cast<uint8_t *>(mFlow.get()), (int64_t)8,
+	                   (uint8_t)0);
+	hipLaunchKernelGGL(k_fill_u8, dim3(1), dim3(64), 0, mStream, reinterpret_cast<uint8_t *>(mCounters.get()),
+	                   (int64_t)32, (uint8_t)0);
+	launch_init();
+	{
+		int32_t *pinned = reinterpret_cast<int32_t *>(mHostScratch.get());
+		CGP_HIP(hipMemcpyAsync(pinned, mStatus.get(), sizeof(int32_t), hipMemcpyDeviceToHost, mStream));
+		CGP_HIP(hipStreamSynchronize(mStream));
+		if (pinned[0]) {
+			fail(CGP_ERR_OVERFLOW, "capacities must be non-negative and each voxel's source capacity plus incoming "
+			                       "capacities, and each edge's two capacities, must sum below 2^31");
+		}
+	}
+	stats.init_ms = ms_since(t_start);
+	trace.emit(CGP_TRACE_STARTED, 0, 0, 0);
+
+	int round = 0;
+	for (;;) {
+		trace.emit(CGP_TRACE_BEGIN_ITERATION, round, 0, 0);
+		auto t0 = std::chrono::steady_clock::now();
+		const int active = run_global_relabel(threshold, stats);
+		stats.relabel_ms += ms_since(t0);
+		++stats.rounds;
+		trace.emit(CGP_TRACE_AFTER_RELABEL, round, 0, 0);
+		if (active == 0) {
+			if (threshold > 0) {
+				threshold = 0;  // exact finish
+				continue;
+			}
+			break;
+		}
+		if (round >= max_rounds) {
+			fail(CGP_ERR_NOT_CONVERGED, "max_rounds (" + std::to_string(max_rounds) + ") reached before convergence");
+		}
+		t0 = std::chrono::steady_clock::now();
+		for (int p = 0; p < phases; ++p) {
+			run_discharge_phase(p % ncolours, max_iters);
+			++stats.discharge_phases;
+		}
+		if (trace.fn) {
+			trace.emit(CGP_TRACE_AFTER_PUSH, round, 0, read_flow());
+		}
+		stats.discharge_ms += ms_since(t0);
+		++round;
+	}
+	auto t0 = std::chrono::steady_clock::now();
+	run_source_closure(stats);
+	const int64_t flow = read_flow();
+	stats.cut_ms = ms_since(t0);
+	{
+		unsigned long long *pinned = reinterpret_cast<unsigned long long *>(mHostScratch.get());
+		CGP_HIP(hipMemcpyAsync(pinned, mCounters.get(), 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, mStream));
+		CGP_HIP(hipStreamSynchronize(mStream));
+		stats.discharge_tile_visits = (int64_t)pinned[0];
+		stats.bfs_tile_visits = (int64_t)pinned[1];
+		stats.cut_tile_visits = (int64_t)pinned[2];
+	}
+	stats.flow = flow;
+	stats.total_ms = ms_since(t_start);
+	stats.voxels = mVox;
+	stats.tile_x = mShape.tx;
+	stats.tile_y = mShape.ty;
+	stats.tile_z = mShape.tz;
+	stats.conn = mConn;
+	mSolved = true;
+	trace.emit(CGP_TRACE_AFTER_PUSH, round, 1, flow);
+	trace.emit(CGP_TRACE_FINISHED, round, 1, flow);
+	if (stats_out) {
+		*stats_out = stats;
+	}
+	return flow;
+}
+
+void GridSolver::sink_side(uint8_t *host_out, uint8_t mask, uint8_t background)
+{
+	if (!mSolved) {
+		fail(CGP_ERR_STATE, "fill_segments before max_flow");
+	}
+	DeviceBuffer<uint8_t> tmp(mVox);
+	hipLaunchKernelGGL(k_sink_side_mask, dim3(blocks_for(mVox, 256, 8192)), dim3(256), 0, mStream, mH.get(), mVox,
+	                   tmp.get(), mask, background);
+	CGP_LAUNCH_CHECK();
+	CGP_HIP(hipMemcpyAsync(host_out, tmp.get(), mVox, hipMemcpyDeviceToHost, mStream));
+	CGP_HIP(hipStreamSynchronize(mStream));
+}
+
+void GridSolver::source_side(uint8_t *host_out)
+{
+	if (!mSolved) {
+		fail(CGP_ERR_STATE, "source_set before max_flow");
+	}
+	CGP_HIP(hipMemcpyAsync(host_out, mS.get(), mVox, hipMemcpyDeviceToHost, mStream));
+	CGP_HIP(hipStreamSynchronize(mStream));
+}
+
+void GridSolver::labels(int32_t *host_out)
+{
+	if (!mSolved) {
+		fail(CGP_ERR_STATE, "labels before max_flow");
+	}
+	CGP_HIP(hipMemcpyAsync(host_out, mH.get(), mVox * sizeof(int32_t), hipMemcpyDeviceToHost, mStream));
+	CGP_HIP(hipStreamSynchronize(mStream));
+}
+
+}  // namespace cgp

@@ -1,0 +1,100 @@
+// Compile-time geometry of the 3D grid path: neighbourhood directions and the
+// LDS tile (own voxels plus a one-voxel halo) that every grid kernel stages.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace cgp {
+
+// Directed residual directions. d < H: the negative half neighbourhood in the
+// reference's table order (VonNeumannNeighborhood<3>::offset(1..3),
+// MooreNeighborhood<3>::offset(1..13), cugip/neighborhood.hpp); d >= H: mirror.
+template <int kConn>
+struct Dirs;
+
+template <>
+struct Dirs<6> {
+	static constexpr int K = 6;
+	static constexpr int H = 3;
+	__host__ __device__ static constexpr int dx(int d) { return d == 0 ? -1 : (d == 3 ? 1 : 0); }
+	__host__ __device__ static constexpr int dy(int d) { return d == 1 ? -1 : (d == 4 ? 1 : 0); }
+	__host__ __device__ static constexpr int dz(int d) { return d == 2 ? -1 : (d == 5 ? 1 : 0); }
+};
+
+template <>
+struct Dirs<26> {
+	static constexpr int K = 26;
+	static constexpr int H = 13;
+	__host__ __device__ static constexpr int hx(int h) { return h % 3 - 1; }
+	__host__ __device__ static constexpr int hy(int h) { return (h / 3) % 3 - 1; }
+	__host__ __device__ static constexpr int hz(int h) { return h / 9 - 1; }
+	__host__ __device__ static constexpr int dx(int d) { return d < H ? hx(d) : -hx(d - H); }
+	__host__ __device__ static constexpr int dy(int d) { return d < H ? hy(d) : -hy(d - H); }
+	__host__ __device__ static constexpr int dz(int d) { return d < H ? hz(d) : -hz(d - H); }
+};
+
+template <int kConn>
+__host__ __device__ constexpr int opposite(int d)
+{
+	return d < Dirs<kConn>::H ? d + Dirs<kConn>::H : d - Dirs<kConn>::H;
+}
+
+// Tile of TX x TY x TZ voxels processed by one workgroup of TX*TY threads; thread
+// (lx, ly) owns the z-column lz = 0..TZ-1. Labels are staged with a halo of one
+// voxel on every side (PX x PY x PZ).
+template <int kConn, int kTX, int kTY, int kTZ>
+struct Tile {
+	using D = Dirs<kConn>;
+	static constexpr int K = D::K;
+	static constexpr int TX = kTX, TY = kTY, TZ = kTZ;
+	static constexpr int NT = TX * TY;
+	static constexpr int NV = TX * TY * TZ;
+	static constexpr int PX = TX + 2, PY = TY + 2, PZ = TZ + 2;
+	static constexpr int NP = PX * PY * PZ;
+	static_assert(NT % 64 == 0, "a tile must map to whole wave64s");
+
+	__host__ __device__ static constexpr int poff(int d) { return (D::dz(d) * PY + D::dy(d)) * PX + D::dx(d); }
+	__host__ __device__ static constexpr int voff(int d) { return (D::dz(d) * TY + D::dy(d)) * TX + D::dx(d); }
+	__host__ __device__ static constexpr int pidx(int x, int y, int z) { return ((z + 1) * PY + (y + 1)) * PX + (x + 1); }
+	__host__ __device__ static constexpr int vidx(int x, int y, int z) { return (z * TY + y) * TX + x; }
+};
+
+// Everything a grid kernel needs, passed by value.
+struct GridView {
+	int32_t nx, ny, nz;
+	int32_t ntx, nty, ntz;   // tiles per axis
+	int64_t nvox;            // voxels = stride between residual planes
+	int32_t *e;              // excess
+	int32_t *rt;             // residual sink t-link
+	int32_t *h;              // label: BFS distance to t, kLabelInf if t is unreachable
+	int32_t *r;              // K residual planes, r[d * nvox + v] = residual v -> v + o_d
+	uint8_t *s;              // S_min byte mask
+	uint32_t *omask;         // open-direction bits (r > 0) used by the S_min closure
+	const int32_t *cs, *ct;  // input t-link capacities
+	const int32_t *cn, *cnb; // input n-link capacities, H planes (cnb may be null)
+	unsigned long long *flow;
+	int32_t *status;         // nonzero: input error detected on the device
+	unsigned long long *counters; // [0] discharge visits, [1] bfs visits, [2] cut visits
+};
+
+__device__ inline int64_t voxel_index(const GridView &g, int x, int y, int z)
+{
+	return ((int64_t)z * g.ny + y) * g.nx + x;
+}
+
+__device__ inline bool inside(const GridView &g, int x, int y, int z)
+{
+	return x >= 0 && x < g.nx && y >= 0 && y < g.ny && z >= 0 && z < g.nz;
+}
+
+__device__ inline void tile_coords(const GridView &g, int tile, int &tx, int &ty, int &tz)
+{
+	tx = tile % g.ntx;
+	const int rest = tile / g.ntx;
+	ty = rest % g.nty;
+	tz = rest / g.nty;
+}
+
+}  // namespace cgp

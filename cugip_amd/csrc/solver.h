@@ -1,0 +1,108 @@
+// Host-side solver classes behind the C ABI.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "common.h"
+
+namespace cgp {
+
+struct TraceHook {
+	cgp_trace_fn fn = nullptr;
+	void *user = nullptr;
+	void emit(int event, int iteration, int done, int64_t flow, const int32_t *levels = nullptr, int count = 0) const
+	{
+		if (fn) {
+			fn(user, event, iteration, done, flow, levels, count);
+		}
+	}
+};
+
+cgp_policy default_policy();
+
+// 3D grid graph (6- or 26-connected), implicit edges, SoA int32 state in HBM.
+class GridSolver {
+public:
+	explicit GridSolver(hipStream_t stream) : mStream(stream) {}
+
+	void set_grid(int nx, int ny, int nz, int conn, const int32_t *cs, const int32_t *ct, const int32_t *cn,
+	              const int32_t *cnb, bool device_pointers);
+	int64_t solve(const cgp_policy &policy, cgp_stats *stats, const TraceHook &trace);
+	void sink_side(uint8_t *host_out, uint8_t mask, uint8_t background);
+	void source_side(uint8_t *host_out);
+	void labels(int32_t *host_out);
+	int64_t vertex_count() const { return mVox; }
+	bool solved() const { return mSolved; }
+
+private:
+	struct Shape {
+		int tx, ty, tz;
+	};
+	Shape pick_shape(const cgp_policy &policy) const;
+	void configure_tiles(const Shape &shape);
+	void launch_init();
+	int run_global_relabel(int32_t threshold, cgp_stats &stats);
+	int count_flags(const uint8_t *flags, int colour);
+	void launch_compact(const uint8_t *flags, int colour);
+	void run_discharge_phase(int colour, int max_iters);
+	void run_source_closure(cgp_stats &stats);
+	int64_t read_flow();
+
+	hipStream_t mStream;
+	int mNx = 0, mNy = 0, mNz = 0, mConn = 0;
+	int64_t mVox = 0;
+	Shape mShape{0, 0, 0};
+	int mNtx = 0, mNty = 0, mNtz = 0, mTiles = 0, mMaxColourTiles = 0;
+	bool mSolved = false;
+	bool mHasBack = false;
+
+	DeviceBuffer<int32_t> mCs, mCt, mCn, mCnb;
+	DeviceBuffer<int32_t> mE, mRt, mH, mR;
+	DeviceBuffer<uint8_t> mS;
+	DeviceBuffer<uint32_t> mOmask;
+	DeviceBuffer<unsigned long long> mFlow, mCounters;
+	DeviceBuffer<int32_t> mStatus, mCount, mList, mAllTiles;
+	DeviceBuffer<uint8_t> mActive, mDirtyA, mDirtyB;
+	PinnedBuffer<int64_t> mHostScratch;
+};
+
+// Generic graph (CSR), the data layout of the reference's Graph<TFlow>.
+class CsrSolver {
+public:
+	explicit CsrSolver(hipStream_t stream) : mStream(stream) {}
+
+	void set_vertex_count(int n);
+	void set_nweights(int m, const cgp_edge *edges, const int32_t *fwd, const int32_t *bwd);
+	void set_tweights(const int32_t *source, const int32_t *sink);
+	int64_t solve(const cgp_policy &policy, cgp_stats *stats, const TraceHook &trace);
+	void sink_side(uint8_t *host_out, uint8_t mask, uint8_t background);
+	void source_side(uint8_t *host_out);
+	void labels(int32_t *host_out);
+	int64_t vertex_count() const { return mN; }
+	bool solved() const { return mSolved; }
+	bool ready() const { return mN > 0 && mHaveTweights; }
+
+private:
+	int bfs(int32_t threshold, std::vector<int32_t> &level_starts, cgp_stats &stats);
+	bool push_sweep(const std::vector<int32_t> &level_starts, cgp_stats &stats);
+	void source_closure(cgp_stats &stats);
+	int32_t read_i32(const int32_t *device_ptr);
+
+	hipStream_t mStream;
+	int mN = 0;
+	int64_t mArcs = 0;
+	bool mSolved = false;
+	bool mHaveTweights = false;
+	std::vector<int32_t> mHostFirst;   // n + 1
+	std::vector<int64_t> mIncoming;    // sum of capacities into each vertex
+	std::vector<int32_t> mHostSource;  // source t-links (headroom check)
+	DeviceBuffer<int32_t> mFirst, mHead, mMate, mArcCap, mRes;
+	DeviceBuffer<int32_t> mCs, mCt, mE, mRt, mH, mQueue;
+	DeviceBuffer<int32_t> mS;          // S_min marks
+	DeviceBuffer<int32_t> mTail, mPushed, mStatus, mLevelStarts;
+	DeviceBuffer<unsigned long long> mFlow;
+	PinnedBuffer<int32_t> mPinned;
+};
+
+}  // namespace cgp

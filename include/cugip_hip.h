@@ -1,0 +1,154 @@
+/*
+ * cugip_hip.h -- C ABI of the MI355X (gfx950) push-relabel graph cut.
+ *
+ * This is the boundary between host code and the HIP implementation. The
+ * header-only C++ drop-in (include/cugip/graph_cut.hpp, cugip::Graph<TFlow>)
+ * forwards every call to these functions; any other host language binds them
+ * directly (plain pointers and sizes, no C++ or torch types).
+ *
+ * Semantics (all capacities are non-negative int32, flows are exact int64):
+ *   - A graph has n vertices plus the terminals s and t. Each vertex v has a
+ *     source t-link s->v and a sink t-link v->t. Each undirected edge record
+ *     (first, second), first < second, carries a capacity first->second (fwd)
+ *     and second->first (bwd).
+ *   - cgp_max_flow computes the maximum s-t flow value exactly.
+ *   - cgp_fill_segments writes the reference's segmentation: `mask` for every
+ *     vertex that can still reach t in the final residual graph (T_min, the
+ *     vertices whose last BFS label is finite), `background` elsewhere.
+ *   - cgp_source_set writes the canonical minimum cut: 1 for every vertex
+ *     reachable from s in the residual graph of a maximum flow (S_min), else 0.
+ *     S_min and T_min do not depend on which maximum flow was found.
+ *
+ * Every function returns CGP_OK (0) or a CGP_ERR_* code; cgp_last_error gives
+ * the message. One handle is used by one host thread at a time; different
+ * handles are independent (each owns a HIP stream and its device buffers).
+ */
+#ifndef CUGIP_HIP_H
+#define CUGIP_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CGP_OK 0
+#define CGP_ERR_INVALID 1       /* bad argument, dimension or call order */
+#define CGP_ERR_NOMEM 2         /* device or host allocation failed */
+#define CGP_ERR_NOT_CONVERGED 3 /* round cap reached; no result is returned */
+#define CGP_ERR_OVERFLOW 4      /* a capacity sum does not fit the int32 state */
+#define CGP_ERR_DEVICE 5        /* HIP runtime error */
+#define CGP_ERR_STATE 6         /* result requested before cgp_max_flow */
+
+typedef struct cgp_graph_s *cgp_graph;
+
+/* Layout-compatible with cugip::EdgeRecord (advanced_operations/detail/edge_record.hpp:10-29):
+   `second` at byte offset 0, `first` at byte offset 4, first = min(endpoints). */
+typedef struct cgp_edge {
+	uint32_t second;
+	uint32_t first;
+} cgp_edge;
+
+/* Run-time solver policy. Zero fields take the library default.
+   Counterpart of cugip::GraphCutPolicy (advanced_operations/detail/graph_cut_policies.hpp:23-89). */
+typedef struct cgp_policy {
+	int32_t tile_x, tile_y, tile_z;       /* grid path: tile shape (only supported shapes are accepted) */
+	int32_t discharges_per_relabel;       /* grid path: discharge phases between two global relabels */
+	int32_t max_tile_iterations;          /* grid path: region relabel+push rounds per tile visit */
+	int32_t max_rounds;                   /* global relabels before CGP_ERR_NOT_CONVERGED */
+	float minimal_residual;               /* EdgeReverseTraversable::minimalResidual (edge_record.hpp:60-72);
+	                                         > 0 closes thin edges during the relabel rounds, and the solve
+	                                         then finishes with exact (threshold 0) rounds, so the result
+	                                         stays exact */
+	int32_t reserved[8];
+} cgp_policy;
+
+/* Per-solve counters and device timings. */
+typedef struct cgp_stats {
+	int64_t flow;              /* max-flow value */
+	int32_t rounds;            /* global relabels (BFS from the sink) */
+	int32_t bfs_passes;        /* tile passes (grid) or BFS levels (generic) over all rounds */
+	int32_t discharge_phases;  /* grid: colour phases; generic: push levels */
+	int32_t cut_passes;        /* passes of the S_min closure */
+	int64_t bfs_tile_visits;   /* grid: tiles processed by the global relabel */
+	int64_t discharge_tile_visits;
+	int64_t cut_tile_visits;
+	double init_ms, relabel_ms, discharge_ms, cut_ms, total_ms; /* host wall time per phase */
+	int64_t voxels;            /* vertices (grid: voxels) */
+	int32_t tile_x, tile_y, tile_z, conn;
+	int32_t reserved[8];
+} cgp_stats;
+
+/* Trace hooks of the reference's trace-object concept (advanced_operations/graph_cut.hpp:26-45). */
+#define CGP_TRACE_STARTED 0
+#define CGP_TRACE_BEGIN_ITERATION 1
+#define CGP_TRACE_AFTER_RELABEL 2
+#define CGP_TRACE_AFTER_PUSH 3
+#define CGP_TRACE_FINISHED 4
+/* level_starts: cumulative vertex counts per BFS level (levelStarts of the reference), valid
+   for CGP_TRACE_AFTER_RELABEL only; flow: flow into t so far. */
+typedef void (*cgp_trace_fn)(void *user, int32_t event, int32_t iteration, int32_t done, int64_t flow,
+                             const int32_t *level_starts, int32_t level_count);
+
+/* ---- handle ------------------------------------------------------------------------- */
+int cgp_create(cgp_graph *out);
+int cgp_destroy(cgp_graph graph);
+/* Message of the last failed call on `graph`, or of the last failed cgp_create when graph is NULL. */
+const char *cgp_last_error(cgp_graph graph);
+
+/* ---- generic graphs: cugip::Graph<TFlow> (advanced_operations/graph_cut.hpp:47-146) ---- */
+/* replaces Graph<TFlow>::set_vertex_count, graph_cut.hpp:213-237 */
+int cgp_set_vertex_count(cgp_graph graph, int32_t vertex_count);
+/* replaces Graph<TFlow>::set_nweights, graph_cut.hpp:239-290 (fwd = first->second, bwd = second->first) */
+int cgp_set_nweights(cgp_graph graph, int32_t edge_count, const cgp_edge *edges, const int32_t *fwd,
+                     const int32_t *bwd);
+/* replaces Graph<TFlow>::set_tweights, graph_cut.hpp:292-302 */
+int cgp_set_tweights(cgp_graph graph, const int32_t *source, const int32_t *sink);
+
+/* ---- 3D grid graphs (implicit edges, no CSR) --------------------------------------------
+   Voxel v = (z*ny + y)*nx + x. conn is 6 or 26. cap_n holds conn/2 planes of nx*ny*nz int32:
+   plane j is the capacity v -> v + o_j where o_j is the j-th offset of the negative half
+   neighbourhood (VonNeumannNeighborhood<3>::offset(1+j) for 6, MooreNeighborhood<3>::offset(1+j)
+   for 26; cugip/neighborhood.hpp). cap_n_back (may be NULL = symmetric) holds v + o_j -> v.
+   Entries whose neighbour falls outside the volume are ignored. This is the grid counterpart of
+   the caller-side construction in benchmarks/graph_cut/graph_cut.cpp:32-98. */
+int cgp_set_grid(cgp_graph graph, int32_t nx, int32_t ny, int32_t nz, int32_t conn, const int32_t *cap_source,
+                 const int32_t *cap_sink, const int32_t *cap_n, const int32_t *cap_n_back);
+/* Same, with all pointers in device memory of the handle's device (copied device to device). */
+int cgp_set_grid_device(cgp_graph graph, int32_t nx, int32_t ny, int32_t nz, int32_t conn,
+                        const int32_t *cap_source, const int32_t *cap_sink, const int32_t *cap_n,
+                        const int32_t *cap_n_back);
+
+/* ---- solving ----------------------------------------------------------------------------- */
+int cgp_set_policy(cgp_graph graph, const cgp_policy *policy);   /* NULL restores the defaults */
+int cgp_get_policy(cgp_graph graph, cgp_policy *policy);
+int cgp_set_trace(cgp_graph graph, cgp_trace_fn fn, void *user); /* fn NULL disables tracing */
+/* replaces Graph<TFlow>::max_flow / max_flow_with_tracing, graph_cut.hpp:69-100 */
+int cgp_max_flow(cgp_graph graph, int64_t *flow, cgp_stats *stats);
+/* replaces Graph<TFlow>::fill_segments, graph_cut.hpp:318-333 (T_min; host buffer of n bytes) */
+int cgp_fill_segments(cgp_graph graph, uint8_t *out, uint8_t mask, uint8_t background);
+/* canonical minimum cut S_min as 0/1 bytes (host buffer of n bytes) */
+int cgp_source_set(cgp_graph graph, uint8_t *out);
+/* final vertex labels (BFS distance to t, or CGP_LABEL_UNREACHABLE); debug view of
+   Graph<TFlow>::mLabels, graph_cut.hpp:135 */
+#define CGP_LABEL_UNREACHABLE 0x3fffffff
+int cgp_labels(cgp_graph graph, int32_t *out);
+
+/* ---- synthetic instances ------------------------------------------------------------------
+   Blob volume (3-8 ellipsoids at 180 on 60, N(0, 25^2) noise, clamped to 0-255) and the int32
+   capacities derived from it (Boykov-Jolly t-links, contrast-sensitive n-links). Integer-only:
+   the same arguments give the same bytes everywhere. */
+int cgp_generate_volume(int32_t nx, int32_t ny, int32_t nz, uint64_t seed, uint64_t instance, uint8_t *out);
+int cgp_caps_from_volume(int32_t nx, int32_t ny, int32_t nz, int32_t conn, const uint8_t *volume,
+                         int32_t *cap_source, int32_t *cap_sink, int32_t *cap_n);
+
+/* ---- device ------------------------------------------------------------------------------- */
+int cgp_device_count(int32_t *count);
+/* Version of this ABI; bumps on any incompatible change. */
+int32_t cgp_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CUGIP_HIP_H */
