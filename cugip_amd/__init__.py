@@ -69,6 +69,15 @@ class Stats(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
 
 
+class KernelProfile(ctypes.Structure):
+    _fields_ = [
+        ("name", ctypes.c_char * 32),
+        ("launches", ctypes.c_int32), ("reserved0", ctypes.c_int32),
+        ("voxel_visits", ctypes.c_int64),
+        ("device_ms", ctypes.c_double), ("algorithmic_bytes", ctypes.c_double), ("bytes_per_voxel", ctypes.c_double),
+    ]
+
+
 TraceFn = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64,
                            ctypes.POINTER(ctypes.c_int32), ctypes.c_int32)
 
@@ -95,6 +104,8 @@ _lib.cgp_labels.argtypes = [_vp, _i32p]
 _lib.cgp_generate_volume.argtypes = [ctypes.c_int32] * 3 + [ctypes.c_uint64, ctypes.c_uint64, _u8p]
 _lib.cgp_caps_from_volume.argtypes = [ctypes.c_int32] * 4 + [_u8p, _i32p, _i32p, _i32p]
 _lib.cgp_device_count.argtypes = [_i32p]
+_lib.cgp_set_profiling.argtypes = [_vp, ctypes.c_int32]
+_lib.cgp_get_profile.argtypes = [_vp, ctypes.POINTER(KernelProfile), ctypes.c_int32, _i32p]
 _lib.cgp_abi_version.restype = ctypes.c_int32
 
 
@@ -203,6 +214,24 @@ class _Handle:
         """Canonical minimum cut S_min as 0/1 bytes."""
         out = np.empty(self._n, np.uint8)
         self._check(_lib.cgp_source_set(self._h, out.ctypes.data_as(_u8p)))
+        return out
+
+    def set_profiling(self, enable=True):
+        """Per-kernel event timing of the next solves (serialises them)."""
+        self._check(_lib.cgp_set_profiling(self._h, 1 if enable else 0))
+
+    def profile(self):
+        """Per-kernel records of the last solve: list of dicts (name, launches, voxel_visits, device_ms,
+        algorithmic_bytes, bytes_per_voxel)."""
+        arr = (KernelProfile * 16)()
+        n = ctypes.c_int32(0)
+        self._check(_lib.cgp_get_profile(self._h, arr, 16, ctypes.byref(n)))
+        out = []
+        for i in range(n.value):
+            r = arr[i]
+            out.append(dict(name=r.name.decode(), launches=r.launches, voxel_visits=r.voxel_visits,
+                            device_ms=r.device_ms, algorithmic_bytes=r.algorithmic_bytes,
+                            bytes_per_voxel=r.bytes_per_voxel))
         return out
 
     def labels(self):

@@ -15,6 +15,7 @@ struct cgp_graph_s {
 	int device = 0;
 	Mode mode = Mode::None;
 	cgp_policy policy = cgp::default_policy();
+	bool profiling = false;
 	cgp::TraceHook trace;
 	std::unique_ptr<cgp::CsrSolver> csr;
 	std::unique_ptr<cgp::GridSolver> grid;
@@ -228,6 +229,7 @@ int cgp_max_flow(cgp_graph g, int64_t *flow, cgp_stats *stats)
 			f = g->csr->solve(g->policy, stats, g->trace);
 			break;
 		case cgp_graph_s::Mode::Grid:
+			g->grid->set_profiling(g->profiling);
 			f = g->grid->solve(g->policy, stats, g->trace);
 			break;
 		default:
@@ -277,6 +279,31 @@ int cgp_labels(cgp_graph g, int32_t *out)
 			g->grid->labels(out);
 		} else {
 			generic(g).labels(out);
+		}
+	});
+}
+
+int cgp_set_profiling(cgp_graph g, int32_t enable)
+{
+	if (!g) {
+		return CGP_ERR_INVALID;
+	}
+	g->profiling = enable != 0;
+	return CGP_OK;
+}
+
+int cgp_get_profile(cgp_graph g, cgp_kernel_profile *out, int32_t capacity, int32_t *count)
+{
+	return guarded(g, [&] {
+		if (capacity < 0 || (capacity > 0 && !out)) {
+			cgp::fail(CGP_ERR_INVALID, "invalid output array");
+		}
+		int n = 0;
+		if (g->mode == cgp_graph_s::Mode::Grid && g->grid) {
+			n = g->grid->profile(out, capacity);
+		}
+		if (count) {
+			*count = n;
 		}
 	});
 }

@@ -26,7 +26,8 @@
 namespace cgp {
 
 // ---------------------------------------------------------------------------------------------
-// init: residual planes, cancellation, excess, residual sink link; validates int32 headroom.
+// init: residual planes, cancellation, excess, residual sink link. Sets status bit 0 for a negative
+// capacity and bit 1 when a voxel's inflow or an edge's capacity pair does not fit int32.
 template <int kConn>
 __global__ __launch_bounds__(256) void k_grid_init(GridView g)
 {
@@ -34,14 +35,14 @@ __global__ __launch_bounds__(256) void k_grid_init(GridView g)
 	constexpr int K = D::K, H = D::H;
 	__shared__ long long red[4];
 	long long cancelled = 0;
-	bool bad = false;
+	bool negative = false, overflow = false;
 	const int64_t stride = (int64_t)gridDim.x * blockDim.x;
 	for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.nvox; v += stride) {
 		const int x = (int)(v % g.nx);
 		const int y = (int)((v / g.nx) % g.ny);
 		const int z = (int)(v / ((int64_t)g.nx * g.ny));
 		const int32_t cs = g.cs[v], ct = g.ct[v];
-		bad |= (cs < 0) | (ct < 0);
+		negative |= (cs < 0) | (ct < 0);
 		const int32_t m = cs < ct ? cs : ct;
 		cancelled += m;
 		long long incoming = cs - m;
@@ -58,18 +59,18 @@ __global__ __launch_bounds__(256) void k_grid_init(GridView g)
 					in = g.cn[(int64_t)(d - H) * g.nvox + u];
 					out = g.cnb ? g.cnb[(int64_t)(d - H) * g.nvox + u] : in;
 				}
-				bad |= (out < 0) | (in < 0);
-				bad |= ((long long)out + (long long)in) > 0x7fffffffLL;
+				negative |= (out < 0) | (in < 0);
+				overflow |= ((long long)out + (long long)in) > 0x7fffffffLL;
 				incoming += in;
 			}
 			g.r[(int64_t)d * g.nvox + v] = out;
 		}
-		bad |= incoming > 0x7fffffffLL;
+		overflow |= incoming > 0x7fffffffLL;
 		g.e[v] = cs - m;
 		g.rt[v] = ct - m;
 	}
-	if (bad) {
-		atomicOr(g.status, 1);
+	if (negative || overflow) {
+		atomicOr(g.status, (negative ? 1 : 0) | (overflow ? 2 : 0));
 	}
 	const long long total = block_sum_i64<256>(cancelled, red);
 	if (threadIdx.x == 0 && total) {

@@ -68,6 +68,78 @@ double ms_since(std::chrono::steady_clock::time_point t)
 
 }  // namespace
 
+GridSolver::~GridSolver()
+{
+	if (mEvStart) {
+		(void)hipEventDestroy(mEvStart);
+	}
+	if (mEvStop) {
+		(void)hipEventDestroy(mEvStop);
+	}
+}
+
+template <typename F>
+void GridSolver::launch(Kern kind, int64_t voxels, double bytes_per_voxel, F &&fn)
+{
+	if (!mProfile) {
+		fn();
+		CGP_LAUNCH_CHECK();
+		return;
+	}
+	if (!mEvStart) {
+		CGP_HIP(hipEventCreate(&mEvStart));
+		CGP_HIP(hipEventCreate(&mEvStop));
+	}
+	unsigned long long *pinned = reinterpret_cast<unsigned long long *>(mHostScratch.get());
+	unsigned long long before = 0;
+	if (voxels < 0) {
+		CGP_HIP(hipMemcpyAsync(pinned, mCounters.get(), sizeof(unsigned long long), hipMemcpyDeviceToHost, mStream));
+		CGP_HIP(hipStreamSynchronize(mStream));
+		before = pinned[0];
+	}
+	CGP_HIP(hipEventRecord(mEvStart, mStream));
+	fn();
+	CGP_LAUNCH_CHECK();
+	CGP_HIP(hipEventRecord(mEvStop, mStream));
+	CGP_HIP(hipEventSynchronize(mEvStop));
+	float ms = 0.0f;
+	CGP_HIP(hipEventElapsedTime(&ms, mEvStart, mEvStop));
+	if (voxels < 0) {
+		CGP_HIP(hipMemcpyAsync(pinned, mCounters.get(), sizeof(unsigned long long), hipMemcpyDeviceToHost, mStream));
+		CGP_HIP(hipStreamSynchronize(mStream));
+		voxels = (int64_t)(pinned[0] - before) * mShape.tx * mShape.ty * mShape.tz;
+	}
+	KernelRecord &rec = mProf[kind];
+	rec.launches += 1;
+	rec.voxels += voxels;
+	rec.ms += ms;
+	rec.bytes_per_voxel = bytes_per_voxel;
+}
+
+int GridSolver::profile(cgp_kernel_profile *out, int capacity) const
+{
+	static const char *names[kKernCount] = {"grid_init", "grid_bfs", "grid_discharge", "grid_source_closure",
+	                                        "compact_tiles", "fill"};
+	int n = 0;
+	for (int k = 0; k < kKernCount; ++k) {
+		if (mProf[k].launches == 0) {
+			continue;
+		}
+		if (out && n < capacity) {
+			cgp_kernel_profile &p = out[n];
+			std::memset(&p, 0, sizeof(p));
+			std::strncpy(p.name, names[k], sizeof(p.name) - 1);
+			p.launches = mProf[k].launches;
+			p.voxel_visits = mProf[k].voxels;
+			p.device_ms = mProf[k].ms;
+			p.bytes_per_voxel = mProf[k].bytes_per_voxel;
+			p.algorithmic_bytes = (double)mProf[k].voxels * mProf[k].bytes_per_voxel;
+		}
+		++n;
+	}
+	return n;
+}
+
 GridSolver::Shape GridSolver::pick_shape(const cgp_policy &policy) const
 {
 	if (policy.tile_x || policy.tile_y || policy.tile_z) {
@@ -202,21 +274,27 @@ void GridSolver::launch_init()
 {
 	GridView g = CGP_VIEW();
 	const int blocks = blocks_for(mVox, 256, 8192);
-	if (mConn == 6) {
-		hipLaunchKernelGGL(k_grid_init<6>, dim3(blocks), dim3(256), 0, mStream, g);
-	} else {
-		hipLaunchKernelGGL(k_grid_init<26>, dim3(blocks), dim3(256), 0, mStream, g);
-	}
-	CGP_LAUNCH_CHECK();
+	// reads cs, ct and the n-link planes, writes e, rt and the K residual planes
+	const double bytes = 4.0 * (2 + (mConn / 2) * (mHasBack ? 2 : 1)) + 4.0 * (2 + mConn);
+	launch(kInit, mVox, bytes, [&] {
+		if (mConn == 6) {
+			hipLaunchKernelGGL(k_grid_init<6>, dim3(blocks), dim3(256), 0, mStream, g);
+		} else {
+			hipLaunchKernelGGL(k_grid_init<26>, dim3(blocks), dim3(256), 0, mStream, g);
+		}
+	});
 }
 
 void GridSolver::launch_compact(const uint8_t *flags, int colour)
 {
-	hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, mStream, mCount.get(), (int64_t)1, 0);
+	launch(kFill, 0, 0.0, [&] {
+		hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, mStream, mCount.get(), (int64_t)1, 0);
+	});
 	const int colour_mode = mConn == 6 ? 2 : 8;
-	hipLaunchKernelGGL(k_compact_tiles, dim3(blocks_for(mTiles, 256, 1 << 20)), dim3(256), 0, mStream, flags, mTiles,
-	                   mNtx, mNty, colour_mode, colour, mList.get(), mCount.get());
-	CGP_LAUNCH_CHECK();
+	launch(kCompact, 0, 0.0, [&] {
+		hipLaunchKernelGGL(k_compact_tiles, dim3(blocks_for(mTiles, 256, 1 << 20)), dim3(256), 0, mStream, flags,
+		                   mTiles, mNtx, mNty, colour_mode, colour, mList.get(), mCount.get());
+	});
 }
 
 int GridSolver::count_flags(const uint8_t *flags, int colour)
@@ -231,19 +309,27 @@ int GridSolver::count_flags(const uint8_t *flags, int colour)
 int GridSolver::run_global_relabel(int32_t threshold, cgp_stats &stats)
 {
 	GridView g = CGP_VIEW();
-	hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(mVox, 256, 8192)), dim3(256), 0, mStream, mH.get(), mVox, kLabelInf);
+	launch(kFill, mVox, 4.0, [&] {
+		hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(mVox, 256, 8192)), dim3(256), 0, mStream, mH.get(), mVox,
+		                   kLabelInf);
+	});
 	const int32_t *list = mAllTiles.get();
 	const int32_t *count = mAllTiles.get() + mTiles;
 	int blocks = mTiles;
+	const int64_t tile_voxels = (int64_t)mShape.tx * mShape.ty * mShape.tz;
 	for (;;) {
-		hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(mTiles, 256)), dim3(256), 0, mStream, mDirtyA.get(), (int64_t)mTiles,
-		                   (uint8_t)0);
-		dispatch_shape(mConn, mShape.tx, mShape.ty, mShape.tz, [&](auto tag) {
-			using S = decltype(tag);
-			hipLaunchKernelGGL((k_grid_bfs<S::conn, S::tx, S::ty, S::tz>), dim3(blocks), dim3(S::tx * S::ty), 0, mStream,
-			                   g, list, count, threshold, mDirtyA.get(), mActive.get());
+		launch(kFill, 0, 0.0, [&] {
+			hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(mTiles, 256)), dim3(256), 0, mStream, mDirtyA.get(),
+			                   (int64_t)mTiles, (uint8_t)0);
 		});
-		CGP_LAUNCH_CHECK();
+		// reads r[K], rt, e and h, writes h
+		launch(kBfs, (int64_t)blocks * tile_voxels, 4.0 * (mConn + 4), [&] {
+			dispatch_shape(mConn, mShape.tx, mShape.ty, mShape.tz, [&](auto tag) {
+				using S = decltype(tag);
+				hipLaunchKernelGGL((k_grid_bfs<S::conn, S::tx, S::ty, S::tz>), dim3(blocks), dim3(S::tx * S::ty), 0,
+				                   mStream, g, list, count, threshold, mDirtyA.get(), mActive.get());
+			});
+		});
 		++stats.bfs_passes;
 		const int dirty = count_flags(mDirtyA.get(), -1);
 		if (dirty == 0) {
@@ -262,31 +348,42 @@ void GridSolver::run_discharge_phase(int colour, int max_iters)
 	launch_compact(mActive.get(), colour);
 	// every tile of one colour; blocks beyond the compacted count exit at once
 	const int blocks = mMaxColourTiles;
-	dispatch_shape(mConn, mShape.tx, mShape.ty, mShape.tz, [&](auto tag) {
-		using S = decltype(tag);
-		hipLaunchKernelGGL((k_grid_discharge<S::conn, S::tx, S::ty, S::tz>), dim3(blocks), dim3(S::tx * S::ty), 0,
-		                   mStream, g, mList.get(), mCount.get(), max_iters, mActive.get());
+	// reads and writes e, rt, h and r[K]
+	launch(kDischarge, -1, 8.0 * (mConn + 3), [&] {
+		dispatch_shape(mConn, mShape.tx, mShape.ty, mShape.tz, [&](auto tag) {
+			using S = decltype(tag);
+			hipLaunchKernelGGL((k_grid_discharge<S::conn, S::tx, S::ty, S::tz>), dim3(blocks), dim3(S::tx * S::ty), 0,
+			                   mStream, g, mList.get(), mCount.get(), max_iters, mActive.get());
+		});
 	});
-	CGP_LAUNCH_CHECK();
 }
 
 void GridSolver::run_source_closure(cgp_stats &stats)
 {
 	GridView g = CGP_VIEW();
-	hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(mVox, 256, 8192)), dim3(256), 0, mStream, mS.get(), mVox, (uint8_t)0);
+	launch(kFill, mVox, 1.0, [&] {
+		hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(mVox, 256, 8192)), dim3(256), 0, mStream, mS.get(), mVox,
+		                   (uint8_t)0);
+	});
 	const int32_t *list = mAllTiles.get();
 	const int32_t *count = mAllTiles.get() + mTiles;
 	int blocks = mTiles;
 	int first = 1;
+	const int64_t tile_voxels = (int64_t)mShape.tx * mShape.ty * mShape.tz;
 	for (;;) {
-		hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(mTiles, 256)), dim3(256), 0, mStream, mDirtyB.get(), (int64_t)mTiles,
-		                   (uint8_t)0);
-		dispatch_shape(mConn, mShape.tx, mShape.ty, mShape.tz, [&](auto tag) {
-			using S = decltype(tag);
-			hipLaunchKernelGGL((k_grid_source_closure<S::conn, S::tx, S::ty, S::tz>), dim3(blocks), dim3(S::tx * S::ty), 0,
-			                   mStream, g, list, count, first, mDirtyB.get());
+		launch(kFill, 0, 0.0, [&] {
+			hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(mTiles, 256)), dim3(256), 0, mStream, mDirtyB.get(),
+			                   (int64_t)mTiles, (uint8_t)0);
 		});
-		CGP_LAUNCH_CHECK();
+		// first pass: reads r[K], e and s, writes the open mask and s; later passes: mask and s
+		const double bytes = first ? 4.0 * (mConn + 1) + 1 + 4 + 1 : 4.0 + 1 + 1;
+		launch(kClosure, (int64_t)blocks * tile_voxels, bytes, [&] {
+			dispatch_shape(mConn, mShape.tx, mShape.ty, mShape.tz, [&](auto tag) {
+				using S = decltype(tag);
+				hipLaunchKernelGGL((k_grid_source_closure<S::conn, S::tx, S::ty, S::tz>), dim3(blocks),
+				                   dim3(S::tx * S::ty), 0, mStream, g, list, count, first, mDirtyB.get());
+			});
+		});
 		++stats.cut_passes;
 		first = 0;
 		const int dirty = count_flags(mDirtyB.get(), -1);
@@ -323,6 +420,9 @@ int64_t GridSolver::solve(const cgp_policy &policy, cgp_stats *stats_out, const 
 	std::memset(&stats, 0, sizeof(stats));
 	const auto t_start = std::chrono::steady_clock::now();
 	mSolved = false;
+	for (KernelRecord &rec : mProf) {
+		rec = KernelRecord();
+	}
 	configure_tiles(pick_shape(policy));
 	const int ncolours = mConn == 6 ? 2 : 8;
 	const int phases = policy.discharges_per_relabel > 0 ? policy.discharges_per_relabel : 2 * ncolours;
@@ -343,15 +443,19 @@ int64_t GridSolver::solve(const cgp_policy &policy, cgp_stats *stats_out, const 
 		int32_t *pinned = reinterpret_cast<int32_t *>(mHostScratch.get());
 		CGP_HIP(hipMemcpyAsync(pinned, mStatus.get(), sizeof(int32_t), hipMemcpyDeviceToHost, mStream));
 		CGP_HIP(hipStreamSynchronize(mStream));
-		if (pinned[0]) {
-			fail(CGP_ERR_OVERFLOW, "capacities must be non-negative and each voxel's source capacity plus incoming "
-			                       "capacities, and each edge's two capacities, must sum below 2^31");
+		if (pinned[0] & 1) {
+			fail(CGP_ERR_INVALID, "capacities must be non-negative");
+		}
+		if (pinned[0] & 2) {
+			fail(CGP_ERR_OVERFLOW, "each voxel's source capacity plus incoming capacities, and each edge's two "
+			                       "capacities, must sum below 2^31");
 		}
 	}
 	stats.init_ms = ms_since(t_start);
 	trace.emit(CGP_TRACE_STARTED, 0, 0, 0);
 
 	int round = 0;
+	int64_t phase = 0;  // colours rotate across rounds, so every colour is discharged whatever `phases` is
 	for (;;) {
 		trace.emit(CGP_TRACE_BEGIN_ITERATION, round, 0, 0);
 		auto t0 = std::chrono::steady_clock::now();
@@ -370,8 +474,8 @@ int64_t GridSolver::solve(const cgp_policy &policy, cgp_stats *stats_out, const 
 			fail(CGP_ERR_NOT_CONVERGED, "max_rounds (" + std::to_string(max_rounds) + ") reached before convergence");
 		}
 		t0 = std::chrono::steady_clock::now();
-		for (int p = 0; p < phases; ++p) {
-			run_discharge_phase(p % ncolours, max_iters);
+		for (int p = 0; p < phases; ++p, ++phase) {
+			run_discharge_phase((int)(phase % ncolours), max_iters);
 			++stats.discharge_phases;
 		}
 		if (trace.fn) {

@@ -34,8 +34,23 @@ public:
 	void labels(int32_t *host_out);
 	int64_t vertex_count() const { return mVox; }
 	bool solved() const { return mSolved; }
+	void set_profiling(bool enable) { mProfile = enable; }
+	int profile(cgp_kernel_profile *out, int capacity) const;
+	~GridSolver();
 
 private:
+	enum Kern { kInit = 0, kBfs, kDischarge, kClosure, kCompact, kFill, kKernCount };
+	struct KernelRecord {
+		int launches = 0;
+		int64_t voxels = 0;
+		double ms = 0.0;
+		double bytes_per_voxel = 0.0;
+	};
+	// Launch through here: with profiling on, the launch is bracketed by events and waited for.
+	// voxels < 0 reads the voxel visits from the discharge tile counter.
+	template <typename F>
+	void launch(Kern kind, int64_t voxels, double bytes_per_voxel, F &&fn);
+
 	struct Shape {
 		int tx, ty, tz;
 	};
@@ -65,6 +80,9 @@ private:
 	DeviceBuffer<int32_t> mStatus, mCount, mList, mAllTiles;
 	DeviceBuffer<uint8_t> mActive, mDirtyA, mDirtyB;
 	PinnedBuffer<int64_t> mHostScratch;
+	bool mProfile = false;
+	KernelRecord mProf[kKernCount];
+	hipEvent_t mEvStart = nullptr, mEvStop = nullptr;
 };
 
 // Generic graph (CSR), the data layout of the reference's Graph<TFlow>.

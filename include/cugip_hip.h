@@ -134,6 +134,24 @@ int cgp_source_set(cgp_graph graph, uint8_t *out);
 #define CGP_LABEL_UNREACHABLE 0x3fffffff
 int cgp_labels(cgp_graph graph, int32_t *out);
 
+/* ---- profiling -----------------------------------------------------------------------------
+   When enabled, the grid solver brackets every kernel launch with HIP events on its own stream
+   and waits for each launch, so per-kernel device time and algorithmic bytes can be read back
+   after cgp_max_flow. This serialises the solve: leave it off when timing whole solves.
+   algorithmic_bytes = voxel_visits x bytes_per_voxel, where bytes_per_voxel counts each plane
+   a kernel must read or write once per voxel it processes (halo values staged in LDS count 0). */
+typedef struct cgp_kernel_profile {
+	char name[32];
+	int32_t launches;
+	int32_t reserved0;
+	int64_t voxel_visits;
+	double device_ms;
+	double algorithmic_bytes;
+	double bytes_per_voxel;
+} cgp_kernel_profile;
+int cgp_set_profiling(cgp_graph graph, int32_t enable);
+int cgp_get_profile(cgp_graph graph, cgp_kernel_profile *out, int32_t capacity, int32_t *count);
+
 /* ---- synthetic instances ------------------------------------------------------------------
    Blob volume (3-8 ellipsoids at 180 on 60, N(0, 25^2) noise, clamped to 0-255) and the int32
    capacities derived from it (Boykov-Jolly t-links, contrast-sensitive n-links). Integer-only:

@@ -1,0 +1,203 @@
+"""Grid path on the GPU, through the C ABI: bit-exact parity with the CPU oracle (max-flow value,
+S_min and T_min), the committed golden fixtures, full-size properties, edge cases and errors."""
+import numpy as np
+import pytest
+
+import cugip_amd
+import oracle
+from tests.gpu_util import digest, grid_cut_capacity, say
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_solve(cs, ct, cn, cnb=None, conn=6, **policy):
+    with cugip_amd.GridGraph() as g:
+        g.set_grid(cs, ct, cn, cnb, conn=conn)
+        if policy:
+            g.set_policy(**policy)
+        flow = g.max_flow()
+        return flow, g.source_set(), g.fill_segments(1, 0), g.stats.as_dict()
+
+
+def check_against_oracle(cs, ct, cn, cnb=None, conn=6, method="bk", **policy):
+    nz, ny, nx = cs.shape
+    flow, smin, tmin, stats = gpu_solve(cs, ct, cn, cnb, conn, **policy)
+    ref_flow, ref_smin, ref_tmin = oracle.grid_solve(method, nx, ny, nz, conn, cs, ct, cn, cnb)
+    assert flow == ref_flow
+    assert np.array_equal(smin, ref_smin), int((smin != ref_smin).sum())
+    assert np.array_equal(tmin, ref_tmin), int((tmin != ref_tmin).sum())
+    return stats
+
+
+CASES = [
+    # dims, conn, seed, instance, policy
+    ((37, 29, 23), 6, 3, 0, {}),
+    ((64, 64, 64), 6, 0, 0, {}),
+    ((64, 64, 64), 6, 1, 0, dict(tile_x=32, tile_y=8, tile_z=8)),
+    ((64, 64, 64), 6, 2, 0, dict(tile_x=64, tile_y=4, tile_z=4)),
+    ((64, 64, 64), 6, 3, 0, dict(tile_x=32, tile_y=4, tile_z=4)),
+    ((64, 64, 64), 6, 4, 0, dict(tile_x=64, tile_y=4, tile_z=8)),
+    ((100, 7, 33), 6, 5, 2, dict(discharges_per_relabel=1, max_tile_iterations=1)),
+    ((45, 61, 12), 6, 6, 1, dict(discharges_per_relabel=7, max_tile_iterations=64)),
+    ((70, 52, 31), 6, 7, 0, dict(minimal_residual=3.5)),
+    ((24, 20, 16), 26, 1, 0, {}),
+    ((40, 33, 17), 26, 2, 0, dict(tile_x=16, tile_y=8, tile_z=4)),
+    ((35, 18, 21), 26, 3, 4, dict(tile_x=16, tile_y=4, tile_z=4)),
+    ((48, 40, 32), 26, 0, 0, dict(discharges_per_relabel=3)),
+]
+
+
+@pytest.mark.parametrize("dims,conn,seed,instance,policy", CASES)
+def test_parity_synthetic(dims, conn, seed, instance, policy):
+    nx, ny, nz = dims
+    say(f"grid parity {dims} conn {conn} seed {seed} instance {instance} policy {policy}")
+    vol, cs, ct, cn = cugip_amd.make_instance(nx, ny, nz, seed, instance, conn)
+    check_against_oracle(cs, ct, cn, None, conn, **policy)
+
+
+@pytest.mark.parametrize("conn", [6, 26])
+@pytest.mark.parametrize("seed", range(4))
+def test_parity_random_asymmetric(conn, seed):
+    """Random integer capacities (with zeros) and independent back capacities."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = (int(v) for v in rng.integers(3, 40, 3))
+    say(f"grid random asym {nx}x{ny}x{nz} conn {conn} seed {seed}")
+    half = conn // 2
+    cs = (rng.integers(0, 200, (nz, ny, nx)) * (rng.random((nz, ny, nx)) < 0.3)).astype(np.int32)
+    ct = (rng.integers(0, 200, (nz, ny, nx)) * (rng.random((nz, ny, nx)) < 0.3)).astype(np.int32)
+    cn = (rng.integers(0, 60, (half, nz, ny, nx)) * (rng.random((half, nz, ny, nx)) < 0.8)).astype(np.int32)
+    cnb = (rng.integers(0, 60, (half, nz, ny, nx)) * (rng.random((half, nz, ny, nx)) < 0.8)).astype(np.int32)
+    check_against_oracle(cs, ct, cn, cnb, conn, method="dinic")
+
+
+@pytest.mark.parametrize("dims", [(1, 1, 1), (1, 1, 50), (50, 1, 1), (1, 37, 1), (7, 1, 9), (33, 9, 5), (65, 9, 5)])
+def test_degenerate_shapes(dims):
+    nx, ny, nz = dims
+    say(f"grid degenerate {dims}")
+    rng = np.random.default_rng(nx * 100 + ny * 10 + nz)
+    cs = rng.integers(0, 50, (nz, ny, nx)).astype(np.int32)
+    ct = rng.integers(0, 50, (nz, ny, nx)).astype(np.int32)
+    cn = rng.integers(0, 30, (3, nz, ny, nx)).astype(np.int32)
+    check_against_oracle(cs, ct, cn, method="dinic")
+    cn26 = rng.integers(0, 30, (13, nz, ny, nx)).astype(np.int32)
+    check_against_oracle(cs, ct, cn26, conn=26, method="dinic")
+
+
+def test_closed_forms():
+    say("grid closed forms")
+    rng = np.random.default_rng(3)
+    shape = (9, 10, 11)
+    cs = rng.integers(0, 100, shape).astype(np.int32)
+    ct = rng.integers(0, 100, shape).astype(np.int32)
+    zero = np.zeros((3,) + shape, np.int32)
+    # no n-links: every voxel is cut on its own
+    flow, smin, tmin, _ = gpu_solve(cs, ct, zero)
+    assert flow == int(np.minimum(cs, ct).astype(np.int64).sum())
+    assert np.array_equal(smin, (cs > ct).reshape(-1)) and np.array_equal(tmin, (ct > cs).reshape(-1))
+    # no t-links: no flow at all
+    cn = rng.integers(0, 100, (3,) + shape).astype(np.int32)
+    flow, smin, tmin, _ = gpu_solve(np.zeros(shape, np.int32), np.zeros(shape, np.int32), cn)
+    assert flow == 0 and not smin.any() and not tmin.any()
+
+
+def test_golden_fixtures(golden):
+    for rec in golden["instances"]:
+        nx, ny, nz = rec["dims"]
+        say(f"golden {rec['dims']} conn {rec['conn']} seed {rec['seed']} instance {rec['instance']}")
+        vol, cs, ct, cn = cugip_amd.make_instance(nx, ny, nz, rec["seed"], rec["instance"], rec["conn"])
+        flow, smin, tmin, _ = gpu_solve(cs, ct, cn, conn=rec["conn"])
+        assert flow == rec["flow"]
+        assert digest(smin) == rec["smin_sha256"] and digest(tmin) == rec["tmin_sha256"]
+
+
+def test_full_size_properties():
+    """256^3: the S_min and T_min cuts both have capacity equal to the flow, S_min and T_min are
+    disjoint, and repeated solves on one handle agree bit for bit."""
+    nx = ny = nz = 256
+    say("grid 256^3 properties")
+    vol, cs, ct, cn = cugip_amd.make_instance(nx, ny, nz, seed=2)
+    with cugip_amd.GridGraph() as g:
+        g.set_grid(cs, ct, cn)
+        flow = g.max_flow()
+        smin = g.source_set()
+        tmin = g.fill_segments(1, 0)
+        flow2 = g.max_flow()
+        assert flow2 == flow and np.array_equal(g.source_set(), smin)
+    assert not (smin.astype(bool) & tmin.astype(bool)).any()
+    assert grid_cut_capacity(smin, cs, ct, cn) == flow
+    assert grid_cut_capacity(1 - tmin, cs, ct, cn) == flow
+
+
+def test_device_pointer_entry():
+    from tests.hip_rt import DeviceArray
+    nx, ny, nz = 48, 40, 24
+    say("grid device pointers")
+    vol, cs, ct, cn = cugip_amd.make_instance(nx, ny, nz, seed=12)
+    dev = [DeviceArray(a) for a in (cs, ct, cn)]
+    with cugip_amd.GridGraph() as g:
+        g.set_grid_device((nz, ny, nx), 6, dev[0].address, dev[1].address, dev[2].address)
+        flow = g.max_flow()
+        smin = g.source_set()
+    for d in dev:
+        d.free()
+    ref_flow, ref_smin, _ = oracle.grid_solve("bk", nx, ny, nz, 6, cs, ct, cn)
+    assert flow == ref_flow and np.array_equal(smin, ref_smin)
+
+
+def test_errors():
+    say("grid errors")
+    shape = (4, 5, 6)
+    ones = np.ones(shape, np.int32)
+    cn = np.ones((3,) + shape, np.int32)
+    with cugip_amd.GridGraph() as g:
+        with pytest.raises(cugip_amd.CgpError) as e:
+            g.source_set()
+        assert e.value.code == cugip_amd.CGP_ERR_STATE
+        g.set_grid(-ones, ones, cn)
+        with pytest.raises(cugip_amd.CgpError) as e:
+            g.max_flow()
+        assert e.value.code == cugip_amd.CGP_ERR_INVALID
+        big = np.full(shape, 2 ** 31 - 1, np.int32)
+        g.set_grid(big, ones, cn)
+        with pytest.raises(cugip_amd.CgpError) as e:
+            g.max_flow()
+        assert e.value.code == cugip_amd.CGP_ERR_OVERFLOW
+        g.set_grid(ones, ones, cn)
+        g.set_policy(tile_x=7, tile_y=3, tile_z=1)
+        with pytest.raises(cugip_amd.CgpError) as e:
+            g.max_flow()
+        assert e.value.code == cugip_amd.CGP_ERR_INVALID
+        g.set_policy()  # defaults
+        assert g.max_flow() == int(np.prod(shape))
+    with pytest.raises(cugip_amd.CgpError) as e:
+        cugip_amd.GridGraph().set_grid(ones, ones, cn, conn=8)
+
+
+def test_round_cap_reports_not_converged():
+    say("grid round cap")
+    vol, cs, ct, cn = cugip_amd.make_instance(64, 64, 32, seed=1)
+    with cugip_amd.GridGraph() as g:
+        g.set_grid(cs, ct, cn)
+        g.set_policy(max_rounds=1, discharges_per_relabel=1, max_tile_iterations=1)
+        with pytest.raises(cugip_amd.CgpError) as e:
+            g.max_flow()
+        assert e.value.code == cugip_amd.CGP_ERR_NOT_CONVERGED
+        with pytest.raises(cugip_amd.CgpError):
+            g.source_set()  # no result after a failed solve
+        g.set_policy()
+        flow = g.max_flow()
+    assert flow == oracle.grid_solve("bk", 64, 64, 32, 6, cs, ct, cn)[0]
+
+
+def test_cpp_dropin_grid():
+    import os
+    import subprocess
+    say("cpp grid program")
+    exe = os.path.join(os.path.dirname(__file__), "cpp", "graph_cut_grid")
+    out = subprocess.run(["timeout", "-k", "5", "120", exe, "53", "41", "29", "6", "17"], capture_output=True,
+                         text=True)
+    assert out.returncode == 0, out.stdout + out.stderr
+    vol, cs, ct, cn = cugip_amd.make_instance(53, 41, 29, seed=17)
+    flow, smin, tmin = oracle.grid_solve("bk", 53, 41, 29, 6, cs, ct, cn)
+    assert out.stdout.split() == ["flow", str(flow), "source_side", str(int(smin.sum())), "sink_side",
+                                  str(int(tmin.sum()))]
