@@ -32,6 +32,7 @@ CGP_ERR_STATE = 6
 LABEL_UNREACHABLE = 0x3FFFFFFF
 
 TRACE_STARTED, TRACE_BEGIN_ITERATION, TRACE_AFTER_RELABEL, TRACE_AFTER_PUSH, TRACE_FINISHED = range(5)
+ENGINE_AUTO, ENGINE_TILES, ENGINE_WAVE = 0, 1, 2
 
 
 class Edge(ctypes.Structure):
@@ -47,7 +48,8 @@ class Policy(ctypes.Structure):
         ("max_tile_iterations", ctypes.c_int32),
         ("max_rounds", ctypes.c_int32),
         ("minimal_residual", ctypes.c_float),
-        ("reserved", ctypes.c_int32 * 8),
+        ("engine", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 7),
     ]
 
 

@@ -7,6 +7,7 @@
 #include <cstring>
 
 #include "grid_kernels.h"
+#include "grid_wave.h"
 #include "solver.h"
 
 namespace cgp {
@@ -140,8 +141,21 @@ int GridSolver::profile(cgp_kernel_profile *out, int capacity) const
 	return n;
 }
 
-GridSolver::Shape GridSolver::pick_shape(const cgp_policy &policy) const
+GridSolver::Shape GridSolver::pick_shape(const cgp_policy &policy)
 {
+	const bool wave_shape = (!policy.tile_x && !policy.tile_y && !policy.tile_z) ||
+	                           (policy.tile_x == wave::TX && policy.tile_y == wave::TY && policy.tile_z == wave::TZ);
+	if (policy.engine == CGP_ENGINE_WAVE) {
+		if (mConn != 6 || !wave_shape) {
+			fail(CGP_ERR_INVALID, "the wave engine handles 6-connected grids with 32x8x4 tiles only");
+		}
+	} else if (policy.engine != CGP_ENGINE_AUTO && policy.engine != CGP_ENGINE_TILES) {
+		fail(CGP_ERR_INVALID, "unknown engine " + std::to_string(policy.engine));
+	}
+	mWave = policy.engine == CGP_ENGINE_WAVE;
+	if (mWave) {
+		return {wave::TX, wave::TY, wave::TZ};
+	}
 	if (policy.tile_x || policy.tile_y || policy.tile_z) {
 		if (!shape_supported(mConn, policy.tile_x, policy.tile_y, policy.tile_z)) {
 			fail(CGP_ERR_INVALID, "unsupported tile shape " + std::to_string(policy.tile_x) + "x" +
@@ -232,6 +246,12 @@ void GridSolver::configure_tiles(const Shape &shape)
 	}
 	mMaxColourTiles = *std::max_element(per_colour.begin(), per_colour.end());
 	CGP_HIP(hipMemcpyAsync(mAllTiles.get(), all.data(), all.size() * sizeof(int32_t), hipMemcpyHostToDevice, mStream));
+	if (mWave) {
+		mBits.allocate((size_t)wave::kPlanes * mTiles * wave::R);
+		mSBits.allocate((size_t)mTiles * wave::R);
+		mReach.allocate((size_t)mTiles * wave::R);
+		mFaces.allocate((size_t)mTiles * wave::kFaces);
+	}
 	CGP_HIP(hipStreamSynchronize(mStream));
 }
 
@@ -240,9 +260,12 @@ static GridView make_view(int nx, int ny, int nz, int ntx, int nty, int ntz, int
                           DeviceBuffer<uint8_t> &s, DeviceBuffer<uint32_t> &omask, DeviceBuffer<int32_t> &cs,
                           DeviceBuffer<int32_t> &ct, DeviceBuffer<int32_t> &cn, DeviceBuffer<int32_t> &cnb,
                           DeviceBuffer<unsigned long long> &flow, DeviceBuffer<int32_t> &status,
-                          DeviceBuffer<unsigned long long> &counters)
+                          DeviceBuffer<unsigned long long> &counters, DeviceBuffer<int32_t> &faces,
+                          DeviceBuffer<uint32_t> &reach)
 {
 	GridView g;
+	g.faces = faces.get();
+	g.reach = reach.get();
 	g.nx = nx;
 	g.ny = ny;
 	g.nz = nz;
@@ -268,7 +291,7 @@ static GridView make_view(int nx, int ny, int nz, int ntx, int nty, int ntz, int
 
 #define CGP_VIEW()                                                                                                 \
 	make_view(mNx, mNy, mNz, mNtx, mNty, mNtz, mVox, mE, mRt, mH, mR, mS, mOmask, mCs, mCt, mCn, mCnb, mFlow, mStatus, \
-	          mCounters)
+	          mCounters, mFaces, mReach)
 
 void GridSolver::launch_init()
 {
@@ -276,6 +299,12 @@ void GridSolver::launch_init()
 	const int blocks = blocks_for(mVox, 256, 8192);
 	// reads cs, ct and the n-link planes, writes e, rt and the K residual planes
 	const double bytes = 4.0 * (2 + (mConn / 2) * (mHasBack ? 2 : 1)) + 4.0 * (2 + mConn);
+	if (mWave) {
+		launch(kInit, mVox, bytes + 1.0, [&] {
+			hipLaunchKernelGGL(wave::k_wave_init, dim3(mTiles), dim3(64), 0, mStream, g, mBits.get(), mTiles);
+		});
+		return;
+	}
 	launch(kInit, mVox, bytes, [&] {
 		if (mConn == 6) {
 			hipLaunchKernelGGL(k_grid_init<6>, dim3(blocks), dim3(256), 0, mStream, g);
@@ -309,10 +338,18 @@ int GridSolver::count_flags(const uint8_t *flags, int colour)
 int GridSolver::run_global_relabel(int32_t threshold, cgp_stats &stats)
 {
 	GridView g = CGP_VIEW();
-	launch(kFill, mVox, 4.0, [&] {
-		hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(mVox, 256, 8192)), dim3(256), 0, mStream, mH.get(), mVox,
-		                   kLabelInf);
-	});
+	if (mWave) {
+		const int64_t n = (int64_t)mTiles * wave::kFaces;
+		launch(kFill, mVox, 4.0 * n / mVox, [&] {
+			hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(n, 256, 8192)), dim3(256), 0, mStream, mFaces.get(), n,
+			                   kLabelInf);
+		});
+	} else {
+		launch(kFill, mVox, 4.0, [&] {
+			hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(mVox, 256, 8192)), dim3(256), 0, mStream, mH.get(), mVox,
+			                   kLabelInf);
+		});
+	}
 	const int32_t *list = mAllTiles.get();
 	const int32_t *count = mAllTiles.get() + mTiles;
 	int blocks = mTiles;
@@ -322,14 +359,23 @@ int GridSolver::run_global_relabel(int32_t threshold, cgp_stats &stats)
 			hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(mTiles, 256)), dim3(256), 0, mStream, mDirtyA.get(),
 			                   (int64_t)mTiles, (uint8_t)0);
 		});
-		// reads r[K], rt, e and h, writes h
-		launch(kBfs, (int64_t)blocks * tile_voxels, 4.0 * (mConn + 4), [&] {
-			dispatch_shape(mConn, mShape.tx, mShape.ty, mShape.tz, [&](auto tag) {
-				using S = decltype(tag);
-				hipLaunchKernelGGL((k_grid_bfs<S::conn, S::tx, S::ty, S::tz>), dim3(blocks), dim3(S::tx * S::ty), 0,
-				                   mStream, g, list, count, threshold, mDirtyA.get(), mActive.get());
+		if (mWave) {
+			// reads the tile's bit planes (8 bits/voxel) and its own face labels (832 per 1024 voxels),
+			// writes face labels and the reach plane
+			launch(kBfs, (int64_t)blocks * tile_voxels, 1.0 + 2.0 * 4.0 * wave::kFaces / wave::NV + 0.125, [&] {
+				hipLaunchKernelGGL(wave::k_wave_bfs, dim3(blocks), dim3(64), 0, mStream, g, mBits.get(), mTiles, list,
+				                   count, threshold, mDirtyA.get(), mActive.get(), (int32_t *)nullptr);
 			});
-		});
+		} else {
+			// reads r[K], rt, e and h, writes h
+			launch(kBfs, (int64_t)blocks * tile_voxels, 4.0 * (mConn + 4), [&] {
+				dispatch_shape(mConn, mShape.tx, mShape.ty, mShape.tz, [&](auto tag) {
+					using S = decltype(tag);
+					hipLaunchKernelGGL((k_grid_bfs<S::conn, S::tx, S::ty, S::tz>), dim3(blocks), dim3(S::tx * S::ty),
+					                   0, mStream, g, list, count, threshold, mDirtyA.get(), mActive.get());
+				});
+			});
+		}
 		++stats.bfs_passes;
 		const int dirty = count_flags(mDirtyA.get(), -1);
 		if (dirty == 0) {
@@ -348,6 +394,14 @@ void GridSolver::run_discharge_phase(int colour, int max_iters)
 	launch_compact(mActive.get(), colour);
 	// every tile of one colour; blocks beyond the compacted count exit at once
 	const int blocks = mMaxColourTiles;
+	if (mWave) {
+		// reads and writes e, rt, r[6] and the bit planes; writes the face labels
+		launch(kDischarge, -1, 8.0 * (mConn + 2) + 2.0 + 4.0 * wave::kFaces / wave::NV, [&] {
+			hipLaunchKernelGGL(wave::k_wave_discharge, dim3(blocks), dim3(64), 0, mStream, g, mBits.get(), mTiles,
+			                   mList.get(), mCount.get(), max_iters, mActive.get());
+		});
+		return;
+	}
 	// reads and writes e, rt, h and r[K]
 	launch(kDischarge, -1, 8.0 * (mConn + 3), [&] {
 		dispatch_shape(mConn, mShape.tx, mShape.ty, mShape.tz, [&](auto tag) {
@@ -361,10 +415,18 @@ void GridSolver::run_discharge_phase(int colour, int max_iters)
 void GridSolver::run_source_closure(cgp_stats &stats)
 {
 	GridView g = CGP_VIEW();
-	launch(kFill, mVox, 1.0, [&] {
-		hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(mVox, 256, 8192)), dim3(256), 0, mStream, mS.get(), mVox,
-		                   (uint8_t)0);
-	});
+	if (mWave) {
+		launch(kFill, mVox, 0.125, [&] {
+			hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for((int64_t)mTiles * wave::R * 4, 256, 8192)), dim3(256), 0,
+			                   mStream, reinterpret_cast<uint8_t *>(mSBits.get()), (int64_t)mTiles * wave::R * 4,
+			                   (uint8_t)0);
+		});
+	} else {
+		launch(kFill, mVox, 1.0, [&] {
+			hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(mVox, 256, 8192)), dim3(256), 0, mStream, mS.get(), mVox,
+			                   (uint8_t)0);
+		});
+	}
 	const int32_t *list = mAllTiles.get();
 	const int32_t *count = mAllTiles.get() + mTiles;
 	int blocks = mTiles;
@@ -375,15 +437,23 @@ void GridSolver::run_source_closure(cgp_stats &stats)
 			hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(mTiles, 256)), dim3(256), 0, mStream, mDirtyB.get(),
 			                   (int64_t)mTiles, (uint8_t)0);
 		});
-		// first pass: reads r[K], e and s, writes the open mask and s; later passes: mask and s
-		const double bytes = first ? 4.0 * (mConn + 1) + 1 + 4 + 1 : 4.0 + 1 + 1;
-		launch(kClosure, (int64_t)blocks * tile_voxels, bytes, [&] {
-			dispatch_shape(mConn, mShape.tx, mShape.ty, mShape.tz, [&](auto tag) {
-				using S = decltype(tag);
-				hipLaunchKernelGGL((k_grid_source_closure<S::conn, S::tx, S::ty, S::tz>), dim3(blocks),
-				                   dim3(S::tx * S::ty), 0, mStream, g, list, count, first, mDirtyB.get());
+		if (mWave) {
+			// reads 7 bit planes and the S bits, writes the S bits
+			launch(kClosure, (int64_t)blocks * tile_voxels, 1.0, [&] {
+				hipLaunchKernelGGL(wave::k_wave_closure, dim3(blocks), dim3(64), 0, mStream, g, mBits.get(),
+				                   mSBits.get(), mTiles, list, count, first, mDirtyB.get());
 			});
-		});
+		} else {
+			// first pass: reads r[K], e and s, writes the open mask and s; later passes: mask and s
+			const double bytes = first ? 4.0 * (mConn + 1) + 1 + 4 + 1 : 4.0 + 1 + 1;
+			launch(kClosure, (int64_t)blocks * tile_voxels, bytes, [&] {
+				dispatch_shape(mConn, mShape.tx, mShape.ty, mShape.tz, [&](auto tag) {
+					using S = decltype(tag);
+					hipLaunchKernelGGL((k_grid_source_closure<S::conn, S::tx, S::ty, S::tz>), dim3(blocks),
+					                   dim3(S::tx * S::ty), 0, mStream, g, list, count, first, mDirtyB.get());
+				});
+			});
+		}
 		++stats.cut_passes;
 		first = 0;
 		const int dirty = count_flags(mDirtyB.get(), -1);
@@ -518,8 +588,14 @@ void GridSolver::sink_side(uint8_t *host_out, uint8_t mask, uint8_t background)
 		fail(CGP_ERR_STATE, "fill_segments before max_flow");
 	}
 	DeviceBuffer<uint8_t> tmp(mVox);
-	hipLaunchKernelGGL(k_sink_side_mask, dim3(blocks_for(mVox, 256, 8192)), dim3(256), 0, mStream, mH.get(), mVox,
-	                   tmp.get(), mask, background);
+	if (mWave) {
+		GridView g = CGP_VIEW();
+		hipLaunchKernelGGL(wave::k_wave_plane_bytes, dim3(mTiles), dim3(64), 0, mStream, g, mReach.get(), tmp.get(),
+		                   mask, background);
+	} else {
+		hipLaunchKernelGGL(k_sink_side_mask, dim3(blocks_for(mVox, 256, 8192)), dim3(256), 0, mStream, mH.get(), mVox,
+		                   tmp.get(), mask, background);
+	}
 	CGP_LAUNCH_CHECK();
 	CGP_HIP(hipMemcpyAsync(host_out, tmp.get(), mVox, hipMemcpyDeviceToHost, mStream));
 	CGP_HIP(hipStreamSynchronize(mStream));
@@ -530,6 +606,12 @@ void GridSolver::source_side(uint8_t *host_out)
 	if (!mSolved) {
 		fail(CGP_ERR_STATE, "source_set before max_flow");
 	}
+	if (mWave) {
+		GridView g = CGP_VIEW();
+		hipLaunchKernelGGL(wave::k_wave_plane_bytes, dim3(mTiles), dim3(64), 0, mStream, g, mSBits.get(), mS.get(),
+		                   (uint8_t)1, (uint8_t)0);
+		CGP_LAUNCH_CHECK();
+	}
 	CGP_HIP(hipMemcpyAsync(host_out, mS.get(), mVox, hipMemcpyDeviceToHost, mStream));
 	CGP_HIP(hipStreamSynchronize(mStream));
 }
@@ -538,6 +620,13 @@ void GridSolver::labels(int32_t *host_out)
 {
 	if (!mSolved) {
 		fail(CGP_ERR_STATE, "labels before max_flow");
+	}
+	if (mWave) {
+		// the face labels are final: one pass over every tile yields every voxel's exact label
+		GridView g = CGP_VIEW();
+		hipLaunchKernelGGL(wave::k_wave_bfs, dim3(mTiles), dim3(64), 0, mStream, g, mBits.get(), mTiles,
+		                   mAllTiles.get(), mAllTiles.get() + mTiles, 0, mDirtyA.get(), mActive.get(), mH.get());
+		CGP_LAUNCH_CHECK();
 	}
 	CGP_HIP(hipMemcpyAsync(host_out, mH.get(), mVox * sizeof(int32_t), hipMemcpyDeviceToHost, mStream));
 	CGP_HIP(hipStreamSynchronize(mStream));

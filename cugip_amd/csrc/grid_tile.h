@@ -77,6 +77,8 @@ struct GridView {
 	unsigned long long *flow;
 	int32_t *status;         // nonzero: input error detected on the device
 	unsigned long long *counters; // [0] discharge visits, [1] bfs visits, [2] cut visits
+	int32_t *faces;          // wave engine: labels of each tile's face voxels
+	uint32_t *reach;         // wave engine: "reaches t" bit plane of the last global relabel
 };
 
 __device__ inline int64_t voxel_index(const GridView &g, int x, int y, int z)

@@ -54,7 +54,7 @@ private:
 	struct Shape {
 		int tx, ty, tz;
 	};
-	Shape pick_shape(const cgp_policy &policy) const;
+	Shape pick_shape(const cgp_policy &policy);
 	void configure_tiles(const Shape &shape);
 	void launch_init();
 	int run_global_relabel(int32_t threshold, cgp_stats &stats);
@@ -79,6 +79,9 @@ private:
 	DeviceBuffer<unsigned long long> mFlow, mCounters;
 	DeviceBuffer<int32_t> mStatus, mCount, mList, mAllTiles;
 	DeviceBuffer<uint8_t> mActive, mDirtyA, mDirtyB;
+	DeviceBuffer<uint32_t> mBits, mSBits, mReach;  // wave engine: bit planes, S_min and reach planes
+	DeviceBuffer<int32_t> mFaces;                  // wave engine: face labels
+	bool mWave = false;                    // wave engine (6-connected, 32x8x4 tiles)
 	PinnedBuffer<int64_t> mHostScratch;
 	bool mProfile = false;
 	KernelRecord mProf[kKernCount];

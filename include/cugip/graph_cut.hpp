@@ -160,6 +160,7 @@ struct GraphCutPolicy {
 	int dischargesPerRelabel = 0;
 	int maxTileIterations = 0;
 	int maxRounds = 0;
+	int engine = CGP_ENGINE_AUTO;
 };
 
 // ---- host-side view handed to trace objects (graph_cut_data.hpp:20-140) ------------------------
@@ -404,6 +405,7 @@ private:
 		p.discharges_per_relabel = aPolicy.dischargesPerRelabel;
 		p.max_tile_iterations = aPolicy.maxTileIterations;
 		p.max_rounds = aPolicy.maxRounds;
+		p.engine = aPolicy.engine;
 	}
 	template <typename TPolicy>
 	static void apply_schedule(cgp_policy &, const TPolicy &, long)

@@ -60,8 +60,16 @@ typedef struct cgp_policy {
 	                                         > 0 closes thin edges during the relabel rounds, and the solve
 	                                         then finishes with exact (threshold 0) rounds, so the result
 	                                         stays exact */
-	int32_t reserved[8];
+	int32_t engine;                       /* grid path: CGP_ENGINE_* (0 = automatic) */
+	int32_t reserved[7];
 } cgp_policy;
+
+/* Grid engines. AUTO picks TILES (the faster engine on the measured 6- and 26-connected workloads).
+   WAVE: one wave64 per 32x8x4 tile, tile-major bit planes, bit-parallel BFS (6-connected only).
+   TILES: one workgroup per tile of a policy-chosen shape, voxel-parallel LDS relaxation (6 and 26). */
+#define CGP_ENGINE_AUTO 0
+#define CGP_ENGINE_TILES 1
+#define CGP_ENGINE_WAVE 2
 
 /* Per-solve counters and device timings. */
 typedef struct cgp_stats {

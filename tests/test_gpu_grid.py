@@ -55,6 +55,32 @@ def test_parity_synthetic(dims, conn, seed, instance, policy):
     check_against_oracle(cs, ct, cn, None, conn, **policy)
 
 
+@pytest.mark.parametrize("engine", [cugip_amd.ENGINE_WAVE, cugip_amd.ENGINE_TILES])
+@pytest.mark.parametrize("dims,seed,policy", [
+    ((64, 64, 64), 0, {}),
+    ((96, 40, 28), 1, dict(max_tile_iterations=1)),
+    ((33, 17, 9), 2, dict(discharges_per_relabel=1)),
+    ((128, 64, 48), 3, dict(max_tile_iterations=64, discharges_per_relabel=6)),
+    ((61, 45, 37), 4, dict(minimal_residual=2.0)),
+])
+def test_parity_engines(engine, dims, seed, policy):
+    nx, ny, nz = dims
+    say(f"grid engine {engine} {dims} seed {seed} policy {policy}")
+    vol, cs, ct, cn = cugip_amd.make_instance(nx, ny, nz, seed, 0, 6)
+    check_against_oracle(cs, ct, cn, None, 6, engine=engine, **policy)
+
+
+def test_wave_engine_rejects_other_shapes():
+    say("wave engine shape check")
+    vol, cs, ct, cn = cugip_amd.make_instance(16, 16, 16, 1, 0, 26)
+    with cugip_amd.GridGraph() as g:
+        g.set_grid(cs, ct, cn, conn=26)
+        g.set_policy(engine=cugip_amd.ENGINE_WAVE)
+        with pytest.raises(cugip_amd.CgpError) as e:
+            g.max_flow()
+        assert e.value.code == cugip_amd.CGP_ERR_INVALID
+
+
 @pytest.mark.parametrize("conn", [6, 26])
 @pytest.mark.parametrize("seed", range(4))
 def test_parity_random_asymmetric(conn, seed):
@@ -79,6 +105,7 @@ def test_degenerate_shapes(dims):
     ct = rng.integers(0, 50, (nz, ny, nx)).astype(np.int32)
     cn = rng.integers(0, 30, (3, nz, ny, nx)).astype(np.int32)
     check_against_oracle(cs, ct, cn, method="dinic")
+    check_against_oracle(cs, ct, cn, method="dinic", engine=cugip_amd.ENGINE_TILES)
     cn26 = rng.integers(0, 30, (13, nz, ny, nx)).astype(np.int32)
     check_against_oracle(cs, ct, cn26, conn=26, method="dinic")
 
