@@ -71,6 +71,13 @@ class Stats(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
 
 
+class GridInstance(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+                ("reserved0", ctypes.c_int32),
+                ("cap_source", ctypes.c_void_p), ("cap_sink", ctypes.c_void_p),
+                ("cap_n", ctypes.c_void_p), ("cap_n_back", ctypes.c_void_p)]
+
+
 class KernelProfile(ctypes.Structure):
     _fields_ = [
         ("name", ctypes.c_char * 32),
@@ -109,6 +116,12 @@ _lib.cgp_device_count.argtypes = [_i32p]
 _lib.cgp_set_profiling.argtypes = [_vp, ctypes.c_int32]
 _lib.cgp_get_profile.argtypes = [_vp, ctypes.POINTER(KernelProfile), ctypes.c_int32, _i32p]
 _lib.cgp_abi_version.restype = ctypes.c_int32
+_lib.cgp_set_grid_batch.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(GridInstance),
+                                    ctypes.c_int32]
+_lib.cgp_batch_size.argtypes = [_vp, _i32p]
+_lib.cgp_solve_batch.argtypes = [_vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(Stats)]
+_lib.cgp_batch_source_set.argtypes = [_vp, ctypes.c_int32, _u8p]
+_lib.cgp_batch_fill_segments.argtypes = [_vp, ctypes.c_int32, _u8p, ctypes.c_uint8, ctypes.c_uint8]
 
 
 class CgpError(RuntimeError):
@@ -303,6 +316,67 @@ class GridGraph(_Handle):
         self.shape = (nz, ny, nx)
 
 
+class GridBatch(_Handle):
+    """Independent 3D grids of one connectivity solved together (cgp_set_grid_batch / cgp_solve_batch)."""
+
+    def set_batch(self, grids, conn=6):
+        """grids: sequence of (cap_source, cap_sink, cap_n[, cap_n_back]) with (nz, ny, nx) volumes."""
+        keep = []
+        arr = (GridInstance * max(len(grids), 1))()
+        shapes = []
+        for i, g in enumerate(grids):
+            cs, ct, cn = (_i32(a) for a in g[:3])
+            cb = _i32(g[3]) if len(g) > 3 and g[3] is not None else None
+            if cs.ndim != 3 or ct.shape != cs.shape or cn.shape != (conn // 2,) + cs.shape or \
+                    (cb is not None and cb.shape != cn.shape):
+                raise CgpError(CGP_ERR_INVALID, "grid %d: capacity shapes do not match the volume" % i)
+            nz, ny, nx = cs.shape
+            keep += [cs, ct, cn, cb]
+            arr[i] = GridInstance(nx, ny, nz, 0, cs.ctypes.data, ct.ctypes.data, cn.ctypes.data,
+                                  cb.ctypes.data if cb is not None else None)
+            shapes.append((nz, ny, nx))
+        self._check(_lib.cgp_set_grid_batch(self._h, len(grids), conn, arr, 0))
+        self.shapes = shapes
+        self._n = sum(a * b * c for a, b, c in shapes)
+
+    def set_batch_device(self, shapes, conn, pointers):
+        """pointers: per grid (cs_ptr, ct_ptr, cn_ptr[, cnb_ptr]) raw device pointers."""
+        arr = (GridInstance * max(len(shapes), 1))()
+        for i, ((nz, ny, nx), ptrs) in enumerate(zip(shapes, pointers)):
+            cnb = ptrs[3] if len(ptrs) > 3 else None
+            arr[i] = GridInstance(nx, ny, nz, 0, ptrs[0], ptrs[1], ptrs[2], cnb)
+        self._check(_lib.cgp_set_grid_batch(self._h, len(shapes), conn, arr, 1))
+        self.shapes = [tuple(s) for s in shapes]
+        self._n = sum(a * b * c for a, b, c in self.shapes)
+
+    def solve(self, policy=None):
+        """Per-grid exact max-flow values (list of int). Stats of the solve land in self.stats."""
+        if policy is not None:
+            self.set_policy(policy)
+        flows = (ctypes.c_int64 * max(len(self.shapes), 1))()
+        st = Stats()
+        self._check(_lib.cgp_solve_batch(self._h, flows, ctypes.byref(st)))
+        self.stats = st
+        return [int(flows[i]) for i in range(len(self.shapes))]
+
+    def _shape_of(self, index):
+        if not 0 <= index < len(self.shapes):
+            raise CgpError(CGP_ERR_INVALID, "grid index %d out of range (batch of %d)" % (index, len(self.shapes)))
+        return self.shapes[index]
+
+    def batch_source_set(self, index):
+        """Canonical minimum cut S_min of grid `index`, 0/1 bytes shaped like its volume."""
+        out = np.empty(self._shape_of(index), np.uint8)
+        self._check(_lib.cgp_batch_source_set(self._h, index, out.ctypes.data_as(_u8p)))
+        return out
+
+    def batch_fill_segments(self, index, mask=255, background=0):
+        """T_min of grid `index` (fill_segments semantics), shaped like its volume."""
+        out = np.empty(self._shape_of(index), np.uint8)
+        self._check(_lib.cgp_batch_fill_segments(self._h, index, out.ctypes.data_as(_u8p), mask, background))
+        return out
+
+
 def generate_volume(nx, ny, nz, seed, instance=0):
     """Synthetic blob volume, uint8 (nz, ny, nx)."""
     out = np.empty((nz, ny, nx), np.uint8)
@@ -323,6 +397,30 @@ def caps_from_volume(volume, conn=6):
     if rc != CGP_OK:
         raise CgpError(rc, "cgp_caps_from_volume failed")
     return cs, ct, cn
+
+
+def _splitmix64(x):
+    x = (x + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+    z = x
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+    return z ^ (z >> 31)
+
+
+def batch_shapes(count, seed=0, lo=32, hi=128):
+    """Shapes (nz, ny, nx) of the batch workload: each edge length log-uniform in [lo, hi], drawn
+    per axis and per grid from SplitMix64 counters of (seed, grid, axis)."""
+    import math
+    key = _splitmix64(_splitmix64(seed) ^ 0xB5AD4ECEDA1CE2A9)
+    shapes = []
+    for i in range(count):
+        dims = []
+        for axis in range(3):
+            u = (_splitmix64(key + 3 * i + axis) >> 11) / float(1 << 53)
+            dims.append(int(min(hi, max(lo, round(lo * math.exp(u * math.log(hi / lo)))))))
+        nx, ny, nz = dims
+        shapes.append((nz, ny, nx))
+    return shapes
 
 
 def make_instance(nx, ny, nz, seed, instance=0, conn=6):

@@ -4,6 +4,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "common.h"
 #include "solver.h"
@@ -49,6 +50,25 @@ int guarded(cgp_graph g, Fn &&fn)
 		g->last_error = e.what();
 		return CGP_ERR_DEVICE;
 	}
+}
+
+cgp::GridSolver &grid_of(cgp_graph g)
+{
+	if (g->mode != cgp_graph_s::Mode::Grid || !g->grid) {
+		cgp::fail(CGP_ERR_STATE, "no grid set: call cgp_set_grid or cgp_set_grid_batch first");
+	}
+	return *g->grid;
+}
+
+// The grid of a single-grid handle; a batch must name the grid it means.
+cgp::GridSolver &single_grid(cgp_graph g)
+{
+	cgp::GridSolver &grid = grid_of(g);
+	if (grid.instance_count() != 1) {
+		cgp::fail(CGP_ERR_INVALID, "the handle holds a batch of " + std::to_string(grid.instance_count()) +
+		                               " grids: use the cgp_batch_* accessors");
+	}
+	return grid;
 }
 
 cgp::CsrSolver &generic(cgp_graph g)
@@ -183,6 +203,70 @@ int cgp_set_grid_device(cgp_graph g, int32_t nx, int32_t ny, int32_t nz, int32_t
 	return set_grid_impl(g, nx, ny, nz, conn, cap_source, cap_sink, cap_n, cap_n_back, true);
 }
 
+int cgp_set_grid_batch(cgp_graph g, int32_t count, int32_t conn, const cgp_grid_instance *grids,
+                       int32_t device_pointers)
+{
+	return guarded(g, [&] {
+		if (count <= 0 || !grids) {
+			cgp::fail(CGP_ERR_INVALID, "a batch needs at least one grid");
+		}
+		std::vector<cgp::GridSpec> specs((size_t)count);
+		for (int32_t i = 0; i < count; ++i) {
+			const cgp_grid_instance &in = grids[i];
+			specs[(size_t)i] = cgp::GridSpec{in.nx, in.ny, in.nz, in.cap_source, in.cap_sink, in.cap_n, in.cap_n_back};
+		}
+		g->csr.reset();
+		if (!g->grid) {
+			g->grid.reset(new cgp::GridSolver(g->stream));
+		}
+		g->mode = cgp_graph_s::Mode::Grid;
+		g->grid->set_batch(conn, specs.data(), count, device_pointers != 0);
+	});
+}
+
+int cgp_batch_size(cgp_graph g, int32_t *count)
+{
+	return guarded(g, [&] {
+		if (!count) {
+			cgp::fail(CGP_ERR_INVALID, "output pointer is null");
+		}
+		*count = grid_of(g).instance_count();
+	});
+}
+
+int cgp_solve_batch(cgp_graph g, int64_t *flows, cgp_stats *stats)
+{
+	return guarded(g, [&] {
+		cgp::GridSolver &grid = grid_of(g);
+		grid.set_profiling(g->profiling);
+		grid.solve(g->policy, stats, g->trace);
+		if (flows) {
+			const std::vector<int64_t> &f = grid.flows();
+			std::memcpy(flows, f.data(), f.size() * sizeof(int64_t));
+		}
+	});
+}
+
+int cgp_batch_source_set(cgp_graph g, int32_t index, uint8_t *out)
+{
+	return guarded(g, [&] {
+		if (!out) {
+			cgp::fail(CGP_ERR_INVALID, "output pointer is null");
+		}
+		grid_of(g).source_side(index, out);
+	});
+}
+
+int cgp_batch_fill_segments(cgp_graph g, int32_t index, uint8_t *out, uint8_t mask, uint8_t background)
+{
+	return guarded(g, [&] {
+		if (!out) {
+			cgp::fail(CGP_ERR_INVALID, "output pointer is null");
+		}
+		grid_of(g).sink_side(index, out, mask, background);
+	});
+}
+
 int cgp_set_policy(cgp_graph g, const cgp_policy *policy)
 {
 	return guarded(g, [&] {
@@ -248,7 +332,7 @@ int cgp_fill_segments(cgp_graph g, uint8_t *out, uint8_t mask, uint8_t backgroun
 			cgp::fail(CGP_ERR_INVALID, "output pointer is null");
 		}
 		if (g->mode == cgp_graph_s::Mode::Grid) {
-			g->grid->sink_side(out, mask, background);
+			single_grid(g).sink_side(0, out, mask, background);
 		} else {
 			generic(g).sink_side(out, mask, background);
 		}
@@ -262,7 +346,7 @@ int cgp_source_set(cgp_graph g, uint8_t *out)
 			cgp::fail(CGP_ERR_INVALID, "output pointer is null");
 		}
 		if (g->mode == cgp_graph_s::Mode::Grid) {
-			g->grid->source_side(out);
+			single_grid(g).source_side(0, out);
 		} else {
 			generic(g).source_side(out);
 		}
@@ -276,7 +360,7 @@ int cgp_labels(cgp_graph g, int32_t *out)
 			cgp::fail(CGP_ERR_INVALID, "output pointer is null");
 		}
 		if (g->mode == cgp_graph_s::Mode::Grid) {
-			g->grid->labels(out);
+			single_grid(g).labels(0, out);
 		} else {
 			generic(g).labels(out);
 		}

@@ -27,54 +27,69 @@ namespace cgp {
 
 // ---------------------------------------------------------------------------------------------
 // init: residual planes, cancellation, excess, residual sink link. Sets status bit 0 for a negative
-// capacity and bit 1 when a voxel's inflow or an edge's capacity pair does not fit int32.
-template <int kConn>
-__global__ __launch_bounds__(256) void k_grid_init(GridView g)
+// capacity and bit 1 when a voxel's inflow or an edge's capacity pair does not fit int32. One
+// workgroup per tile (grid-stride over the tiles of every instance), thread (lx, ly) owning a
+// z-column, so x stays the coalesced axis.
+template <int kConn, int kTX, int kTY, int kTZ>
+__global__ __launch_bounds__(kTX *kTY) void k_grid_init(GridView gv, int ntiles)
 {
+	using T = Tile<kConn, kTX, kTY, kTZ>;
 	using D = Dirs<kConn>;
 	constexpr int K = D::K, H = D::H;
-	__shared__ long long red[4];
-	long long cancelled = 0;
-	bool negative = false, overflow = false;
-	const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-	for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.nvox; v += stride) {
-		const int x = (int)(v % g.nx);
-		const int y = (int)((v / g.nx) % g.ny);
-		const int z = (int)(v / ((int64_t)g.nx * g.ny));
-		const int32_t cs = g.cs[v], ct = g.ct[v];
-		negative |= (cs < 0) | (ct < 0);
-		const int32_t m = cs < ct ? cs : ct;
-		cancelled += m;
-		long long incoming = cs - m;
+	__shared__ long long red[T::NT / 64];
+	const int tid = threadIdx.x;
+	const int lx = tid % T::TX, ly = tid / T::TX;
+	for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+		int local;
+		const GridView g = instance_view(gv, tile, local);
+		int tx, ty, tz;
+		tile_coords(g, local, tx, ty, tz);
+		const int x = tx * T::TX + lx, y = ty * T::TY + ly;
+		long long cancelled = 0;
+		bool negative = false, overflow = false;
 #pragma unroll
-		for (int d = 0; d < K; ++d) {
-			const int ux = x + D::dx(d), uy = y + D::dy(d), uz = z + D::dz(d);
-			int32_t out = 0, in = 0;
-			if (inside(g, ux, uy, uz)) {
-				if (d < H) {
-					out = g.cn[(int64_t)d * g.nvox + v];
-					in = g.cnb ? g.cnb[(int64_t)d * g.nvox + v] : out;
-				} else {
-					const int64_t u = voxel_index(g, ux, uy, uz);
-					in = g.cn[(int64_t)(d - H) * g.nvox + u];
-					out = g.cnb ? g.cnb[(int64_t)(d - H) * g.nvox + u] : in;
-				}
-				negative |= (out < 0) | (in < 0);
-				overflow |= ((long long)out + (long long)in) > 0x7fffffffLL;
-				incoming += in;
+		for (int k = 0; k < T::TZ; ++k) {
+			const int z = tz * T::TZ + k;
+			if (x >= g.nx || y >= g.ny || z >= g.nz) {
+				continue;
 			}
-			g.r[(int64_t)d * g.nvox + v] = out;
+			const int64_t v = voxel_index(g, x, y, z);
+			const int32_t cs = g.cs[v], ct = g.ct[v];
+			negative |= (cs < 0) | (ct < 0);
+			const int32_t m = cs < ct ? cs : ct;
+			cancelled += m;
+			long long incoming = cs - m;
+#pragma unroll
+			for (int d = 0; d < K; ++d) {
+				const int ux = x + D::dx(d), uy = y + D::dy(d), uz = z + D::dz(d);
+				int32_t out = 0, in = 0;
+				if (inside(g, ux, uy, uz)) {
+					if (d < H) {
+						out = g.cn[(int64_t)d * g.nvox + v];
+						in = g.cnb ? g.cnb[(int64_t)d * g.nvox + v] : out;
+					} else {
+						const int64_t u = voxel_index(g, ux, uy, uz);
+						in = g.cn[(int64_t)(d - H) * g.nvox + u];
+						out = g.cnb ? g.cnb[(int64_t)(d - H) * g.nvox + u] : in;
+					}
+					negative |= (out < 0) | (in < 0);
+					overflow |= ((long long)out + (long long)in) > 0x7fffffffLL;
+					incoming += in;
+				}
+				g.r[(int64_t)d * g.nvox + v] = out;
+			}
+			overflow |= incoming > 0x7fffffffLL;
+			g.e[v] = cs - m;
+			g.rt[v] = ct - m;
 		}
-		overflow |= incoming > 0x7fffffffLL;
-		g.e[v] = cs - m;
-		g.rt[v] = ct - m;
-	}
-	if (negative || overflow) {
-		atomicOr(g.status, (negative ? 1 : 0) | (overflow ? 2 : 0));
-	}
-	const long long total = block_sum_i64<256>(cancelled, red);
-	if (threadIdx.x == 0 && total) {
-		atomicAdd(g.flow, (unsigned long long)total);
+		if (negative || overflow) {
+			atomicOr(g.status, (negative ? 1 : 0) | (overflow ? 2 : 0));
+		}
+		const long long total = block_sum_i64<T::NT>(cancelled, red);
+		if (tid == 0 && total) {
+			atomicAdd(g.flow, (unsigned long long)total);
+		}
+		__syncthreads();  // red is reused by the next tile
 	}
 }
 
@@ -102,7 +117,7 @@ __device__ inline void mark_neighbour_tile(const GridView &g, uint8_t *flags, in
 	const int dx = slot % 3 - 1, dy = (slot / 3) % 3 - 1, dz = slot / 9 - 1;
 	const int nx = tx + dx, ny = ty + dy, nz = tz + dz;
 	if (nx >= 0 && nx < g.ntx && ny >= 0 && ny < g.nty && nz >= 0 && nz < g.ntz) {
-		flags[(nz * g.nty + ny) * g.ntx + nx] = 1;
+		flags[g.tbase + (nz * g.nty + ny) * g.ntx + nx] = 1;
 	}
 }
 
@@ -172,7 +187,7 @@ __device__ inline int leaving_slot(int lx, int ly, int lz, int d)
 // that neighbour's tile in dirty_next, so a pass that marks nothing ends the relabel. Every
 // processed tile refreshes its active flag (some voxel with excess and a finite label).
 template <int kConn, int kTX, int kTY, int kTZ>
-__global__ __launch_bounds__(kTX *kTY) void k_grid_bfs(GridView g, const int32_t *__restrict__ list,
+__global__ __launch_bounds__(kTX *kTY) void k_grid_bfs(GridView gv, const int32_t *__restrict__ list,
                                                        const int32_t *__restrict__ count, int32_t threshold,
                                                        uint8_t *__restrict__ dirty_next,
                                                        uint8_t *__restrict__ tile_active)
@@ -182,8 +197,10 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_bfs(GridView g, const int32_t
 		return;
 	}
 	const int tile = list[blockIdx.x];
+	int local;
+	const GridView g = instance_view(gv, tile, local);
 	int tx, ty, tz;
-	tile_coords(g, tile, tx, ty, tz);
+	tile_coords(g, local, tx, ty, tz);
 	const int ox = tx * T::TX, oy = ty * T::TY, oz = tz * T::TZ;
 
 	__shared__ int32_t hl[T::NP];
@@ -258,7 +275,7 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_bfs(GridView g, const int32_t
 // Discharge one tile: repeated {region relabel, top-down level push} in LDS, then write back.
 // Runs only on tiles of one colour at a time, so no neighbouring tile is running.
 template <int kConn, int kTX, int kTY, int kTZ>
-__global__ __launch_bounds__(kTX *kTY) void k_grid_discharge(GridView g, const int32_t *__restrict__ list,
+__global__ __launch_bounds__(kTX *kTY) void k_grid_discharge(GridView gv, const int32_t *__restrict__ list,
                                                              const int32_t *__restrict__ count, int max_iters,
                                                              uint8_t *__restrict__ tile_active)
 {
@@ -269,8 +286,10 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_discharge(GridView g, const i
 		return;
 	}
 	const int tile = list[blockIdx.x];
+	int local;
+	const GridView g = instance_view(gv, tile, local);
 	int tx, ty, tz;
-	tile_coords(g, tile, tx, ty, tz);
+	tile_coords(g, local, tx, ty, tz);
 	const int ox = tx * T::TX, oy = ty * T::TY, oz = tz * T::TZ;
 
 	__shared__ int32_t hl[T::NP];
@@ -487,7 +506,7 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_discharge(GridView g, const i
 // may set bits in each other's voxels. first_pass seeds from the excess and records the
 // open-direction bits of every voxel.
 template <int kConn, int kTX, int kTY, int kTZ>
-__global__ __launch_bounds__(kTX *kTY) void k_grid_source_closure(GridView g, const int32_t *__restrict__ list,
+__global__ __launch_bounds__(kTX *kTY) void k_grid_source_closure(GridView gv, const int32_t *__restrict__ list,
                                                                   const int32_t *__restrict__ count, int first_pass,
                                                                   uint8_t *__restrict__ dirty_next)
 {
@@ -497,8 +516,10 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_source_closure(GridView g, co
 		return;
 	}
 	const int tile = list[blockIdx.x];
+	int local;
+	const GridView g = instance_view(gv, tile, local);
 	int tx, ty, tz;
-	tile_coords(g, tile, tx, ty, tz);
+	tile_coords(g, local, tx, ty, tz);
 	const int ox = tx * T::TX, oy = ty * T::TY, oz = tz * T::TZ;
 
 	__shared__ uint8_t sl[T::NV];
@@ -607,21 +628,14 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_source_closure(GridView g, co
 
 // ---------------------------------------------------------------------------------------------
 // Compact the indices of flagged tiles (optionally of one colour) with a wave64 ballot and one
-// atomic per wave. colour_mode 2: (tx+ty+tz) & 1; 8: (tx&1) | (ty&1)<<1 | (tz&1)<<2.
-__global__ __launch_bounds__(256) void k_compact_tiles(const uint8_t *__restrict__ flags, int ntiles, int ntx,
-                                                       int nty, int colour_mode, int colour,
+// atomic per wave. colours[t] is the colour of tile t inside its instance: 6-connected
+// (tx+ty+tz) & 1, 26-connected (tx&1) | (ty&1)<<1 | (tz&1)<<2.
+__global__ __launch_bounds__(256) void k_compact_tiles(const uint8_t *__restrict__ flags, int ntiles,
+                                                       const uint8_t *__restrict__ colours, int colour,
                                                        int32_t *__restrict__ list, int32_t *__restrict__ count)
 {
 	const int t = blockIdx.x * blockDim.x + threadIdx.x;
-	bool pred = false;
-	if (t < ntiles && flags[t]) {
-		pred = true;
-		if (colour >= 0) {
-			const int tx = t % ntx, ty = (t / ntx) % nty, tz = t / (ntx * nty);
-			const int c = colour_mode == 2 ? ((tx + ty + tz) & 1) : ((tx & 1) | ((ty & 1) << 1) | ((tz & 1) << 2));
-			pred = c == colour;
-		}
-	}
+	const bool pred = t < ntiles && flags[t] && (colour < 0 || colours[t] == colour);
 	const unsigned long long ballot = __ballot(pred);
 	const int lane = threadIdx.x & 63;
 	int base = 0;

@@ -146,6 +146,9 @@ GridSolver::Shape GridSolver::pick_shape(const cgp_policy &policy)
 	const bool wave_shape = (!policy.tile_x && !policy.tile_y && !policy.tile_z) ||
 	                           (policy.tile_x == wave::TX && policy.tile_y == wave::TY && policy.tile_z == wave::TZ);
 	if (policy.engine == CGP_ENGINE_WAVE) {
+		if (mInst.size() != 1) {
+			fail(CGP_ERR_INVALID, "the wave engine solves single grids, not batches");
+		}
 		if (mConn != 6 || !wave_shape) {
 			fail(CGP_ERR_INVALID, "the wave engine handles 6-connected grids with 32x8x4 tiles only");
 		}
@@ -169,83 +172,157 @@ GridSolver::Shape GridSolver::pick_shape(const cgp_policy &policy)
 void GridSolver::set_grid(int nx, int ny, int nz, int conn, const int32_t *cs, const int32_t *ct, const int32_t *cn,
                           const int32_t *cnb, bool device_pointers)
 {
-	if (nx <= 0 || ny <= 0 || nz <= 0) {
-		fail(CGP_ERR_INVALID, "grid dimensions must be positive");
+	const GridSpec spec{nx, ny, nz, cs, ct, cn, cnb};
+	set_batch(conn, &spec, 1, device_pointers);
+}
+
+void GridSolver::set_batch(int conn, const GridSpec *specs, int count, bool device_pointers)
+{
+	if (count <= 0 || !specs) {
+		fail(CGP_ERR_INVALID, "a batch needs at least one grid");
 	}
 	if (conn != 6 && conn != 26) {
 		fail(CGP_ERR_INVALID, "connectivity must be 6 or 26");
 	}
-	if (!cs || !ct || !cn) {
-		fail(CGP_ERR_INVALID, "capacity pointers must not be null");
-	}
-	const int64_t vox = (int64_t)nx * ny * nz;
-	if (vox >= (int64_t)1 << 31) {
-		fail(CGP_ERR_INVALID, "grids of 2^31 voxels or more are not supported");
+	std::vector<InstanceDesc> inst((size_t)count);
+	int64_t total = 0;
+	bool any_back = false;
+	for (int i = 0; i < count; ++i) {
+		const GridSpec &sp = specs[i];
+		const std::string which = count > 1 ? "grid " + std::to_string(i) + ": " : std::string();
+		if (sp.nx <= 0 || sp.ny <= 0 || sp.nz <= 0) {
+			fail(CGP_ERR_INVALID, which + "grid dimensions must be positive");
+		}
+		if (!sp.cs || !sp.ct || !sp.cn) {
+			fail(CGP_ERR_INVALID, which + "capacity pointers must not be null");
+		}
+		const int64_t vox = (int64_t)sp.nx * sp.ny * sp.nz;
+		if (vox >= (int64_t)1 << 31) {
+			fail(CGP_ERR_INVALID, which + "grids of 2^31 voxels or more are not supported");
+		}
+		InstanceDesc &d = inst[(size_t)i];
+		std::memset(&d, 0, sizeof(d));
+		d.nx = sp.nx;
+		d.ny = sp.ny;
+		d.nz = sp.nz;
+		d.conn = conn;
+		d.vbase = total;
+		d.nvox = vox;
+		total += vox;
+		any_back |= sp.cnb != nullptr;
 	}
 	const int half = conn / 2;
 	mSolved = false;
-	mNx = nx;
-	mNy = ny;
-	mNz = nz;
+	mInst = inst;
+	mFlows.assign((size_t)count, 0);
+	mNx = inst[0].nx;
+	mNy = inst[0].ny;
+	mNz = inst[0].nz;
 	mConn = conn;
-	mVox = vox;
-	mHasBack = cnb != nullptr;
-	mCs.allocate(vox);
-	mCt.allocate(vox);
-	mCn.allocate((size_t)vox * half);
+	mVox = total;
+	mHasBack = any_back;
+	mCs.allocate(total);
+	mCt.allocate(total);
+	mCn.allocate((size_t)total * half);
 	if (mHasBack) {
-		mCnb.allocate((size_t)vox * half);
+		mCnb.allocate((size_t)total * half);
 	} else {
 		mCnb.release();
 	}
 	const hipMemcpyKind kind = device_pointers ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
-	CGP_HIP(hipMemcpyAsync(mCs.get(), cs, mCs.bytes(), kind, mStream));
-	CGP_HIP(hipMemcpyAsync(mCt.get(), ct, mCt.bytes(), kind, mStream));
-	CGP_HIP(hipMemcpyAsync(mCn.get(), cn, mCn.bytes(), kind, mStream));
-	if (mHasBack) {
-		CGP_HIP(hipMemcpyAsync(mCnb.get(), cnb, mCnb.bytes(), kind, mStream));
+	for (int i = 0; i < count; ++i) {
+		const GridSpec &sp = specs[i];
+		const InstanceDesc &d = inst[(size_t)i];
+		const size_t vb = (size_t)d.nvox * sizeof(int32_t);
+		CGP_HIP(hipMemcpyAsync(mCs.get() + d.vbase, sp.cs, vb, kind, mStream));
+		CGP_HIP(hipMemcpyAsync(mCt.get() + d.vbase, sp.ct, vb, kind, mStream));
+		CGP_HIP(hipMemcpyAsync(mCn.get() + d.vbase * half, sp.cn, vb * half, kind, mStream));
+		if (mHasBack) {
+			// a grid without back capacities is symmetric: its back planes are its forward planes
+			CGP_HIP(hipMemcpyAsync(mCnb.get() + d.vbase * half, sp.cnb ? sp.cnb : sp.cn, vb * half, kind, mStream));
+		}
 	}
-	mE.allocate(vox);
-	mRt.allocate(vox);
-	mH.allocate(vox);
-	mR.allocate((size_t)vox * conn);
-	mS.allocate(vox);
-	mOmask.allocate(vox);
-	mFlow.allocate(1);
+	mE.allocate(total);
+	mRt.allocate(total);
+	mH.allocate(total);
+	mR.allocate((size_t)total * conn);
+	mS.allocate(total);
+	mOmask.allocate(total);
+	mFlow.allocate(count);
 	mCounters.allocate(4);
 	mStatus.allocate(1);
 	mCount.allocate(1);
-	mHostScratch.allocate(8);
+	mHostScratch.allocate(count > 8 ? count : 8);
 	CGP_HIP(hipStreamSynchronize(mStream));
+}
+
+const InstanceDesc &GridSolver::instance(int index) const
+{
+	if (index < 0 || index >= (int)mInst.size()) {
+		fail(CGP_ERR_INVALID, "grid index " + std::to_string(index) + " out of range (batch of " +
+		                          std::to_string(mInst.size()) + ")");
+	}
+	return mInst[(size_t)index];
 }
 
 void GridSolver::configure_tiles(const Shape &shape)
 {
 	mShape = shape;
-	mNtx = (mNx + shape.tx - 1) / shape.tx;
-	mNty = (mNy + shape.ty - 1) / shape.ty;
-	mNtz = (mNz + shape.tz - 1) / shape.tz;
-	mTiles = mNtx * mNty * mNtz;
+	const int ncolours = mConn == 6 ? 2 : 8;
+	std::vector<int> per_colour(ncolours, 0);
+	int64_t tiles = 0;
+	for (InstanceDesc &d : mInst) {
+		d.ntx = (d.nx + shape.tx - 1) / shape.tx;
+		d.nty = (d.ny + shape.ty - 1) / shape.ty;
+		d.ntz = (d.nz + shape.tz - 1) / shape.tz;
+		d.tbase = (int32_t)tiles;
+		tiles += (int64_t)d.ntx * d.nty * d.ntz;
+		if (tiles >= (int64_t)1 << 31) {
+			fail(CGP_ERR_INVALID, "a batch may hold at most 2^31 - 1 tiles");
+		}
+	}
+	mNtx = mInst[0].ntx;
+	mNty = mInst[0].nty;
+	mNtz = mInst[0].ntz;
+	mTiles = (int)tiles;
+	std::vector<uint8_t> colour((size_t)mTiles);
+	std::vector<int32_t> owner(mInst.size() > 1 ? (size_t)mTiles : 0);
+	for (size_t i = 0; i < mInst.size(); ++i) {
+		const InstanceDesc &d = mInst[i];
+		const int n = d.ntx * d.nty * d.ntz;
+		for (int t = 0; t < n; ++t) {
+			const int tx = t % d.ntx, ty = (t / d.ntx) % d.nty, tz = t / (d.ntx * d.nty);
+			const int c = mConn == 6 ? ((tx + ty + tz) & 1) : ((tx & 1) | ((ty & 1) << 1) | ((tz & 1) << 2));
+			colour[(size_t)(d.tbase + t)] = (uint8_t)c;
+			++per_colour[c];
+			if (!owner.empty()) {
+				owner[(size_t)(d.tbase + t)] = (int32_t)i;
+			}
+		}
+	}
+	// largest colour class: the grid of a discharge phase must cover every tile of one colour
+	mMaxColourTiles = *std::max_element(per_colour.begin(), per_colour.end());
 	mActive.allocate(mTiles);
 	mDirtyA.allocate(mTiles);
 	mDirtyB.allocate(mTiles);
 	mList.allocate(mTiles);
 	mAllTiles.allocate(mTiles + 1);
+	mTileColour.allocate(mTiles);
 	std::vector<int32_t> all(mTiles + 1);
 	for (int t = 0; t < mTiles; ++t) {
 		all[t] = t;
 	}
 	all[mTiles] = mTiles;  // count slot for the all-tiles list
-	// largest colour class: the grid of a discharge phase must cover every tile of one colour
-	const int ncolours = mConn == 6 ? 2 : 8;
-	std::vector<int> per_colour(ncolours, 0);
-	for (int t = 0; t < mTiles; ++t) {
-		const int tx = t % mNtx, ty = (t / mNtx) % mNty, tz = t / (mNtx * mNty);
-		const int c = mConn == 6 ? ((tx + ty + tz) & 1) : ((tx & 1) | ((ty & 1) << 1) | ((tz & 1) << 2));
-		++per_colour[c];
-	}
-	mMaxColourTiles = *std::max_element(per_colour.begin(), per_colour.end());
 	CGP_HIP(hipMemcpyAsync(mAllTiles.get(), all.data(), all.size() * sizeof(int32_t), hipMemcpyHostToDevice, mStream));
+	CGP_HIP(hipMemcpyAsync(mTileColour.get(), colour.data(), colour.size(), hipMemcpyHostToDevice, mStream));
+	if (mInst.size() > 1) {
+		mInstDev.allocate(mInst.size());
+		mTileInst.allocate(mTiles);
+		CGP_HIP(hipMemcpyAsync(mInstDev.get(), mInst.data(), mInst.size() * sizeof(InstanceDesc),
+		                       hipMemcpyHostToDevice, mStream));
+		CGP_HIP(hipMemcpyAsync(mTileInst.get(), owner.data(), owner.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+		                       mStream));
+	}
 	if (mWave) {
 		mBits.allocate((size_t)wave::kPlanes * mTiles * wave::R);
 		mSBits.allocate((size_t)mTiles * wave::R);
@@ -286,17 +363,29 @@ static GridView make_view(int nx, int ny, int nz, int ntx, int nty, int ntz, int
 	g.flow = flow.get();
 	g.status = status.get();
 	g.counters = counters.get();
+	g.inst = nullptr;
+	g.tile_inst = nullptr;
+	g.tbase = 0;
 	return g;
 }
 
-#define CGP_VIEW()                                                                                                 \
-	make_view(mNx, mNy, mNz, mNtx, mNty, mNtz, mVox, mE, mRt, mH, mR, mS, mOmask, mCs, mCt, mCn, mCnb, mFlow, mStatus, \
-	          mCounters, mFaces, mReach)
+#define CGP_VIEW() batch_view(make_view(mNx, mNy, mNz, mNtx, mNty, mNtz, mVox, mE, mRt, mH, mR, mS, mOmask, mCs, \
+                                       mCt, mCn, mCnb, mFlow, mStatus, mCounters, mFaces, mReach),              \
+                             mInst.size() > 1 ? mInstDev.get() : nullptr, mTileInst.get())
+
+// A batch's kernels find each tile's instance through the instance table.
+static GridView batch_view(GridView g, const InstanceDesc *inst, const int32_t *tile_inst)
+{
+	if (inst) {
+		g.inst = inst;
+		g.tile_inst = tile_inst;
+	}
+	return g;
+}
 
 void GridSolver::launch_init()
 {
 	GridView g = CGP_VIEW();
-	const int blocks = blocks_for(mVox, 256, 8192);
 	// reads cs, ct and the n-link planes, writes e, rt and the K residual planes
 	const double bytes = 4.0 * (2 + (mConn / 2) * (mHasBack ? 2 : 1)) + 4.0 * (2 + mConn);
 	if (mWave) {
@@ -306,11 +395,11 @@ void GridSolver::launch_init()
 		return;
 	}
 	launch(kInit, mVox, bytes, [&] {
-		if (mConn == 6) {
-			hipLaunchKernelGGL(k_grid_init<6>, dim3(blocks), dim3(256), 0, mStream, g);
-		} else {
-			hipLaunchKernelGGL(k_grid_init<26>, dim3(blocks), dim3(256), 0, mStream, g);
-		}
+		dispatch_shape(mConn, mShape.tx, mShape.ty, mShape.tz, [&](auto tag) {
+			using S = decltype(tag);
+			hipLaunchKernelGGL((k_grid_init<S::conn, S::tx, S::ty, S::tz>), dim3(std::min(mTiles, 8192)),
+			                   dim3(S::tx * S::ty), 0, mStream, g, mTiles);
+		});
 	});
 }
 
@@ -319,10 +408,9 @@ void GridSolver::launch_compact(const uint8_t *flags, int colour)
 	launch(kFill, 0, 0.0, [&] {
 		hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, mStream, mCount.get(), (int64_t)1, 0);
 	});
-	const int colour_mode = mConn == 6 ? 2 : 8;
 	launch(kCompact, 0, 0.0, [&] {
 		hipLaunchKernelGGL(k_compact_tiles, dim3(blocks_for(mTiles, 256, 1 << 20)), dim3(256), 0, mStream, flags,
-		                   mTiles, mNtx, mNty, colour_mode, colour, mList.get(), mCount.get());
+		                   mTiles, mTileColour.get(), colour, mList.get(), mCount.get());
 	});
 }
 
@@ -468,10 +556,16 @@ void GridSolver::run_source_closure(cgp_stats &stats)
 
 int64_t GridSolver::read_flow()
 {
+	const size_t n = mInst.size();
 	unsigned long long *pinned = reinterpret_cast<unsigned long long *>(mHostScratch.get());
-	CGP_HIP(hipMemcpyAsync(pinned, mFlow.get(), sizeof(unsigned long long), hipMemcpyDeviceToHost, mStream));
+	CGP_HIP(hipMemcpyAsync(pinned, mFlow.get(), n * sizeof(unsigned long long), hipMemcpyDeviceToHost, mStream));
 	CGP_HIP(hipStreamSynchronize(mStream));
-	return (int64_t)pinned[0];
+	int64_t total = 0;
+	for (size_t i = 0; i < n; ++i) {
+		mFlows[i] = (int64_t)pinned[i];
+		total += mFlows[i];
+	}
+	return total;
 }
 
 cgp_policy default_policy()
@@ -504,8 +598,8 @@ int64_t GridSolver::solve(const cgp_policy &policy, cgp_stats *stats_out, const 
 	}
 
 	hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, mStream, mStatus.get(), (int64_t)1, 0);
-	hipLaunchKernelGGL(k_fill_u8, dim3(1), dim3(64), 0, mStream, reinterpret_cast<uint8_t *>(mFlow.get()), (int64_t)8,
-	                   (uint8_t)0);
+	hipLaunchKernelGGL(k_fill_u8, dim3(1), dim3(256), 0, mStream, reinterpret_cast<uint8_t *>(mFlow.get()),
+	                   (int64_t)(8 * mInst.size()), (uint8_t)0);
 	hipLaunchKernelGGL(k_fill_u8, dim3(1), dim3(64), 0, mStream, reinterpret_cast<uint8_t *>(mCounters.get()),
 	                   (int64_t)32, (uint8_t)0);
 	launch_init();
@@ -582,45 +676,48 @@ int64_t GridSolver::solve(const cgp_policy &policy, cgp_stats *stats_out, const 
 	return flow;
 }
 
-void GridSolver::sink_side(uint8_t *host_out, uint8_t mask, uint8_t background)
+void GridSolver::sink_side(int index, uint8_t *host_out, uint8_t mask, uint8_t background)
 {
 	if (!mSolved) {
 		fail(CGP_ERR_STATE, "fill_segments before max_flow");
 	}
-	DeviceBuffer<uint8_t> tmp(mVox);
+	const InstanceDesc &d = instance(index);
+	DeviceBuffer<uint8_t> tmp(d.nvox);
 	if (mWave) {
 		GridView g = CGP_VIEW();
 		hipLaunchKernelGGL(wave::k_wave_plane_bytes, dim3(mTiles), dim3(64), 0, mStream, g, mReach.get(), tmp.get(),
 		                   mask, background);
 	} else {
-		hipLaunchKernelGGL(k_sink_side_mask, dim3(blocks_for(mVox, 256, 8192)), dim3(256), 0, mStream, mH.get(), mVox,
-		                   tmp.get(), mask, background);
+		hipLaunchKernelGGL(k_sink_side_mask, dim3(blocks_for(d.nvox, 256, 8192)), dim3(256), 0, mStream,
+		                   mH.get() + d.vbase, d.nvox, tmp.get(), mask, background);
 	}
 	CGP_LAUNCH_CHECK();
-	CGP_HIP(hipMemcpyAsync(host_out, tmp.get(), mVox, hipMemcpyDeviceToHost, mStream));
+	CGP_HIP(hipMemcpyAsync(host_out, tmp.get(), d.nvox, hipMemcpyDeviceToHost, mStream));
 	CGP_HIP(hipStreamSynchronize(mStream));
 }
 
-void GridSolver::source_side(uint8_t *host_out)
+void GridSolver::source_side(int index, uint8_t *host_out)
 {
 	if (!mSolved) {
 		fail(CGP_ERR_STATE, "source_set before max_flow");
 	}
+	const InstanceDesc &d = instance(index);
 	if (mWave) {
 		GridView g = CGP_VIEW();
 		hipLaunchKernelGGL(wave::k_wave_plane_bytes, dim3(mTiles), dim3(64), 0, mStream, g, mSBits.get(), mS.get(),
 		                   (uint8_t)1, (uint8_t)0);
 		CGP_LAUNCH_CHECK();
 	}
-	CGP_HIP(hipMemcpyAsync(host_out, mS.get(), mVox, hipMemcpyDeviceToHost, mStream));
+	CGP_HIP(hipMemcpyAsync(host_out, mS.get() + d.vbase, d.nvox, hipMemcpyDeviceToHost, mStream));
 	CGP_HIP(hipStreamSynchronize(mStream));
 }
 
-void GridSolver::labels(int32_t *host_out)
+void GridSolver::labels(int index, int32_t *host_out)
 {
 	if (!mSolved) {
 		fail(CGP_ERR_STATE, "labels before max_flow");
 	}
+	const InstanceDesc &d = instance(index);
 	if (mWave) {
 		// the face labels are final: one pass over every tile yields every voxel's exact label
 		GridView g = CGP_VIEW();
@@ -628,7 +725,7 @@ void GridSolver::labels(int32_t *host_out)
 		                   mAllTiles.get(), mAllTiles.get() + mTiles, 0, mDirtyA.get(), mActive.get(), mH.get());
 		CGP_LAUNCH_CHECK();
 	}
-	CGP_HIP(hipMemcpyAsync(host_out, mH.get(), mVox * sizeof(int32_t), hipMemcpyDeviceToHost, mStream));
+	CGP_HIP(hipMemcpyAsync(host_out, mH.get() + d.vbase, d.nvox * sizeof(int32_t), hipMemcpyDeviceToHost, mStream));
 	CGP_HIP(hipStreamSynchronize(mStream));
 }
 

@@ -61,6 +61,18 @@ struct Tile {
 	__host__ __device__ static constexpr int vidx(int x, int y, int z) { return (z * TY + y) * TX + x; }
 };
 
+// One grid of a batch. Per-voxel arrays of all instances are packed one after another: the
+// instance's voxels start at vbase, its residual planes at conn * vbase (plane stride nvox) and
+// its n-link planes at (conn / 2) * vbase. Tiles of all instances share one id space.
+struct InstanceDesc {
+	int32_t nx, ny, nz;
+	int32_t ntx, nty, ntz;
+	int32_t tbase;  // global id of the instance's first tile
+	int32_t conn;
+	int64_t vbase;
+	int64_t nvox;
+};
+
 // Everything a grid kernel needs, passed by value.
 struct GridView {
 	int32_t nx, ny, nz;
@@ -79,7 +91,44 @@ struct GridView {
 	unsigned long long *counters; // [0] discharge visits, [1] bfs visits, [2] cut visits
 	int32_t *faces;          // wave engine: labels of each tile's face voxels
 	uint32_t *reach;         // wave engine: "reaches t" bit plane of the last global relabel
+	const InstanceDesc *inst;  // batch: instance table (null for a single grid)
+	const int32_t *tile_inst;  // batch: instance of every global tile
+	int32_t tbase;             // global id of this view's first tile
 };
+
+// The view of the instance that owns global tile `tile`, and the tile's id inside it. For a single
+// grid this is the view itself.
+__device__ inline GridView instance_view(const GridView &g, int tile, int &local)
+{
+	if (!g.inst) {
+		local = tile;
+		return g;
+	}
+	const int i = g.tile_inst[tile];
+	const InstanceDesc d = g.inst[i];
+	GridView v = g;
+	v.nx = d.nx;
+	v.ny = d.ny;
+	v.nz = d.nz;
+	v.ntx = d.ntx;
+	v.nty = d.nty;
+	v.ntz = d.ntz;
+	v.nvox = d.nvox;
+	v.tbase = d.tbase;
+	v.e = g.e + d.vbase;
+	v.rt = g.rt + d.vbase;
+	v.h = g.h + d.vbase;
+	v.s = g.s + d.vbase;
+	v.omask = g.omask + d.vbase;
+	v.cs = g.cs + d.vbase;
+	v.ct = g.ct + d.vbase;
+	v.r = g.r + d.vbase * d.conn;
+	v.cn = g.cn + d.vbase * (d.conn / 2);
+	v.cnb = g.cnb ? g.cnb + d.vbase * (d.conn / 2) : nullptr;
+	v.flow = g.flow + i;
+	local = tile - d.tbase;
+	return v;
+}
 
 __device__ inline int64_t voxel_index(const GridView &g, int x, int y, int z)
 {

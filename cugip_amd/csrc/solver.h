@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "common.h"
+#include "grid_tile.h"
 
 namespace cgp {
 
@@ -21,17 +22,30 @@ struct TraceHook {
 
 cgp_policy default_policy();
 
-// 3D grid graph (6- or 26-connected), implicit edges, SoA int32 state in HBM.
+// One grid of a batch, as the caller passes it (layout of cgp_set_grid).
+struct GridSpec {
+	int nx, ny, nz;
+	const int32_t *cs, *ct, *cn, *cnb;
+};
+
+// 3D grid graphs (6- or 26-connected), implicit edges, SoA int32 state in HBM. Holds one grid or a
+// batch of independent grids of one connectivity, solved together: their tiles share the tile
+// lists of every phase, so one launch serves all instances.
 class GridSolver {
 public:
 	explicit GridSolver(hipStream_t stream) : mStream(stream) {}
 
 	void set_grid(int nx, int ny, int nz, int conn, const int32_t *cs, const int32_t *ct, const int32_t *cn,
 	              const int32_t *cnb, bool device_pointers);
+	void set_batch(int conn, const GridSpec *specs, int count, bool device_pointers);
+	// Returns the total flow of all instances; per-instance values in flows().
 	int64_t solve(const cgp_policy &policy, cgp_stats *stats, const TraceHook &trace);
-	void sink_side(uint8_t *host_out, uint8_t mask, uint8_t background);
-	void source_side(uint8_t *host_out);
-	void labels(int32_t *host_out);
+	const std::vector<int64_t> &flows() const { return mFlows; }
+	void sink_side(int index, uint8_t *host_out, uint8_t mask, uint8_t background);
+	void source_side(int index, uint8_t *host_out);
+	void labels(int index, int32_t *host_out);
+	int instance_count() const { return (int)mInst.size(); }
+	int64_t instance_voxels(int index) const { return mInst[index].nvox; }
 	int64_t vertex_count() const { return mVox; }
 	bool solved() const { return mSolved; }
 	void set_profiling(bool enable) { mProfile = enable; }
@@ -63,10 +77,16 @@ private:
 	void run_discharge_phase(int colour, int max_iters);
 	void run_source_closure(cgp_stats &stats);
 	int64_t read_flow();
+	const InstanceDesc &instance(int index) const;
 
 	hipStream_t mStream;
-	int mNx = 0, mNy = 0, mNz = 0, mConn = 0;
-	int64_t mVox = 0;
+	int mNx = 0, mNy = 0, mNz = 0, mConn = 0;  // dimensions of a single grid (instance 0)
+	int64_t mVox = 0;                          // voxels of all instances
+	std::vector<InstanceDesc> mInst;           // host copy of the instance table
+	std::vector<int64_t> mFlows;               // per-instance flow of the last solve
+	DeviceBuffer<InstanceDesc> mInstDev;       // batches only
+	DeviceBuffer<int32_t> mTileInst;           // batches only: instance of every tile
+	DeviceBuffer<uint8_t> mTileColour;         // colour of every tile inside its instance
 	Shape mShape{0, 0, 0};
 	int mNtx = 0, mNty = 0, mNtz = 0, mTiles = 0, mMaxColourTiles = 0;
 	bool mSolved = false;

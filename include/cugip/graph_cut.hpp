@@ -18,7 +18,8 @@
 //    additionally returns the canonical minimum cut S_min;
 //  * trace hooks receive std::vector<int> level starts instead of thrust::host_vector<int>, and a
 //    host-side GraphCutData snapshot instead of device pointers;
-//  * set_grid() is new: 3D grid graphs with implicit 6/26-connected edges (no CSR).
+//  * set_grid() is new: 3D grid graphs with implicit 6/26-connected edges (no CSR);
+//  * GridBatch is new: many independent grids solved together.
 #pragma once
 
 #include <cstdint>
@@ -415,6 +416,145 @@ private:
 	std::unique_ptr<cgp_graph_s, Deleter> mHandle;
 	int mVertexCount = 0;
 	int64_t mFlow = 0;
+	cgp_stats mStats{};
+};
+
+// Independent 3D grids of one connectivity, solved together in the same launches
+// (cgp_set_grid_batch / cgp_solve_batch). No counterpart in the reference, whose Graph<TFlow>
+// holds one graph; each grid has the layout of Graph<TFlow>::set_grid.
+template <typename TFlow>
+class GridBatch {
+public:
+	explicit GridBatch(int aConnectivity = 6) : mConnectivity(aConnectivity)
+	{
+		cgp_graph g = nullptr;
+		detail::check(nullptr, cgp_create(&g));
+		mHandle.reset(g);
+	}
+
+	// Copies the capacities; returns the grid's index in the batch.
+	int add_grid(int aNx, int aNy, int aNz, const TFlow *aCapSource, const TFlow *aCapSink, const TFlow *aCapN,
+	             const TFlow *aCapNBack = nullptr)
+	{
+		const size_t n = (size_t)aNx * aNy * aNz;
+		const size_t planes = (size_t)(mConnectivity / 2) * n;
+		Grid grid;
+		grid.nx = aNx;
+		grid.ny = aNy;
+		grid.nz = aNz;
+		grid.source = detail::to_capacities(aCapSource, n);
+		grid.sink = detail::to_capacities(aCapSink, n);
+		grid.n = detail::to_capacities(aCapN, planes);
+		if (aCapNBack) {
+			grid.back = detail::to_capacities(aCapNBack, planes);
+		}
+		mGrids.push_back(std::move(grid));
+		mUploaded = false;
+		return (int)mGrids.size() - 1;
+	}
+
+	int size() const { return (int)mGrids.size(); }
+
+	// Solves every grid; returns the per-grid max-flow values.
+	std::vector<TFlow> max_flow()
+	{
+		GraphCutPolicy policy;
+		return max_flow(policy);
+	}
+
+	template <typename TPolicy>
+	std::vector<TFlow> max_flow(TPolicy &aPolicy)
+	{
+		upload();
+		cgp_policy p;
+		detail::check(handle(), cgp_get_policy(handle(), &p));
+		p.minimal_residual = aPolicy.relabelPolicy.edgeTraversalCheck.minimalResidual;
+		p.tile_x = aPolicy.tileX;
+		p.tile_y = aPolicy.tileY;
+		p.tile_z = aPolicy.tileZ;
+		p.discharges_per_relabel = aPolicy.dischargesPerRelabel;
+		p.max_tile_iterations = aPolicy.maxTileIterations;
+		p.max_rounds = aPolicy.maxRounds;
+		p.engine = aPolicy.engine;
+		detail::check(handle(), cgp_set_policy(handle(), &p));
+		mFlows.assign(mGrids.size(), 0);
+		detail::check(handle(), cgp_solve_batch(handle(), mFlows.data(), &mStats));
+		std::vector<TFlow> result;
+		result.reserve(mFlows.size());
+		for (int64_t f : mFlows) {
+			result.push_back(detail::narrow_flow<TFlow>(f));
+		}
+		return result;
+	}
+
+	int64_t max_flow_value(int aIndex) const { return mFlows.at((size_t)aIndex); }
+	const cgp_stats &stats() const { return mStats; }
+
+	// Canonical minimum cut S_min of grid aIndex: 1 where the voxel is reachable from s.
+	template <typename THostArrayView>
+	void source_set(int aIndex, THostArrayView aVertices) const
+	{
+		std::vector<uint8_t> tmp(voxels(aIndex));
+		detail::check(handle(), cgp_batch_source_set(handle(), aIndex, tmp.data()));
+		for (size_t i = 0; i < tmp.size(); ++i) {
+			aVertices[i] = tmp[i];
+		}
+	}
+
+	// fill_segments of grid aIndex (voxels that still reach t).
+	template <typename THostArrayView>
+	void fill_segments(int aIndex, THostArrayView aVertices, uint8_t mMask = 255, uint8_t mBackground = 0) const
+	{
+		std::vector<uint8_t> tmp(voxels(aIndex));
+		detail::check(handle(), cgp_batch_fill_segments(handle(), aIndex, tmp.data(), mMask, mBackground));
+		for (size_t i = 0; i < tmp.size(); ++i) {
+			aVertices[i] = tmp[i];
+		}
+	}
+
+	cgp_graph handle() const { return mHandle.get(); }
+
+private:
+	struct Deleter {
+		void operator()(cgp_graph g) const { cgp_destroy(g); }
+	};
+	struct Grid {
+		int nx = 0, ny = 0, nz = 0;
+		std::vector<int32_t> source, sink, n, back;
+	};
+
+	size_t voxels(int aIndex) const
+	{
+		const Grid &g = mGrids.at((size_t)aIndex);
+		return (size_t)g.nx * g.ny * g.nz;
+	}
+
+	void upload()
+	{
+		if (mUploaded) {
+			return;
+		}
+		std::vector<cgp_grid_instance> desc(mGrids.size());
+		for (size_t i = 0; i < mGrids.size(); ++i) {
+			const Grid &g = mGrids[i];
+			desc[i].nx = g.nx;
+			desc[i].ny = g.ny;
+			desc[i].nz = g.nz;
+			desc[i].reserved0 = 0;
+			desc[i].cap_source = g.source.data();
+			desc[i].cap_sink = g.sink.data();
+			desc[i].cap_n = g.n.data();
+			desc[i].cap_n_back = g.back.empty() ? nullptr : g.back.data();
+		}
+		detail::check(handle(), cgp_set_grid_batch(handle(), (int32_t)desc.size(), mConnectivity, desc.data(), 0));
+		mUploaded = true;
+	}
+
+	std::unique_ptr<cgp_graph_s, Deleter> mHandle;
+	int mConnectivity;
+	std::vector<Grid> mGrids;
+	bool mUploaded = false;
+	std::vector<int64_t> mFlows;
 	cgp_stats mStats{};
 };
 

@@ -127,6 +127,31 @@ int cgp_set_grid_device(cgp_graph graph, int32_t nx, int32_t ny, int32_t nz, int
                         const int32_t *cap_source, const int32_t *cap_sink, const int32_t *cap_n,
                         const int32_t *cap_n_back);
 
+/* ---- batches of independent 3D grids -------------------------------------------------------
+   One handle holds `count` grids of one connectivity (each with the layout of cgp_set_grid) and
+   solves them together: the tiles of every grid share the tile lists of each phase, so a launch
+   serves all grids and small or uneven grids keep the device full. No counterpart in the
+   reference (its Graph<TFlow> holds one graph); the batched entry of the grid path. After
+   cgp_set_grid_batch, cgp_max_flow returns the total flow of all grids; cgp_fill_segments,
+   cgp_source_set and cgp_labels are refused unless count is 1. */
+typedef struct cgp_grid_instance {
+	int32_t nx, ny, nz;
+	int32_t reserved0;           /* 0 */
+	const int32_t *cap_source;
+	const int32_t *cap_sink;
+	const int32_t *cap_n;
+	const int32_t *cap_n_back;   /* NULL = symmetric */
+} cgp_grid_instance;
+/* device_pointers != 0: every capacity pointer is device memory of the handle's device. */
+int cgp_set_grid_batch(cgp_graph graph, int32_t count, int32_t conn, const cgp_grid_instance *grids,
+                       int32_t device_pointers);
+int cgp_batch_size(cgp_graph graph, int32_t *count);
+/* Solves every grid; flows (may be NULL) receives `count` exact values; stats cover the batch. */
+int cgp_solve_batch(cgp_graph graph, int64_t *flows, cgp_stats *stats);
+/* S_min and T_min (fill_segments semantics) of grid `index`, host buffers of its voxel count. */
+int cgp_batch_source_set(cgp_graph graph, int32_t index, uint8_t *out);
+int cgp_batch_fill_segments(cgp_graph graph, int32_t index, uint8_t *out, uint8_t mask, uint8_t background);
+
 /* ---- solving ----------------------------------------------------------------------------- */
 int cgp_set_policy(cgp_graph graph, const cgp_policy *policy);   /* NULL restores the defaults */
 int cgp_get_policy(cgp_graph graph, cgp_policy *policy);

@@ -22,7 +22,7 @@ def declared_functions():
 
 def test_every_declared_symbol_is_exported():
     names = declared_functions()
-    assert len(names) >= 19
+    assert len(names) >= 24
     out = subprocess.run(["nm", "-D", "--defined-only", cugip_amd.LIB_PATH], capture_output=True, text=True,
                          check=True).stdout
     exported = set(re.findall(r"\sT\s(cgp_[a-z0-9_]+)", out))
@@ -44,6 +44,9 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(cugip_amd.Policy) == 4 * 15
     # cgp_stats: 8 + 4*4 + 3*8 + 5*8 + 8 + 4*4 + 8*4
     assert ctypes.sizeof(cugip_amd.Stats) == 8 + 16 + 24 + 40 + 8 + 16 + 32
+    # cgp_grid_instance: 4 int32 + 4 pointers
+    assert ctypes.sizeof(cugip_amd.GridInstance) == 48
+    assert cugip_amd.GridInstance.cap_source.offset == 16 and cugip_amd.GridInstance.cap_n_back.offset == 40
 
 
 def test_no_device_is_an_error_not_a_crash():
@@ -65,6 +68,7 @@ def test_cpp_header_compiles_standalone(tmp_path):
         "int main() { cugip::GraphData<float> d; d.setVertexCount(2); d.addEdge(0, 1, 1.f, 2.f);\n"
         "  cugip::GraphCutPolicy p; p.relabelPolicy.edgeTraversalCheck.minimalResidual = 0.f;\n"
         "  static_assert(sizeof(cugip::EdgeRecord) == 8, \"layout\");\n"
+        "  static_assert(sizeof(cgp_grid_instance) == 48, \"layout\");\n"
         "  return d.edges[0].first == 0 && d.edges[0].second == 1 ? 0 : 1; }\n")
     exe = tmp_path / "use"
     subprocess.run(["g++", "-std=c++14", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), str(src), "-o",
