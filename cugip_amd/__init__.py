@@ -78,6 +78,20 @@ class GridInstance(ctypes.Structure):
                 ("cap_n", ctypes.c_void_p), ("cap_n_back", ctypes.c_void_p)]
 
 
+class MfiInfo(ctypes.Structure):
+    _fields_ = [("dimension", ctypes.c_int32), ("connectivity", ctypes.c_int32),
+                ("width", ctypes.c_int32), ("height", ctypes.c_int32), ("depth", ctypes.c_int32),
+                ("symmetric_arcs", ctypes.c_int32), ("maxflow", ctypes.c_int64),
+                ("active_count", ctypes.c_int32), ("type_terminal_cap", ctypes.c_int32),
+                ("type_neighbor_cap", ctypes.c_int32), ("grid_conn", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 4), ("metadata", ctypes.c_char * 4096)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("reserved", "metadata")}
+        d["metadata"] = self.metadata
+        return d
+
+
 class KernelProfile(ctypes.Structure):
     _fields_ = [
         ("name", ctypes.c_char * 32),
@@ -116,6 +130,11 @@ _lib.cgp_device_count.argtypes = [_i32p]
 _lib.cgp_set_profiling.argtypes = [_vp, ctypes.c_int32]
 _lib.cgp_get_profile.argtypes = [_vp, ctypes.POINTER(KernelProfile), ctypes.c_int32, _i32p]
 _lib.cgp_abi_version.restype = ctypes.c_int32
+_cp = ctypes.c_char_p
+_lib.cgp_mfi_read_info.argtypes = [_cp, ctypes.POINTER(MfiInfo)]
+_lib.cgp_mfi_to_grid.argtypes = [_cp, _i32p, _i32p, _i32p, _i32p]
+_lib.cgp_set_grid_mfi.argtypes = [_vp, _cp, ctypes.POINTER(MfiInfo)]
+_lib.cgp_write_mfi.argtypes = [_cp] + [ctypes.c_int32] * 4 + [_i32p] * 4 + [ctypes.c_int64, _cp]
 _lib.cgp_set_grid_batch.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(GridInstance),
                                     ctypes.c_int32]
 _lib.cgp_batch_size.argtypes = [_vp, _i32p]
@@ -307,6 +326,14 @@ class GridGraph(_Handle):
         self._n = nx * ny * nz
         self.shape = (nz, ny, nx)
 
+    def set_grid_mfi(self, path):
+        """Loads an MFI instance file; returns its header as a dict."""
+        info = MfiInfo()
+        self._check(_lib.cgp_set_grid_mfi(self._h, os.fsencode(path), ctypes.byref(info)))
+        self._n = info.width * info.height * info.depth
+        self.shape = (info.depth, info.height, info.width)
+        return info.as_dict()
+
     def set_grid_device(self, shape, conn, cs_ptr, ct_ptr, cn_ptr, cnb_ptr=None):
         """Capacities already resident in device memory (raw device pointers, e.g. tensor.data_ptr())."""
         nz, ny, nx = shape
@@ -375,6 +402,42 @@ class GridBatch(_Handle):
         out = np.empty(self._shape_of(index), np.uint8)
         self._check(_lib.cgp_batch_fill_segments(self._h, index, out.ctypes.data_as(_u8p), mask, background))
         return out
+
+
+def _global_check(rc):
+    if rc != CGP_OK:
+        raise CgpError(rc, (_lib.cgp_last_error(None) or b"").decode())
+
+
+def mfi_info(path):
+    """Header of an MFI instance file as a dict."""
+    info = MfiInfo()
+    _global_check(_lib.cgp_mfi_read_info(os.fsencode(path), ctypes.byref(info)))
+    return info.as_dict()
+
+
+def mfi_to_grid(path):
+    """An MFI file in the grid layout: (info, cap_source, cap_sink, cap_n, cap_n_back, conn)."""
+    info = mfi_info(path)
+    shape = (info["depth"], info["height"], info["width"])
+    conn = info["grid_conn"]
+    cs = np.empty(shape, np.int32)
+    ct = np.empty(shape, np.int32)
+    cn = np.empty((conn // 2,) + shape, np.int32)
+    cnb = np.empty((conn // 2,) + shape, np.int32)
+    _global_check(_lib.cgp_mfi_to_grid(os.fsencode(path), _p32(cs), _p32(ct), _p32(cn), _p32(cnb)))
+    return info, cs, ct, cn, cnb, conn
+
+
+def write_mfi(path, cap_source, cap_sink, cap_n, cap_n_back=None, conn=6, maxflow=0, metadata=""):
+    """Writes a 3D grid (layout of set_grid) as an MFI file with uint32 capacities."""
+    cs, ct, cn = _i32(cap_source), _i32(cap_sink), _i32(cap_n)
+    nz, ny, nx = cs.shape
+    cb = _i32(cap_n_back) if cap_n_back is not None else None
+    if ct.shape != cs.shape or cn.shape != (conn // 2, nz, ny, nx) or (cb is not None and cb.shape != cn.shape):
+        raise CgpError(CGP_ERR_INVALID, "capacity shapes do not match the volume")
+    _global_check(_lib.cgp_write_mfi(os.fsencode(path), nx, ny, nz, conn, _p32(cs), _p32(ct), _p32(cn),
+                                     _p32(cb) if cb is not None else None, int(maxflow), metadata.encode()))
 
 
 def generate_volume(nx, ny, nz, seed, instance=0):

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "common.h"
+#include "mfi.h"
 #include "solver.h"
 
 struct cgp_graph_s {
@@ -50,6 +51,30 @@ int guarded(cgp_graph g, Fn &&fn)
 		g->last_error = e.what();
 		return CGP_ERR_DEVICE;
 	}
+}
+
+// Calls without a handle report their error through cgp_last_error(NULL).
+template <typename Fn>
+int guarded_global(Fn &&fn)
+{
+	std::string message;
+	int code = CGP_OK;
+	try {
+		fn();
+		return CGP_OK;
+	} catch (const cgp::Error &e) {
+		message = e.what();
+		code = e.code;
+	} catch (const std::bad_alloc &) {
+		message = "host allocation failed";
+		code = CGP_ERR_NOMEM;
+	} catch (const std::exception &e) {
+		message = e.what();
+		code = CGP_ERR_DEVICE;
+	}
+	std::lock_guard<std::mutex> lock(g_create_error_mutex);
+	g_create_error = message;
+	return code;
 }
 
 cgp::GridSolver &grid_of(cgp_graph g)
@@ -201,6 +226,63 @@ int cgp_set_grid_device(cgp_graph g, int32_t nx, int32_t ny, int32_t nz, int32_t
                         const int32_t *cap_sink, const int32_t *cap_n, const int32_t *cap_n_back)
 {
 	return set_grid_impl(g, nx, ny, nz, conn, cap_source, cap_sink, cap_n, cap_n_back, true);
+}
+
+int cgp_mfi_read_info(const char *path, cgp_mfi_info *info)
+{
+	return guarded_global([&] {
+		if (!path || !info) {
+			cgp::fail(CGP_ERR_INVALID, "null argument");
+		}
+		*info = cgp::mfi_read(path).info;
+	});
+}
+
+int cgp_mfi_to_grid(const char *path, int32_t *cap_source, int32_t *cap_sink, int32_t *cap_n, int32_t *cap_n_back)
+{
+	return guarded_global([&] {
+		if (!path || !cap_source || !cap_sink || !cap_n || !cap_n_back) {
+			cgp::fail(CGP_ERR_INVALID, "null argument");
+		}
+		const cgp::MfiGrid grid = cgp::mfi_to_grid(cgp::mfi_read(path));
+		std::memcpy(cap_source, grid.cs.data(), grid.cs.size() * sizeof(int32_t));
+		std::memcpy(cap_sink, grid.ct.data(), grid.ct.size() * sizeof(int32_t));
+		std::memcpy(cap_n, grid.cn.data(), grid.cn.size() * sizeof(int32_t));
+		std::memcpy(cap_n_back, grid.cnb.data(), grid.cnb.size() * sizeof(int32_t));
+	});
+}
+
+int cgp_set_grid_mfi(cgp_graph g, const char *path, cgp_mfi_info *info)
+{
+	return guarded(g, [&] {
+		if (!path) {
+			cgp::fail(CGP_ERR_INVALID, "null path");
+		}
+		const cgp::MfiFile file = cgp::mfi_read(path);
+		const cgp::MfiGrid grid = cgp::mfi_to_grid(file);
+		g->csr.reset();
+		if (!g->grid) {
+			g->grid.reset(new cgp::GridSolver(g->stream));
+		}
+		g->mode = cgp_graph_s::Mode::Grid;
+		g->grid->set_grid(grid.nx, grid.ny, grid.nz, grid.conn, grid.cs.data(), grid.ct.data(), grid.cn.data(),
+		                  grid.cnb.data(), false);
+		if (info) {
+			*info = file.info;
+		}
+	});
+}
+
+int cgp_write_mfi(const char *path, int32_t nx, int32_t ny, int32_t nz, int32_t conn, const int32_t *cap_source,
+                  const int32_t *cap_sink, const int32_t *cap_n, const int32_t *cap_n_back, int64_t maxflow,
+                  const char *metadata)
+{
+	return guarded_global([&] {
+		if (!path) {
+			cgp::fail(CGP_ERR_INVALID, "null path");
+		}
+		cgp::mfi_write_grid(path, nx, ny, nz, conn, cap_source, cap_sink, cap_n, cap_n_back, maxflow, metadata);
+	});
 }
 
 int cgp_set_grid_batch(cgp_graph g, int32_t count, int32_t conn, const cgp_grid_instance *grids,

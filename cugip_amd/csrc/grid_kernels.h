@@ -386,7 +386,10 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_discharge(GridView gv, const 
 		if (level == 0) {
 			break;
 		}
-		// ---- top-down push sweep over the levels present in the tile
+		// ---- top-down push sweep over the levels that hold excess. The labels are exact BFS
+		// distances inside the tile, so a voxel of level L pushes only to voxels of level L - 1:
+		// the next level to visit is L - 1 if this step pushed inside the tile, else the highest
+		// lower level whose voxels already hold excess (their excess does not change in this step).
 		while (level >= 1) {
 			int next = 0;
 #pragma unroll
@@ -397,10 +400,10 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_discharge(GridView gv, const 
 				const int p = T::pidx(lx, ly, k);
 				const int i = T::vidx(lx, ly, k);
 				const int32_t lab = hl[p];
-				if (lab < level && lab > next) {
-					next = lab;
-				}
 				if (lab != level) {
+					if (lab < level && lab > next && el[i] > 0) {
+						next = lab;
+					}
 					continue;
 				}
 				int32_t ex = el[i];
@@ -428,6 +431,7 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_discharge(GridView gv, const 
 								rl[opposite<kConn>(d)][j] += f;
 								atomicOr(&dm[j], 1u << opposite<kConn>(d));
 								atomicAdd(&el[j], f);
+								next = level - 1 > next ? level - 1 : next;
 							}
 						}
 					}

@@ -9,6 +9,7 @@
 #include "grid_kernels.h"
 #include "grid_wave.h"
 #include "solver.h"
+#include "tuned_policy.h"
 
 namespace cgp {
 
@@ -166,7 +167,8 @@ GridSolver::Shape GridSolver::pick_shape(const cgp_policy &policy)
 		}
 		return {policy.tile_x, policy.tile_y, policy.tile_z};
 	}
-	return mConn == 6 ? Shape{32, 8, 4} : Shape{32, 4, 4};
+	const TunedPolicy tp = tuned_policy(mConn, mInst.size() > 1, mVox / (int64_t)mInst.size());
+	return Shape{tp.tile_x, tp.tile_y, tp.tile_z};
 }
 
 void GridSolver::set_grid(int nx, int ny, int nz, int conn, const int32_t *cs, const int32_t *ct, const int32_t *cn,
@@ -589,8 +591,9 @@ int64_t GridSolver::solve(const cgp_policy &policy, cgp_stats *stats_out, const 
 	}
 	configure_tiles(pick_shape(policy));
 	const int ncolours = mConn == 6 ? 2 : 8;
-	const int phases = policy.discharges_per_relabel > 0 ? policy.discharges_per_relabel : 2 * ncolours;
-	const int max_iters = policy.max_tile_iterations > 0 ? policy.max_tile_iterations : 16;
+	const TunedPolicy tp = tuned_policy(mConn, mInst.size() > 1, mVox / (int64_t)mInst.size());
+	const int phases = policy.discharges_per_relabel > 0 ? policy.discharges_per_relabel : tp.phases;
+	const int max_iters = policy.max_tile_iterations > 0 ? policy.max_tile_iterations : tp.iterations;
 	const int max_rounds = policy.max_rounds > 0 ? policy.max_rounds : 1 << 20;
 	int32_t threshold = policy.minimal_residual > 0.0f ? (int32_t)std::floor(policy.minimal_residual) : 0;
 	if (policy.minimal_residual < 0.0f || !std::isfinite(policy.minimal_residual)) {

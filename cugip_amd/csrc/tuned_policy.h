@@ -1,0 +1,37 @@
+// Default grid schedules, one row per (connectivity, batch, grid size class): tile shape,
+// discharge phases per global relabel and in-tile iterations per discharge. Written by
+// `python -m cugip_amd.autotune --write-header` from timings on an MI355X; policy fields a
+// caller sets explicitly override the row. Rows are matched in order: the first row with the
+// grid's connectivity and batch flag whose max_voxels (voxels per grid) is not exceeded.
+#pragma once
+
+#include <cstdint>
+
+namespace cgp {
+
+struct TunedPolicy {
+	int32_t conn, batch;
+	int64_t max_voxels;
+	int32_t tile_x, tile_y, tile_z;
+	int32_t phases, iterations;
+};
+
+static const TunedPolicy kTunedPolicies[] = {
+	// conn batch max_voxels  tile        phases iters
+	{6, 0, INT64_MAX, 32, 8, 4, 4, 16},
+	{6, 1, INT64_MAX, 32, 8, 4, 4, 16},
+	{26, 0, INT64_MAX, 32, 4, 4, 16, 16},
+	{26, 1, INT64_MAX, 32, 4, 4, 16, 16},
+};
+
+inline TunedPolicy tuned_policy(int conn, bool batch, int64_t voxels_per_grid)
+{
+	for (const TunedPolicy &p : kTunedPolicies) {
+		if (p.conn == conn && p.batch == (batch ? 1 : 0) && voxels_per_grid <= p.max_voxels) {
+			return p;
+		}
+	}
+	return TunedPolicy{conn, batch ? 1 : 0, INT64_MAX, 32, conn == 6 ? 8 : 4, 4, conn == 6 ? 4 : 16, 16};
+}
+
+}  // namespace cgp

@@ -273,6 +273,15 @@ public:
 		mVertexCount = (int)n;
 	}
 
+	// Loads an MFI instance file (the grid benchmark format of benchmarks/mfi.h); returns its header.
+	cgp_mfi_info set_grid_mfi(const std::string &aPath)
+	{
+		cgp_mfi_info info;
+		detail::check(handle(), cgp_set_grid_mfi(handle(), aPath.c_str(), &info));
+		mVertexCount = info.width * info.height * info.depth;
+		return info;
+	}
+
 	TFlow max_flow()
 	{
 		DummyTraceObject dummyTraceObject;

@@ -101,7 +101,8 @@ typedef void (*cgp_trace_fn)(void *user, int32_t event, int32_t iteration, int32
 /* ---- handle ------------------------------------------------------------------------- */
 int cgp_create(cgp_graph *out);
 int cgp_destroy(cgp_graph graph);
-/* Message of the last failed call on `graph`, or of the last failed cgp_create when graph is NULL. */
+/* Message of the last failed call on `graph`; with NULL, of the last failed call made without a
+   handle (cgp_create, the MFI file functions). */
 const char *cgp_last_error(cgp_graph graph);
 
 /* ---- generic graphs: cugip::Graph<TFlow> (advanced_operations/graph_cut.hpp:47-146) ---- */
@@ -184,6 +185,36 @@ typedef struct cgp_kernel_profile {
 } cgp_kernel_profile;
 int cgp_set_profiling(cgp_graph graph, int32_t enable);
 int cgp_get_profile(cgp_graph graph, cgp_kernel_profile *out, int32_t capacity, int32_t *count);
+
+/* ---- MFI instance files --------------------------------------------------------------------
+   The regular-grid max-flow benchmark format the reference reads (benchmarks/mfi.h:4-77,
+   benchmarks/mfi.cpp:105-200): 2D 4/8-connected and 3D 6/18/26-connected grids with uint8, uint16
+   or uint32 capacities and the instance's known max flow in the header. An instance loads as a
+   6-connected grid (2D-4C, 3D-6C; depth 1 for 2D) or a 26-connected grid (2D-8C, 3D-18C, 3D-26C;
+   the arcs a lower connectivity lacks have capacity 0), with both directions of every edge. */
+typedef struct cgp_mfi_info {
+	int32_t dimension, connectivity;       /* as stored */
+	int32_t width, height, depth;          /* nx, ny, nz of the grid */
+	int32_t symmetric_arcs;
+	int64_t maxflow;                       /* header value */
+	int32_t active_count;
+	int32_t type_terminal_cap, type_neighbor_cap; /* 0 uint8, 1 uint16, 2 uint32 */
+	int32_t grid_conn;                     /* 6 or 26 */
+	int32_t reserved[4];
+	char metadata[4096];
+} cgp_mfi_info;
+/* Header of an MFI file. Without a handle: on failure cgp_last_error(NULL) has the message. */
+int cgp_mfi_read_info(const char *path, cgp_mfi_info *info);
+/* Capacities of an MFI file in the layout of cgp_set_grid: host buffers of width*height*depth
+   int32 (cap_n and cap_n_back: grid_conn/2 such planes). */
+int cgp_mfi_to_grid(const char *path, int32_t *cap_source, int32_t *cap_sink, int32_t *cap_n, int32_t *cap_n_back);
+/* Reads an MFI file into the handle (as cgp_set_grid); info (may be NULL) receives its header. */
+int cgp_set_grid_mfi(cgp_graph graph, const char *path, cgp_mfi_info *info);
+/* Writes a 3D grid (layout of cgp_set_grid, cap_n_back NULL = symmetric) as an MFI file with
+   uint32 capacities, `maxflow` in the header and optional metadata text. */
+int cgp_write_mfi(const char *path, int32_t nx, int32_t ny, int32_t nz, int32_t conn, const int32_t *cap_source,
+                  const int32_t *cap_sink, const int32_t *cap_n, const int32_t *cap_n_back, int64_t maxflow,
+                  const char *metadata);
 
 /* ---- synthetic instances ------------------------------------------------------------------
    Blob volume (3-8 ellipsoids at 180 on 60, N(0, 25^2) noise, clamped to 0-255) and the int32

@@ -1,0 +1,43 @@
+"""The autotuner's output on a machine without a GPU: a table of winners becomes a header that
+compiles and whose lookup picks the bounded row for small grids and the unbounded row otherwise."""
+import os
+import subprocess
+
+from cugip_amd import autotune
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_header_from_table_compiles_and_matches(tmp_path):
+    table = [
+        dict(workload="big", conn=6, batch=False, max_voxels=None,
+             best=dict(tile=[64, 4, 4], phases=8, iterations=2, ms=5.0)),
+        dict(workload="small", conn=6, batch=False, max_voxels=1 << 21,
+             best=dict(tile=[32, 8, 4], phases=4, iterations=1, ms=1.0)),
+        dict(workload="b26", conn=26, batch=True, max_voxels=None,
+             best=dict(tile=[16, 8, 4], phases=16, iterations=4, ms=9.0)),
+    ]
+    text = autotune.header_text(table)
+    inc = tmp_path / "inc"
+    inc.mkdir()
+    (inc / "tuned_policy.h").write_text(text)
+    src = tmp_path / "t.cpp"
+    src.write_text(
+        '#include "tuned_policy.h"\n#include <cstdio>\n'
+        "int main() { using namespace cgp;\n"
+        "  TunedPolicy a = tuned_policy(6, false, 1000), b = tuned_policy(6, false, 1LL << 24),\n"
+        "              c = tuned_policy(26, true, 5), d = tuned_policy(26, false, 5);\n"
+        '  std::printf("%d %d %d %d %d %d %d %d\\n", a.tile_x, a.iterations, b.tile_x, b.phases, c.tile_y,\n'
+        "              c.iterations, d.tile_x, d.phases);\n  return 0; }\n")
+    exe = tmp_path / "t"
+    subprocess.run(["g++", "-std=c++14", "-Wall", "-Werror", "-I", str(inc), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
+    # small grid -> bounded row; large -> unbounded row; 26-connected batch row; no single 26 row -> fallback
+    assert out == ["32", "1", "64", "8", "8", "4", "32", "16"]
+
+
+def test_candidates_cover_compiled_shapes():
+    src = open(os.path.join(ROOT, "cugip_amd", "csrc", "grid_solver.hip")).read()
+    for conn, shapes in autotune.SHAPES.items():
+        for tx, ty, tz in shapes:
+            assert "F(%d, %d, %d, %d)" % (conn, tx, ty, tz) in src

@@ -228,3 +228,26 @@ def test_cpp_dropin_grid():
     flow, smin, tmin = oracle.grid_solve("bk", 53, 41, 29, 6, cs, ct, cn)
     assert out.stdout.split() == ["flow", str(flow), "source_side", str(int(smin.sum())), "sink_side",
                                   str(int(tmin.sum()))]
+
+
+@pytest.mark.parametrize("conn", [6, 26])
+def test_mfi_instance(tmp_path, conn):
+    """An MFI file (asymmetric capacities) solved through cgp_set_grid_mfi: the flow equals the
+    header's max flow, and S_min / T_min equal the oracle's on the converted grid."""
+    say(f"grid mfi conn {conn}")
+    rng = np.random.default_rng(40 + conn)
+    shape = (23, 31, 37)
+    nz, ny, nx = shape
+    cs = (rng.integers(0, 400, shape) * (rng.random(shape) < 0.25)).astype(np.int32)
+    ct = (rng.integers(0, 400, shape) * (rng.random(shape) < 0.25)).astype(np.int32)
+    cn = rng.integers(0, 90, (conn // 2,) + shape).astype(np.int32)
+    cnb = rng.integers(0, 90, (conn // 2,) + shape).astype(np.int32)
+    ref_flow, ref_smin, ref_tmin = oracle.grid_solve("dinic", nx, ny, nz, conn, cs, ct, cn, cnb)
+    path = str(tmp_path / "i.mfi")
+    cugip_amd.write_mfi(path, cs, ct, cn, cnb, conn=conn, maxflow=ref_flow)
+    with cugip_amd.GridGraph() as g:
+        info = g.set_grid_mfi(path)
+        assert info["maxflow"] == ref_flow and info["grid_conn"] == conn
+        assert g.max_flow() == info["maxflow"]
+        assert np.array_equal(g.source_set(), ref_smin)
+        assert np.array_equal(g.fill_segments(1, 0), ref_tmin)
