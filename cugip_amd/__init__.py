@@ -92,6 +92,13 @@ class MfiInfo(ctypes.Structure):
         return d
 
 
+class VolumeWeights(ctypes.Structure):
+    _fields_ = [("tlink_source", ctypes.c_int32 * 256), ("tlink_sink", ctypes.c_int32 * 256),
+                ("nlink", (ctypes.c_int32 * 256) * 3), ("marker_source", ctypes.c_int32),
+                ("marker_sink", ctypes.c_int32), ("marker_capacity", ctypes.c_int32), ("border", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 4)]
+
+
 class KernelProfile(ctypes.Structure):
     _fields_ = [
         ("name", ctypes.c_char * 32),
@@ -135,6 +142,10 @@ _lib.cgp_mfi_read_info.argtypes = [_cp, ctypes.POINTER(MfiInfo)]
 _lib.cgp_mfi_to_grid.argtypes = [_cp, _i32p, _i32p, _i32p, _i32p]
 _lib.cgp_set_grid_mfi.argtypes = [_vp, _cp, ctypes.POINTER(MfiInfo)]
 _lib.cgp_write_mfi.argtypes = [_cp] + [ctypes.c_int32] * 4 + [_i32p] * 4 + [ctypes.c_int64, _cp]
+_lib.cgp_default_volume_weights.argtypes = [ctypes.POINTER(VolumeWeights)]
+_lib.cgp_set_grid_volume.argtypes = [_vp] + [ctypes.c_int32] * 4 + [_vp, _vp, ctypes.POINTER(VolumeWeights),
+                                                                    ctypes.c_int32]
+_lib.cgp_grid_capacities.argtypes = [_vp, ctypes.c_int32, _i32p, _i32p, _i32p, _i32p]
 _lib.cgp_set_grid_batch.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(GridInstance),
                                     ctypes.c_int32]
 _lib.cgp_batch_size.argtypes = [_vp, _i32p]
@@ -326,12 +337,36 @@ class GridGraph(_Handle):
         self._n = nx * ny * nz
         self.shape = (nz, ny, nx)
 
+    def set_grid_volume(self, volume, markers=None, weights=None, conn=6):
+        """Capacities computed on the device from a uint8 (nz, ny, nx) volume (and optional marker
+        volume) with the lookup tables of `weights` (VolumeWeights; None = default_volume_weights())."""
+        vol = np.ascontiguousarray(volume, dtype=np.uint8)
+        if vol.ndim != 3:
+            raise CgpError(CGP_ERR_INVALID, "volume must be (nz, ny, nx)")
+        mk = None
+        if markers is not None:
+            mk = np.ascontiguousarray(markers, dtype=np.uint8)
+            if mk.shape != vol.shape:
+                raise CgpError(CGP_ERR_INVALID, "markers must match the volume")
+        nz, ny, nx = vol.shape
+        self._check(_lib.cgp_set_grid_volume(self._h, nx, ny, nz, conn, vol.ctypes.data,
+                                             mk.ctypes.data if mk is not None else None,
+                                             ctypes.byref(weights) if weights is not None else None, 0))
+        self._n = nx * ny * nz
+        self.shape = (nz, ny, nx)
+        self.conn = conn
+
+    def capacities(self):
+        """Input capacities held by the handle: (cap_source, cap_sink, cap_n, cap_n_back)."""
+        return _capacities(self, 0, self.shape, self.conn)
+
     def set_grid_mfi(self, path):
         """Loads an MFI instance file; returns its header as a dict."""
         info = MfiInfo()
         self._check(_lib.cgp_set_grid_mfi(self._h, os.fsencode(path), ctypes.byref(info)))
         self._n = info.width * info.height * info.depth
         self.shape = (info.depth, info.height, info.width)
+        self.conn = info.grid_conn
         return info.as_dict()
 
     def set_grid_device(self, shape, conn, cs_ptr, ct_ptr, cn_ptr, cnb_ptr=None):
@@ -341,6 +376,23 @@ class GridGraph(_Handle):
                                              _vp(cnb_ptr) if cnb_ptr else None))
         self._n = nx * ny * nz
         self.shape = (nz, ny, nx)
+        self.conn = conn
+
+
+def _capacities(handle, index, shape, conn):
+    cs = np.empty(shape, np.int32)
+    ct = np.empty(shape, np.int32)
+    cn = np.empty((conn // 2,) + tuple(shape), np.int32)
+    cnb = np.empty((conn // 2,) + tuple(shape), np.int32)
+    handle._check(_lib.cgp_grid_capacities(handle._h, index, _p32(cs), _p32(ct), _p32(cn), _p32(cnb)))
+    return cs, ct, cn, cnb
+
+
+def default_volume_weights():
+    """The lookup tables of the synthetic instances as front-end weights (VolumeWeights)."""
+    w = VolumeWeights()
+    _global_check(_lib.cgp_default_volume_weights(ctypes.byref(w)))
+    return w
 
 
 class GridBatch(_Handle):
@@ -364,6 +416,7 @@ class GridBatch(_Handle):
             shapes.append((nz, ny, nx))
         self._check(_lib.cgp_set_grid_batch(self._h, len(grids), conn, arr, 0))
         self.shapes = shapes
+        self.conn = conn
         self._n = sum(a * b * c for a, b, c in shapes)
 
     def set_batch_device(self, shapes, conn, pointers):
@@ -374,6 +427,7 @@ class GridBatch(_Handle):
             arr[i] = GridInstance(nx, ny, nz, 0, ptrs[0], ptrs[1], ptrs[2], cnb)
         self._check(_lib.cgp_set_grid_batch(self._h, len(shapes), conn, arr, 1))
         self.shapes = [tuple(s) for s in shapes]
+        self.conn = conn
         self._n = sum(a * b * c for a, b, c in self.shapes)
 
     def solve(self, policy=None):
@@ -396,6 +450,10 @@ class GridBatch(_Handle):
         out = np.empty(self._shape_of(index), np.uint8)
         self._check(_lib.cgp_batch_source_set(self._h, index, out.ctypes.data_as(_u8p)))
         return out
+
+    def capacities(self, index):
+        """Input capacities of grid `index`: (cap_source, cap_sink, cap_n, cap_n_back)."""
+        return _capacities(self, index, self._shape_of(index), self.conn)
 
     def batch_fill_segments(self, index, mask=255, background=0):
         """T_min of grid `index` (fill_segments semantics), shaped like its volume."""

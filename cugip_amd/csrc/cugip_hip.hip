@@ -285,6 +285,45 @@ int cgp_write_mfi(const char *path, int32_t nx, int32_t ny, int32_t nz, int32_t 
 	});
 }
 
+int cgp_default_volume_weights(cgp_volume_weights *w)
+{
+	return guarded_global([&] {
+		if (!w) {
+			cgp::fail(CGP_ERR_INVALID, "null argument");
+		}
+		cgp::default_volume_weights(*w);
+	});
+}
+
+int cgp_set_grid_volume(cgp_graph g, int32_t nx, int32_t ny, int32_t nz, int32_t conn, const uint8_t *volume,
+                        const uint8_t *markers, const cgp_volume_weights *weights, int32_t device_pointers)
+{
+	return guarded(g, [&] {
+		cgp_volume_weights defaults;
+		if (!weights) {
+			cgp::default_volume_weights(defaults);
+			weights = &defaults;
+		}
+		g->csr.reset();
+		if (!g->grid) {
+			g->grid.reset(new cgp::GridSolver(g->stream));
+		}
+		g->mode = cgp_graph_s::Mode::Grid;
+		g->grid->set_grid_volume(nx, ny, nz, conn, volume, markers, *weights, device_pointers != 0);
+	});
+}
+
+int cgp_grid_capacities(cgp_graph g, int32_t index, int32_t *cap_source, int32_t *cap_sink, int32_t *cap_n,
+                        int32_t *cap_n_back)
+{
+	return guarded(g, [&] {
+		if (!cap_source || !cap_sink || !cap_n || !cap_n_back) {
+			cgp::fail(CGP_ERR_INVALID, "null argument");
+		}
+		grid_of(g).capacities(index, cap_source, cap_sink, cap_n, cap_n_back);
+	});
+}
+
 int cgp_set_grid_batch(cgp_graph g, int32_t count, int32_t conn, const cgp_grid_instance *grids,
                        int32_t device_pointers)
 {

@@ -143,3 +143,24 @@ cgp_caps_from_volume(
 	}
 	return CGP_OK;
 }
+
+namespace cgp {
+
+// The lookup tables of the synthetic instances as front-end weights: no markers, no border.
+void default_volume_weights(cgp_volume_weights &w)
+{
+	std::memset(&w, 0, sizeof(w));
+	for (int i = 0; i < 256; ++i) {
+		w.tlink_source[i] = CGP_TLINK_SOURCE[i];
+		w.tlink_sink[i] = CGP_TLINK_SINK[i];
+		for (int c = 0; c < 3; ++c) {
+			w.nlink[c][i] = CGP_NLINK_WEIGHT[c][i];
+		}
+	}
+	w.marker_source = -1;
+	w.marker_sink = -1;
+	w.marker_capacity = 0;
+	w.border = 0;
+}
+
+}  // namespace cgp

@@ -652,6 +652,57 @@ __global__ __launch_bounds__(256) void k_compact_tiles(const uint8_t *__restrict
 	}
 }
 
+// Image -> graph front end: t-links by intensity, then marker and border forcing; n-links by the
+// absolute intensity difference per neighbour class (the caller-side construction of
+// benchmarks/graph_cut/graph_cut.cpp:32-98, as table lookups).
+template <int kConn>
+__global__ __launch_bounds__(256) void k_caps_from_volume(int nx, int ny, int nz, const uint8_t *__restrict__ vol,
+                                                          const uint8_t *__restrict__ markers,
+                                                          const cgp_volume_weights *__restrict__ w,
+                                                          int32_t *__restrict__ cs, int32_t *__restrict__ ct,
+                                                          int32_t *__restrict__ cn)
+{
+	using D = Dirs<kConn>;
+	const int64_t n = (int64_t)nx * ny * nz;
+	const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+	for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+		const int x = (int)(v % nx);
+		const int y = (int)((v / nx) % ny);
+		const int z = (int)(v / ((int64_t)nx * ny));
+		const int iv = vol[v];
+		int32_t s = w->tlink_source[iv], t = w->tlink_sink[iv];
+		if (markers) {
+			const int m = markers[v];
+			if (m == w->marker_source) {
+				s = w->marker_capacity;
+				t = 0;
+			} else if (m == w->marker_sink) {
+				s = 0;
+				t = w->marker_capacity;
+			}
+		}
+		const int b = w->border;
+		if (b > 0 && (x < b || y < b || z < b || x >= nx - b || y >= ny - b || z >= nz - b)) {
+			s = w->marker_capacity;
+			t = 0;
+		}
+		cs[v] = s;
+		ct[v] = t;
+#pragma unroll
+		for (int j = 0; j < D::H; ++j) {
+			const int dx = D::dx(j), dy = D::dy(j), dz = D::dz(j);
+			const int ux = x + dx, uy = y + dy, uz = z + dz;
+			int32_t c = 0;
+			if (ux >= 0 && ux < nx && uy >= 0 && uy < ny && uz >= 0 && uz < nz) {
+				const int iu = vol[((int64_t)uz * ny + uy) * nx + ux];
+				const int cls = (dx != 0) + (dy != 0) + (dz != 0) - 1;
+				c = w->nlink[cls][iv > iu ? iv - iu : iu - iv];
+			}
+			cn[(int64_t)j * n + v] = c;
+		}
+	}
+}
+
 __global__ __launch_bounds__(256) void k_fill_u8(uint8_t *__restrict__ p, int64_t n, uint8_t value)
 {
 	const int64_t stride = (int64_t)gridDim.x * blockDim.x;

@@ -178,7 +178,7 @@ void GridSolver::set_grid(int nx, int ny, int nz, int conn, const int32_t *cs, c
 	set_batch(conn, &spec, 1, device_pointers);
 }
 
-void GridSolver::set_batch(int conn, const GridSpec *specs, int count, bool device_pointers)
+std::vector<InstanceDesc> GridSolver::describe(int conn, const GridSpec *specs, int count)
 {
 	if (count <= 0 || !specs) {
 		fail(CGP_ERR_INVALID, "a batch needs at least one grid");
@@ -188,15 +188,11 @@ void GridSolver::set_batch(int conn, const GridSpec *specs, int count, bool devi
 	}
 	std::vector<InstanceDesc> inst((size_t)count);
 	int64_t total = 0;
-	bool any_back = false;
 	for (int i = 0; i < count; ++i) {
 		const GridSpec &sp = specs[i];
 		const std::string which = count > 1 ? "grid " + std::to_string(i) + ": " : std::string();
 		if (sp.nx <= 0 || sp.ny <= 0 || sp.nz <= 0) {
 			fail(CGP_ERR_INVALID, which + "grid dimensions must be positive");
-		}
-		if (!sp.cs || !sp.ct || !sp.cn) {
-			fail(CGP_ERR_INVALID, which + "capacity pointers must not be null");
 		}
 		const int64_t vox = (int64_t)sp.nx * sp.ny * sp.nz;
 		if (vox >= (int64_t)1 << 31) {
@@ -211,18 +207,27 @@ void GridSolver::set_batch(int conn, const GridSpec *specs, int count, bool devi
 		d.vbase = total;
 		d.nvox = vox;
 		total += vox;
-		any_back |= sp.cnb != nullptr;
+	}
+	return inst;
+}
+
+void GridSolver::allocate_grids(int conn, const std::vector<InstanceDesc> &inst, bool has_back)
+{
+	int64_t total = 0;
+	for (const InstanceDesc &d : inst) {
+		total += d.nvox;
 	}
 	const int half = conn / 2;
+	const size_t count = inst.size();
 	mSolved = false;
 	mInst = inst;
-	mFlows.assign((size_t)count, 0);
+	mFlows.assign(count, 0);
 	mNx = inst[0].nx;
 	mNy = inst[0].ny;
 	mNz = inst[0].nz;
 	mConn = conn;
 	mVox = total;
-	mHasBack = any_back;
+	mHasBack = has_back;
 	mCs.allocate(total);
 	mCt.allocate(total);
 	mCn.allocate((size_t)total * half);
@@ -231,6 +236,32 @@ void GridSolver::set_batch(int conn, const GridSpec *specs, int count, bool devi
 	} else {
 		mCnb.release();
 	}
+	mE.allocate(total);
+	mRt.allocate(total);
+	mH.allocate(total);
+	mR.allocate((size_t)total * conn);
+	mS.allocate(total);
+	mOmask.allocate(total);
+	mFlow.allocate(count);
+	mCounters.allocate(4);
+	mStatus.allocate(1);
+	mCount.allocate(1);
+	mHostScratch.allocate(count > 8 ? count : 8);
+}
+
+void GridSolver::set_batch(int conn, const GridSpec *specs, int count, bool device_pointers)
+{
+	const std::vector<InstanceDesc> inst = describe(conn, specs, count);
+	bool any_back = false;
+	for (int i = 0; i < count; ++i) {
+		if (!specs[i].cs || !specs[i].ct || !specs[i].cn) {
+			fail(CGP_ERR_INVALID, (count > 1 ? "grid " + std::to_string(i) + ": " : std::string()) +
+			                          "capacity pointers must not be null");
+		}
+		any_back |= specs[i].cnb != nullptr;
+	}
+	allocate_grids(conn, inst, any_back);
+	const int half = conn / 2;
 	const hipMemcpyKind kind = device_pointers ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
 	for (int i = 0; i < count; ++i) {
 		const GridSpec &sp = specs[i];
@@ -244,17 +275,70 @@ void GridSolver::set_batch(int conn, const GridSpec *specs, int count, bool devi
 			CGP_HIP(hipMemcpyAsync(mCnb.get() + d.vbase * half, sp.cnb ? sp.cnb : sp.cn, vb * half, kind, mStream));
 		}
 	}
-	mE.allocate(total);
-	mRt.allocate(total);
-	mH.allocate(total);
-	mR.allocate((size_t)total * conn);
-	mS.allocate(total);
-	mOmask.allocate(total);
-	mFlow.allocate(count);
-	mCounters.allocate(4);
-	mStatus.allocate(1);
-	mCount.allocate(1);
-	mHostScratch.allocate(count > 8 ? count : 8);
+	CGP_HIP(hipStreamSynchronize(mStream));
+}
+
+void GridSolver::set_grid_volume(int nx, int ny, int nz, int conn, const uint8_t *volume, const uint8_t *markers,
+                                 const cgp_volume_weights &weights, bool device_pointers)
+{
+	if (!volume) {
+		fail(CGP_ERR_INVALID, "volume pointer must not be null");
+	}
+	const GridSpec spec{nx, ny, nz, nullptr, nullptr, nullptr, nullptr};
+	const std::vector<InstanceDesc> inst = describe(conn, &spec, 1);
+	for (int i = 0; i < 256; ++i) {
+		bool bad = weights.tlink_source[i] < 0 || weights.tlink_sink[i] < 0;
+		for (int c = 0; c < 3; ++c) {
+			bad |= weights.nlink[c][i] < 0;
+		}
+		if (bad) {
+			fail(CGP_ERR_INVALID, "capacity tables must be non-negative");
+		}
+	}
+	if (weights.marker_capacity < 0 || weights.border < 0) {
+		fail(CGP_ERR_INVALID, "marker_capacity and border must be non-negative");
+	}
+	allocate_grids(conn, inst, false);
+	const int64_t n = inst[0].nvox;
+	DeviceBuffer<uint8_t> vol, mark;
+	DeviceBuffer<cgp_volume_weights> w(1);
+	const uint8_t *dv = volume, *dm = markers;
+	if (!device_pointers) {
+		vol.allocate(n);
+		CGP_HIP(hipMemcpyAsync(vol.get(), volume, n, hipMemcpyHostToDevice, mStream));
+		dv = vol.get();
+		if (markers) {
+			mark.allocate(n);
+			CGP_HIP(hipMemcpyAsync(mark.get(), markers, n, hipMemcpyHostToDevice, mStream));
+			dm = mark.get();
+		}
+	}
+	CGP_HIP(hipMemcpyAsync(w.get(), &weights, sizeof(weights), hipMemcpyHostToDevice, mStream));
+	const int blocks = blocks_for(n, 256, 8192);
+	if (conn == 6) {
+		hipLaunchKernelGGL(k_caps_from_volume<6>, dim3(blocks), dim3(256), 0, mStream, nx, ny, nz, dv, dm, w.get(),
+		                   mCs.get(), mCt.get(), mCn.get());
+	} else {
+		hipLaunchKernelGGL(k_caps_from_volume<26>, dim3(blocks), dim3(256), 0, mStream, nx, ny, nz, dv, dm, w.get(),
+		                   mCs.get(), mCt.get(), mCn.get());
+	}
+	CGP_LAUNCH_CHECK();
+	CGP_HIP(hipStreamSynchronize(mStream));
+}
+
+void GridSolver::capacities(int index, int32_t *cs, int32_t *ct, int32_t *cn, int32_t *cnb)
+{
+	if (mInst.empty()) {
+		fail(CGP_ERR_STATE, "no grid set");
+	}
+	const InstanceDesc &d = instance(index);
+	const int half = mConn / 2;
+	const size_t vb = (size_t)d.nvox * sizeof(int32_t);
+	CGP_HIP(hipMemcpyAsync(cs, mCs.get() + d.vbase, vb, hipMemcpyDeviceToHost, mStream));
+	CGP_HIP(hipMemcpyAsync(ct, mCt.get() + d.vbase, vb, hipMemcpyDeviceToHost, mStream));
+	CGP_HIP(hipMemcpyAsync(cn, mCn.get() + d.vbase * half, vb * half, hipMemcpyDeviceToHost, mStream));
+	const int32_t *back = mHasBack ? mCnb.get() : mCn.get();
+	CGP_HIP(hipMemcpyAsync(cnb, back + d.vbase * half, vb * half, hipMemcpyDeviceToHost, mStream));
 	CGP_HIP(hipStreamSynchronize(mStream));
 }
 

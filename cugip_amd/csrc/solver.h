@@ -21,6 +21,7 @@ struct TraceHook {
 };
 
 cgp_policy default_policy();
+void default_volume_weights(cgp_volume_weights &w);
 
 // One grid of a batch, as the caller passes it (layout of cgp_set_grid).
 struct GridSpec {
@@ -38,6 +39,10 @@ public:
 	void set_grid(int nx, int ny, int nz, int conn, const int32_t *cs, const int32_t *ct, const int32_t *cn,
 	              const int32_t *cnb, bool device_pointers);
 	void set_batch(int conn, const GridSpec *specs, int count, bool device_pointers);
+	// Capacities computed on the device from a volume (and optional markers).
+	void set_grid_volume(int nx, int ny, int nz, int conn, const uint8_t *volume, const uint8_t *markers,
+	                     const cgp_volume_weights &weights, bool device_pointers);
+	void capacities(int index, int32_t *cs, int32_t *ct, int32_t *cn, int32_t *cnb);
 	// Returns the total flow of all instances; per-instance values in flows().
 	int64_t solve(const cgp_policy &policy, cgp_stats *stats, const TraceHook &trace);
 	const std::vector<int64_t> &flows() const { return mFlows; }
@@ -78,6 +83,8 @@ private:
 	void run_source_closure(cgp_stats &stats);
 	int64_t read_flow();
 	const InstanceDesc &instance(int index) const;
+	std::vector<InstanceDesc> describe(int conn, const GridSpec *specs, int count);
+	void allocate_grids(int conn, const std::vector<InstanceDesc> &inst, bool has_back);
 
 	hipStream_t mStream;
 	int mNx = 0, mNy = 0, mNz = 0, mConn = 0;  // dimensions of a single grid (instance 0)

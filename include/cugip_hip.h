@@ -216,6 +216,36 @@ int cgp_write_mfi(const char *path, int32_t nx, int32_t ny, int32_t nz, int32_t 
                   const int32_t *cap_sink, const int32_t *cap_n, const int32_t *cap_n_back, int64_t maxflow,
                   const char *metadata);
 
+/* ---- on-device image -> graph front end -----------------------------------------------------
+   Capacities computed on the device from a uint8 volume, so that they never exist on the host:
+   the counterpart of the caller-side construction in benchmarks/graph_cut/graph_cut.cpp:32-98
+   (t-links from markers, border voxels forced to the source, contrast-sensitive n-links).
+   Every capacity is a table lookup: t-links by intensity, n-links by |I_v - I_u| per neighbour
+   class (0 face, 1 edge, 2 corner neighbour). Then, in this order: a voxel whose marker equals
+   marker_source gets t-links (marker_capacity, 0) and one equal to marker_sink (0,
+   marker_capacity); a voxel with some coordinate c < border or c >= size - border gets
+   (marker_capacity, 0). The reference's scheme is marker_source 128, marker_sink 255,
+   border 2, zero t-link tables. */
+typedef struct cgp_volume_weights {
+	int32_t tlink_source[256];
+	int32_t tlink_sink[256];
+	int32_t nlink[3][256];
+	int32_t marker_source, marker_sink;   /* -1: no such marker */
+	int32_t marker_capacity;
+	int32_t border;                       /* 0: no border forcing */
+	int32_t reserved[4];
+} cgp_volume_weights;
+/* The tables of the synthetic instances (those of cgp_caps_from_volume), no markers, no border. */
+int cgp_default_volume_weights(cgp_volume_weights *weights);
+/* Sets a grid from a volume (and optional marker volume, NULL = none) of nx*ny*nz bytes; weights
+   NULL = the defaults. device_pointers != 0: volume and markers are device memory. */
+int cgp_set_grid_volume(cgp_graph graph, int32_t nx, int32_t ny, int32_t nz, int32_t conn, const uint8_t *volume,
+                        const uint8_t *markers, const cgp_volume_weights *weights, int32_t device_pointers);
+/* The input capacities the handle holds for grid `index` (0 for a single grid), in the layout of
+   cgp_set_grid; cap_n_back receives the forward planes when the grid is symmetric. */
+int cgp_grid_capacities(cgp_graph graph, int32_t index, int32_t *cap_source, int32_t *cap_sink, int32_t *cap_n,
+                        int32_t *cap_n_back);
+
 /* ---- synthetic instances ------------------------------------------------------------------
    Blob volume (3-8 ellipsoids at 180 on 60, N(0, 25^2) noise, clamped to 0-255) and the int32
    capacities derived from it (Boykov-Jolly t-links, contrast-sensitive n-links). Integer-only:

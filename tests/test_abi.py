@@ -6,6 +6,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 import cugip_amd
@@ -47,6 +48,21 @@ def test_struct_layouts_match_header():
     # cgp_grid_instance: 4 int32 + 4 pointers
     assert ctypes.sizeof(cugip_amd.GridInstance) == 48
     assert cugip_amd.GridInstance.cap_source.offset == 16 and cugip_amd.GridInstance.cap_n_back.offset == 40
+    # cgp_mfi_info: 6 int32, int64, 4 int32, 4 reserved, 4096 bytes of metadata
+    assert ctypes.sizeof(cugip_amd.MfiInfo) == 24 + 8 + 16 + 16 + 4096
+    assert cugip_amd.MfiInfo.maxflow.offset == 24 and cugip_amd.MfiInfo.metadata.offset == 64
+    # cgp_volume_weights: 5 tables of 256 int32, 4 int32, 4 reserved
+    assert ctypes.sizeof(cugip_amd.VolumeWeights) == 5 * 1024 + 16 + 16
+
+
+def test_default_volume_weights_are_the_generator_tables():
+    w = cugip_amd.default_volume_weights()
+    vol = (np.arange(6 * 7 * 8) % 256).astype(np.uint8).reshape(6, 7, 8)
+    for conn in (6, 26):
+        cs, ct, cn = cugip_amd.caps_from_volume(vol, conn)
+        assert np.array_equal(cs, np.array(w.tlink_source)[vol])
+        assert np.array_equal(ct, np.array(w.tlink_sink)[vol])
+    assert (w.marker_source, w.marker_sink, w.border) == (-1, -1, 0)
 
 
 def test_no_device_is_an_error_not_a_crash():
@@ -69,6 +85,7 @@ def test_cpp_header_compiles_standalone(tmp_path):
         "  cugip::GraphCutPolicy p; p.relabelPolicy.edgeTraversalCheck.minimalResidual = 0.f;\n"
         "  static_assert(sizeof(cugip::EdgeRecord) == 8, \"layout\");\n"
         "  static_assert(sizeof(cgp_grid_instance) == 48, \"layout\");\n"
+        "  static_assert(sizeof(cgp_mfi_info) == 4160 && sizeof(cgp_volume_weights) == 5152, \"layout\");\n"
         "  return d.edges[0].first == 0 && d.edges[0].second == 1 ? 0 : 1; }\n")
     exe = tmp_path / "use"
     subprocess.run(["g++", "-std=c++14", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), str(src), "-o",

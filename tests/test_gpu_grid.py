@@ -251,3 +251,75 @@ def test_mfi_instance(tmp_path, conn):
         assert g.max_flow() == info["maxflow"]
         assert np.array_equal(g.source_set(), ref_smin)
         assert np.array_equal(g.fill_segments(1, 0), ref_tmin)
+
+
+def frontend_caps(vol, markers, w, conn):
+    """numpy restatement of the device front end (t-link tables, marker then border forcing,
+    n-link tables by |I_v - I_u| and neighbour class)."""
+    from tests.gpu_util import HALF6, HALF26, _shift
+    vol = vol.astype(np.int64)
+    nz, ny, nx = vol.shape
+    cs = np.array(w.tlink_source, np.int64)[vol]
+    ct = np.array(w.tlink_sink, np.int64)[vol]
+    if markers is not None:
+        src = markers == w.marker_source
+        snk = (markers == w.marker_sink) & ~src
+        cs[src], ct[src] = w.marker_capacity, 0
+        cs[snk], ct[snk] = 0, w.marker_capacity
+    b = w.border
+    if b > 0:
+        z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+        m = (x < b) | (y < b) | (z < b) | (x >= nx - b) | (y >= ny - b) | (z >= nz - b)
+        cs[m], ct[m] = w.marker_capacity, 0
+    table = np.array([list(w.nlink[c]) for c in range(3)], np.int64)
+    offs = HALF6 if conn == 6 else HALF26
+    cn = np.zeros((len(offs),) + vol.shape, np.int64)
+    for j, (dx, dy, dz) in enumerate(offs):
+        nb = _shift(vol, dx, dy, dz, -1)
+        cls = (dx != 0) + (dy != 0) + (dz != 0) - 1
+        inside = nb >= 0
+        cn[j][inside] = table[cls][np.abs(vol - nb)[inside]]
+    return cs.astype(np.int32), ct.astype(np.int32), cn.astype(np.int32)
+
+
+@pytest.mark.parametrize("conn", [6, 26])
+def test_device_front_end_default_tables(conn):
+    """Capacities computed on the device from the volume equal the host generator's, bit for bit,
+    and the solve matches the one from host capacities."""
+    say(f"grid device front end conn {conn}")
+    vol, cs, ct, cn = cugip_amd.make_instance(61, 47, 35, 21, 0, conn)
+    with cugip_amd.GridGraph() as g:
+        g.set_grid_volume(vol, conn=conn)
+        dcs, dct, dcn, dcnb = g.capacities()
+        assert np.array_equal(dcs, cs) and np.array_equal(dct, ct)
+        assert np.array_equal(dcn, cn) and np.array_equal(dcnb, cn)
+        flow = g.max_flow()
+        smin = g.source_set()
+    ref_flow, ref_smin, _ = oracle.grid_solve("bk", 61, 47, 35, conn, cs, ct, cn)
+    assert flow == ref_flow and np.array_equal(smin, ref_smin)
+
+
+def test_device_front_end_markers_and_border():
+    """The reference's scheme: zero data t-links, markers 128 (source) / 255 (sink), border 2
+    forced to the source, contrast n-links; checked against the numpy restatement and the oracle."""
+    say("grid device front end markers")
+    rng = np.random.default_rng(3)
+    vol = cugip_amd.generate_volume(40, 36, 28, 5)
+    markers = np.zeros(vol.shape, np.uint8)
+    markers[rng.random(vol.shape) < 0.01] = 128
+    markers[(rng.random(vol.shape) < 0.01) & (vol > 150)] = 255
+    w = cugip_amd.default_volume_weights()
+    for i in range(256):
+        w.tlink_source[i] = 0
+        w.tlink_sink[i] = 0
+    w.marker_source, w.marker_sink, w.marker_capacity, w.border = 128, 255, 1000, 2
+    cs, ct, cn = frontend_caps(vol, markers, w, 6)
+    with cugip_amd.GridGraph() as g:
+        g.set_grid_volume(vol, markers, w, conn=6)
+        dcs, dct, dcn, _ = g.capacities()
+        assert np.array_equal(dcs, cs) and np.array_equal(dct, ct) and np.array_equal(dcn, cn)
+        flow = g.max_flow()
+        smin = g.source_set()
+    nz, ny, nx = vol.shape
+    ref_flow, ref_smin, _ = oracle.grid_solve("bk", nx, ny, nz, 6, cs, ct, cn)
+    assert flow == ref_flow and np.array_equal(smin, ref_smin)
