@@ -146,6 +146,8 @@ _lib.cgp_default_volume_weights.argtypes = [ctypes.POINTER(VolumeWeights)]
 _lib.cgp_set_grid_volume.argtypes = [_vp] + [ctypes.c_int32] * 4 + [_vp, _vp, ctypes.POINTER(VolumeWeights),
                                                                     ctypes.c_int32]
 _lib.cgp_grid_capacities.argtypes = [_vp, ctypes.c_int32, _i32p, _i32p, _i32p, _i32p]
+_lib.cgp_grid_state.argtypes = [_vp, ctypes.c_int32, _i32p, _i32p, ctypes.POINTER(ctypes.c_uint32)]
+_lib.cgp_save_graphml.argtypes = [_vp, ctypes.c_char_p]
 _lib.cgp_set_grid_batch.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(GridInstance),
                                     ctypes.c_int32]
 _lib.cgp_batch_size.argtypes = [_vp, _i32p]
@@ -283,6 +285,21 @@ class _Handle:
         out = np.empty(self._n, np.int32)
         self._check(_lib.cgp_labels(self._h, _p32(out)))
         return out
+
+    def save_graphml(self, path):
+        """GraphML dump of the graph (Graph::save_to_graphml)."""
+        self._check(_lib.cgp_save_graphml(self._h, os.fsencode(path)))
+
+    def grid_state(self, index=0):
+        """(excess, labels, saturated direction bits) of grid `index` once a solve has started,
+        e.g. from a trace hook; flat arrays in voxel order."""
+        n = int(np.prod(self.shapes[index])) if hasattr(self, "shapes") else self._n
+        e = np.empty(n, np.int32)
+        lab = np.empty(n, np.int32)
+        sat = np.empty(n, np.uint32)
+        self._check(_lib.cgp_grid_state(self._h, index, _p32(e), _p32(lab),
+                                        sat.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))))
+        return e, lab, sat
 
 
 class Graph(_Handle):

@@ -402,6 +402,36 @@ void CsrSolver::set_tweights(const int32_t *source, const int32_t *sink)
 	mSolved = false;
 }
 
+void CsrSolver::export_graph(std::vector<int32_t> &tail, std::vector<int32_t> &head, std::vector<int32_t> &cap,
+                             std::vector<int32_t> &cs, std::vector<int32_t> &ct)
+{
+	if (!ready()) {
+		fail(CGP_ERR_STATE, "no complete graph set");
+	}
+	std::vector<int32_t> heads((size_t)mArcs), caps((size_t)mArcs);
+	if (mArcs) {
+		CGP_HIP(hipMemcpyAsync(heads.data(), mHead.get(), mArcs * sizeof(int32_t), hipMemcpyDeviceToHost, mStream));
+		CGP_HIP(hipMemcpyAsync(caps.data(), mArcCap.get(), mArcs * sizeof(int32_t), hipMemcpyDeviceToHost, mStream));
+	}
+	cs.resize((size_t)mN);
+	ct.resize((size_t)mN);
+	CGP_HIP(hipMemcpyAsync(cs.data(), mCs.get(), mN * sizeof(int32_t), hipMemcpyDeviceToHost, mStream));
+	CGP_HIP(hipMemcpyAsync(ct.data(), mCt.get(), mN * sizeof(int32_t), hipMemcpyDeviceToHost, mStream));
+	CGP_HIP(hipStreamSynchronize(mStream));
+	tail.clear();
+	head.clear();
+	cap.clear();
+	for (int v = 0; v < mN; ++v) {
+		for (int a = mHostFirst[(size_t)v]; a < mHostFirst[(size_t)v + 1]; ++a) {
+			if (caps[(size_t)a] > 0) {
+				tail.push_back(v);
+				head.push_back(heads[(size_t)a]);
+				cap.push_back(caps[(size_t)a]);
+			}
+		}
+	}
+}
+
 int32_t CsrSolver::read_i32(const int32_t *device_ptr)
 {
 	CGP_HIP(hipMemcpyAsync(mPinned.get(), device_ptr, sizeof(int32_t), hipMemcpyDeviceToHost, mStream));

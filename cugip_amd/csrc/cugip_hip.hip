@@ -1,5 +1,6 @@
 // C ABI of the graph cut (include/cugip_hip.h): handle management, argument
 // checks, error capture, and dispatch to the grid or the generic (CSR) solver.
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -7,6 +8,7 @@
 #include <vector>
 
 #include "common.h"
+#include "grid_dirs.h"
 #include "mfi.h"
 #include "solver.h"
 
@@ -321,6 +323,88 @@ int cgp_grid_capacities(cgp_graph g, int32_t index, int32_t *cap_source, int32_t
 			cgp::fail(CGP_ERR_INVALID, "null argument");
 		}
 		grid_of(g).capacities(index, cap_source, cap_sink, cap_n, cap_n_back);
+	});
+}
+
+int cgp_grid_state(cgp_graph g, int32_t index, int32_t *excess, int32_t *labels, uint32_t *saturated)
+{
+	return guarded(g, [&] { grid_of(g).state(index, excess, labels, saturated); });
+}
+
+int cgp_save_graphml(cgp_graph g, const char *path)
+{
+	return guarded(g, [&] {
+		if (!path) {
+			cgp::fail(CGP_ERR_INVALID, "null path");
+		}
+		std::vector<int32_t> tail, head, cap, cs, ct;
+		if (g->mode == cgp_graph_s::Mode::Grid) {
+			cgp::GridSolver &grid = single_grid(g);
+			const cgp::InstanceDesc &d = grid.grid(0);
+			const int conn = grid.conn(), half = conn / 2;
+			const size_t n = (size_t)d.nvox;
+			cs.resize(n);
+			ct.resize(n);
+			std::vector<int32_t> cn(n * half), cnb(n * half);
+			grid.capacities(0, cs.data(), ct.data(), cn.data(), cnb.data());
+			const int8_t(*offs)[3] = conn == 6 ? CGP_HALF_OFFSETS_6 : CGP_HALF_OFFSETS_26;
+			for (int j = 0; j < half; ++j) {
+				for (int z = 0; z < d.nz; ++z) {
+					for (int y = 0; y < d.ny; ++y) {
+						for (int x = 0; x < d.nx; ++x) {
+							const int ux = x + offs[j][0], uy = y + offs[j][1], uz = z + offs[j][2];
+							if (ux < 0 || ux >= d.nx || uy < 0 || uy >= d.ny || uz < 0 || uz >= d.nz) {
+								continue;
+							}
+							const int32_t v = (int32_t)(((int64_t)z * d.ny + y) * d.nx + x);
+							const int32_t u = (int32_t)(((int64_t)uz * d.ny + uy) * d.nx + ux);
+							const int32_t f = cn[(size_t)j * n + (size_t)v], b = cnb[(size_t)j * n + (size_t)v];
+							if (f > 0) {
+								tail.push_back(v);
+								head.push_back(u);
+								cap.push_back(f);
+							}
+							if (b > 0) {
+								tail.push_back(u);
+								head.push_back(v);
+								cap.push_back(b);
+							}
+						}
+					}
+				}
+			}
+		} else {
+			generic(g).export_graph(tail, head, cap, cs, ct);
+		}
+		std::FILE *f = std::fopen(path, "w");
+		if (!f) {
+			cgp::fail(CGP_ERR_INVALID, std::string(path) + ": cannot create");
+		}
+		std::fprintf(f, "<?xml version=\"1.0\" encoding=\"UTF-8\"?>\n"
+		                "<graphml xmlns=\"http://graphml.graphdrawing.org/xmlns\">\n"
+		                "<key id=\"capacity\" for=\"edge\" attr.name=\"capacity\" attr.type=\"long\"/>\n"
+		                "<graph id=\"G\" edgedefault=\"directed\">\n<node id=\"source\"/>\n<node id=\"sink\"/>\n");
+		for (size_t i = 0; i < cs.size(); ++i) {
+			std::fprintf(f, "<node id=\"n%zu\"/>\n", i);
+		}
+		const char *edge = "<edge source=\"%s%d\" target=\"%s%d\"><data key=\"capacity\">%d</data></edge>\n";
+		for (size_t i = 0; i < tail.size(); ++i) {
+			std::fprintf(f, edge, "n", tail[i], "n", head[i], cap[i]);
+		}
+		for (size_t i = 0; i < cs.size(); ++i) {
+			if (cs[i] > 0) {
+				std::fprintf(f, "<edge source=\"source\" target=\"n%zu\"><data key=\"capacity\">%d</data></edge>\n", i,
+				             cs[i]);
+			}
+			if (ct[i] > 0) {
+				std::fprintf(f, "<edge source=\"n%zu\" target=\"sink\"><data key=\"capacity\">%d</data></edge>\n", i,
+				             ct[i]);
+			}
+		}
+		const bool ok = std::fprintf(f, "</graph>\n</graphml>\n") > 0;
+		if (std::fclose(f) != 0 || !ok) {
+			cgp::fail(CGP_ERR_INVALID, std::string(path) + ": write failed");
+		}
 	});
 }
 

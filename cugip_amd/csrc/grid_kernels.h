@@ -703,6 +703,42 @@ __global__ __launch_bounds__(256) void k_caps_from_volume(int nx, int ny, int nz
 	}
 }
 
+// Saturated-direction bits of every voxel of one grid: bit d when the edge towards direction d
+// has capacity > 0 and residual 0 (capacities as in the init kernel).
+template <int kConn>
+__global__ __launch_bounds__(256) void k_saturated_bits(int nx, int ny, int nz, const int32_t *__restrict__ r,
+                                                        const int32_t *__restrict__ cn,
+                                                        const int32_t *__restrict__ cnb, uint32_t *__restrict__ out)
+{
+	using D = Dirs<kConn>;
+	const int64_t n = (int64_t)nx * ny * nz;
+	const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+	for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+		const int x = (int)(v % nx);
+		const int y = (int)((v / nx) % ny);
+		const int z = (int)(v / ((int64_t)nx * ny));
+		uint32_t bits = 0;
+#pragma unroll
+		for (int d = 0; d < D::K; ++d) {
+			const int ux = x + D::dx(d), uy = y + D::dy(d), uz = z + D::dz(d);
+			if (ux < 0 || ux >= nx || uy < 0 || uy >= ny || uz < 0 || uz >= nz) {
+				continue;
+			}
+			int32_t cap;
+			if (d < D::H) {
+				cap = cn[(int64_t)d * n + v];
+			} else {
+				const int64_t u = ((int64_t)uz * ny + uy) * nx + ux;
+				cap = (cnb ? cnb : cn)[(int64_t)(d - D::H) * n + u];
+			}
+			if (cap > 0 && r[(int64_t)d * n + v] == 0) {
+				bits |= 1u << d;
+			}
+		}
+		out[v] = bits;
+	}
+}
+
 __global__ __launch_bounds__(256) void k_fill_u8(uint8_t *__restrict__ p, int64_t n, uint8_t value)
 {
 	const int64_t stride = (int64_t)gridDim.x * blockDim.x;

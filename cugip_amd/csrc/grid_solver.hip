@@ -220,6 +220,7 @@ void GridSolver::allocate_grids(int conn, const std::vector<InstanceDesc> &inst,
 	const int half = conn / 2;
 	const size_t count = inst.size();
 	mSolved = false;
+	mStarted = false;
 	mInst = inst;
 	mFlows.assign(count, 0);
 	mNx = inst[0].nx;
@@ -690,6 +691,7 @@ int64_t GridSolver::solve(const cgp_policy &policy, cgp_stats *stats_out, const 
 	hipLaunchKernelGGL(k_fill_u8, dim3(1), dim3(64), 0, mStream, reinterpret_cast<uint8_t *>(mCounters.get()),
 	                   (int64_t)32, (uint8_t)0);
 	launch_init();
+	mStarted = true;
 	{
 		int32_t *pinned = reinterpret_cast<int32_t *>(mHostScratch.get());
 		CGP_HIP(hipMemcpyAsync(pinned, mStatus.get(), sizeof(int32_t), hipMemcpyDeviceToHost, mStream));
@@ -813,6 +815,45 @@ void GridSolver::labels(int index, int32_t *host_out)
 		CGP_LAUNCH_CHECK();
 	}
 	CGP_HIP(hipMemcpyAsync(host_out, mH.get() + d.vbase, d.nvox * sizeof(int32_t), hipMemcpyDeviceToHost, mStream));
+	CGP_HIP(hipStreamSynchronize(mStream));
+}
+
+void GridSolver::state(int index, int32_t *excess, int32_t *labels, uint32_t *saturated)
+{
+	if (!mStarted) {
+		fail(CGP_ERR_STATE, "no solve has started on this grid");
+	}
+	const InstanceDesc &d = instance(index);
+	if (excess) {
+		CGP_HIP(hipMemcpyAsync(excess, mE.get() + d.vbase, d.nvox * sizeof(int32_t), hipMemcpyDeviceToHost, mStream));
+	}
+	if (saturated) {
+		DeviceBuffer<uint32_t> bits(d.nvox);
+		const int half = mConn / 2;
+		const int32_t *r = mR.get() + d.vbase * mConn;
+		const int32_t *cn = mCn.get() + d.vbase * half;
+		const int32_t *cnb = mHasBack ? mCnb.get() + d.vbase * half : nullptr;
+		const int blocks = blocks_for(d.nvox, 256, 8192);
+		if (mConn == 6) {
+			hipLaunchKernelGGL(k_saturated_bits<6>, dim3(blocks), dim3(256), 0, mStream, d.nx, d.ny, d.nz, r, cn, cnb,
+			                   bits.get());
+		} else {
+			hipLaunchKernelGGL(k_saturated_bits<26>, dim3(blocks), dim3(256), 0, mStream, d.nx, d.ny, d.nz, r, cn, cnb,
+			                   bits.get());
+		}
+		CGP_LAUNCH_CHECK();
+		CGP_HIP(hipMemcpyAsync(saturated, bits.get(), d.nvox * sizeof(uint32_t), hipMemcpyDeviceToHost, mStream));
+		CGP_HIP(hipStreamSynchronize(mStream));
+	}
+	if (labels) {
+		if (mWave) {
+			GridView g = CGP_VIEW();
+			hipLaunchKernelGGL(wave::k_wave_bfs, dim3(mTiles), dim3(64), 0, mStream, g, mBits.get(), mTiles,
+			                   mAllTiles.get(), mAllTiles.get() + mTiles, 0, mDirtyA.get(), mActive.get(), mH.get());
+			CGP_LAUNCH_CHECK();
+		}
+		CGP_HIP(hipMemcpyAsync(labels, mH.get() + d.vbase, d.nvox * sizeof(int32_t), hipMemcpyDeviceToHost, mStream));
+	}
 	CGP_HIP(hipStreamSynchronize(mStream));
 }
 

@@ -43,6 +43,9 @@ public:
 	void set_grid_volume(int nx, int ny, int nz, int conn, const uint8_t *volume, const uint8_t *markers,
 	                     const cgp_volume_weights &weights, bool device_pointers);
 	void capacities(int index, int32_t *cs, int32_t *ct, int32_t *cn, int32_t *cnb);
+	void state(int index, int32_t *excess, int32_t *labels, uint32_t *saturated);
+	int conn() const { return mConn; }
+	const InstanceDesc &grid(int index) const { return instance(index); }
 	// Returns the total flow of all instances; per-instance values in flows().
 	int64_t solve(const cgp_policy &policy, cgp_stats *stats, const TraceHook &trace);
 	const std::vector<int64_t> &flows() const { return mFlows; }
@@ -97,6 +100,7 @@ private:
 	Shape mShape{0, 0, 0};
 	int mNtx = 0, mNty = 0, mNtz = 0, mTiles = 0, mMaxColourTiles = 0;
 	bool mSolved = false;
+	bool mStarted = false;  // a solve has initialised the residual state
 	bool mHasBack = false;
 
 	DeviceBuffer<int32_t> mCs, mCt, mCn, mCnb;
@@ -130,6 +134,9 @@ public:
 	int64_t vertex_count() const { return mN; }
 	bool solved() const { return mSolved; }
 	bool ready() const { return mN > 0 && mHaveTweights; }
+	// Directed edges with positive capacity (tail, head, capacity) and the t-links, for export.
+	void export_graph(std::vector<int32_t> &tail, std::vector<int32_t> &head, std::vector<int32_t> &cap,
+	                  std::vector<int32_t> &cs, std::vector<int32_t> &ct);
 
 private:
 	int bfs(int32_t threshold, std::vector<int32_t> &level_starts, cgp_stats &stats);

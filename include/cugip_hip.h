@@ -168,6 +168,21 @@ int cgp_source_set(cgp_graph graph, uint8_t *out);
 #define CGP_LABEL_UNREACHABLE 0x3fffffff
 int cgp_labels(cgp_graph graph, int32_t *out);
 
+/* ---- diagnostics and export -----------------------------------------------------------------
+   What the reference's benchmark trace object reads from GraphCutData during and after a solve
+   (benchmarks/graph_cut/graph_cut.cu:82-177: saturated edges, excess, labels), and the GraphML
+   dump of Graph::save_to_graphml (advanced_operations/graph_to_graphml.hpp:6-61). */
+/* State of grid `index` (0 for a single grid) once a solve has started, e.g. from a trace hook:
+   excess (int32), labels (int32, CGP_LABEL_UNREACHABLE when t is unreachable; the exact labels of
+   the last global relabel) and saturated directions (uint32 bit d set when the edge to the
+   neighbour in residual direction d has capacity > 0 and residual 0; d < conn/2 are the offsets of
+   cap_n, d >= conn/2 their mirrors). Any output may be NULL. */
+int cgp_grid_state(cgp_graph graph, int32_t index, int32_t *excess, int32_t *labels, uint32_t *saturated);
+/* Writes the graph (generic or single grid) as GraphML: nodes "source", "sink", "n<i>"; one
+   directed edge per positive capacity (both directions of every n-link, source -> n<i>,
+   n<i> -> sink), capacity in the "capacity" key. */
+int cgp_save_graphml(cgp_graph graph, const char *path);
+
 /* ---- profiling -----------------------------------------------------------------------------
    When enabled, the grid solver brackets every kernel launch with HIP events on its own stream
    and waits for each launch, so per-kernel device time and algorithmic bytes can be read back

@@ -323,3 +323,58 @@ def test_device_front_end_markers_and_border():
     nz, ny, nx = vol.shape
     ref_flow, ref_smin, _ = oracle.grid_solve("bk", nx, ny, nz, 6, cs, ct, cn)
     assert flow == ref_flow and np.array_equal(smin, ref_smin)
+
+
+def test_grid_state_and_graphml(tmp_path):
+    """State volumes after a solve: every voxel with excess is in S_min, labels equal cgp_labels,
+    and every edge leaving S_min is saturated (its direction bit is set). The GraphML dump, parsed
+    back, has the same max flow as the grid."""
+    import xml.etree.ElementTree as ET
+    from tests.gpu_util import HALF6
+    say("grid state and graphml")
+    nx, ny, nz = 19, 13, 11
+    vol, cs, ct, cn = cugip_amd.make_instance(nx, ny, nz, 33)
+    seen = []
+    with cugip_amd.GridGraph() as g:
+        g.set_grid(cs, ct, cn)
+
+        def hook(event, iteration, done, flow, levels):
+            if event == cugip_amd.TRACE_AFTER_PUSH and not seen:
+                seen.append(g.grid_state()[0].copy())
+
+        g.set_trace(hook)
+        flow = g.max_flow()
+        g.set_trace(None)
+        smin = g.source_set().astype(bool)
+        excess, labels, sat = g.grid_state()
+        assert np.array_equal(labels, g.labels())
+        path = str(tmp_path / "g.graphml")
+        g.save_graphml(path)
+    assert seen and seen[0].shape == (nx * ny * nz,)
+    assert not (excess > 0)[~smin].any()
+    S = smin.reshape(nz, ny, nx)
+    satv = sat.reshape(nz, ny, nx)
+    for j, (dx, dy, dz) in enumerate(HALF6):
+        z, y, x = np.nonzero(S)
+        ux, uy, uz = x + dx, y + dy, z + dz
+        ok = (ux >= 0) & (uy >= 0) & (uz >= 0)
+        leaving = ok.copy()
+        leaving[ok] = ~S[uz[ok], uy[ok], ux[ok]] & (cn[j][z[ok], y[ok], x[ok]] > 0)
+        assert ((satv[z[leaving], y[leaving], x[leaving]] >> j) & 1).all()
+    ns = {"g": "http://graphml.graphdrawing.org/xmlns"}
+    root = ET.parse(path).getroot()
+    ea, eb, cap, src, snk = [], [], [], np.zeros(nx * ny * nz, np.int64), np.zeros(nx * ny * nz, np.int64)
+    for e in root.iter("{%s}edge" % ns["g"]):
+        a, b = e.get("source"), e.get("target")
+        c = int(e.find("g:data", ns).text)
+        if a == "source":
+            src[int(b[1:])] = c
+        elif b == "sink":
+            snk[int(a[1:])] = c
+        else:
+            ea.append(int(a[1:]))
+            eb.append(int(b[1:]))
+            cap.append(c)
+    assert len(root.findall(".//g:node", ns)) == nx * ny * nz + 2
+    ref = oracle.dinic(nx * ny * nz, np.array(ea), np.array(eb), np.array(cap), np.zeros(len(cap)), src, snk)[0]
+    assert ref == flow
