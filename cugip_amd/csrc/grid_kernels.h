@@ -27,7 +27,8 @@ namespace cgp {
 
 // ---------------------------------------------------------------------------------------------
 // init: residual planes, cancellation, excess, residual sink link. Sets status bit 0 for a negative
-// capacity and bit 1 when a voxel's inflow or an edge's capacity pair does not fit int32. One
+// capacity, bit 1 when a voxel's inflow or an edge's capacity pair does not fit int32, and bit 2
+// when some edge's capacity pair exceeds 32767 (then residuals cannot be staged as int16). One
 // workgroup per tile (grid-stride over the tiles of every instance), thread (lx, ly) owning a
 // z-column, so x stays the coalesced axis.
 template <int kConn, int kTX, int kTY, int kTZ>
@@ -46,7 +47,7 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_init(GridView gv, int ntiles)
 		tile_coords(g, local, tx, ty, tz);
 		const int x = tx * T::TX + lx, y = ty * T::TY + ly;
 		long long cancelled = 0;
-		bool negative = false, overflow = false;
+		bool negative = false, overflow = false, wide = false;
 #pragma unroll
 		for (int k = 0; k < T::TZ; ++k) {
 			const int z = tz * T::TZ + k;
@@ -74,6 +75,7 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_init(GridView gv, int ntiles)
 					}
 					negative |= (out < 0) | (in < 0);
 					overflow |= ((long long)out + (long long)in) > 0x7fffffffLL;
+					wide |= ((long long)out + (long long)in) > 0x7fffLL;
 					incoming += in;
 				}
 				g.r[(int64_t)d * g.nvox + v] = out;
@@ -82,8 +84,8 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_init(GridView gv, int ntiles)
 			g.e[v] = cs - m;
 			g.rt[v] = ct - m;
 		}
-		if (negative || overflow) {
-			atomicOr(g.status, (negative ? 1 : 0) | (overflow ? 2 : 0));
+		if (negative || overflow || wide) {
+			atomicOr(g.status, (negative ? 1 : 0) | (overflow ? 2 : 0) | (wide ? 4 : 0));
 		}
 		const long long total = block_sum_i64<T::NT>(cancelled, red);
 		if (tid == 0 && total) {
@@ -274,13 +276,34 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_bfs(GridView gv, const int32_
 // ---------------------------------------------------------------------------------------------
 // Discharge one tile: repeated {region relabel, top-down level push} in LDS, then write back.
 // Runs only on tiles of one colour at a time, so no neighbouring tile is running.
-template <int kConn, int kTX, int kTY, int kTZ>
-__global__ __launch_bounds__(kTX *kTY) void k_grid_discharge(GridView gv, const int32_t *__restrict__ list,
+// Residuals are staged in LDS as RT: int16_t when every edge's capacity pair fits (a residual never
+// exceeds it), int32_t otherwise. For 6-connected grids the directions written during the visit
+// are kept as one byte per voxel, four to a 32-bit word, so that LDS atomics can set them.
+template <int kConn>
+struct DirtyBits {
+	static constexpr int kWords(int nv) { return kConn <= 8 ? (nv + 3) / 4 : nv; }
+	__device__ static void set(uint32_t *dm, int i, int d)
+	{
+		if (kConn <= 8) {
+			atomicOr(&dm[i >> 2], 1u << (8 * (i & 3) + d));
+		} else {
+			atomicOr(&dm[i], 1u << d);
+		}
+	}
+	__device__ static uint32_t get(const uint32_t *dm, int i)
+	{
+		return kConn <= 8 ? (dm[i >> 2] >> (8 * (i & 3))) & 0xffu : dm[i];
+	}
+};
+
+template <int kConn, int kTX, int kTY, int kTZ, typename RT>
+__global__ __launch_bounds__(kTX *kTY, 6) void k_grid_discharge(GridView gv, const int32_t *__restrict__ list,
                                                              const int32_t *__restrict__ count, int max_iters,
                                                              uint8_t *__restrict__ tile_active)
 {
 	using T = Tile<kConn, kTX, kTY, kTZ>;
 	using D = typename T::D;
+	using Dirty = DirtyBits<kConn>;
 	constexpr int K = T::K;
 	if ((int)blockIdx.x >= *count) {
 		return;
@@ -294,8 +317,8 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_discharge(GridView gv, const 
 
 	__shared__ int32_t hl[T::NP];
 	__shared__ int32_t el[T::NV];
-	__shared__ int32_t rl[K][T::NV];
-	__shared__ uint32_t dm[T::NV];   // residual directions written during this visit
+	__shared__ RT rl[K][T::NV];
+	__shared__ uint32_t dm[Dirty::kWords(T::NV)];   // residual directions written during this visit
 	__shared__ unsigned long long key;
 	__shared__ long long red[T::NT / 64];
 	__shared__ uint8_t marks[27];
@@ -309,6 +332,9 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_discharge(GridView gv, const 
 	if (tid == 0) {
 		key = 0;
 	}
+	for (int w = tid; w < Dirty::kWords(T::NV); w += T::NT) {
+		dm[w] = 0;
+	}
 	load_label_tile<T>(g, hl, ox, oy, oz);
 
 	bool valid[T::TZ];
@@ -318,7 +344,6 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_discharge(GridView gv, const 
 	for (int k = 0; k < T::TZ; ++k) {
 		valid[k] = gx < g.nx && gy < g.ny && oz + k < g.nz;
 		const int i = T::vidx(lx, ly, k);
-		dm[i] = 0;
 		int32_t ev = 0, rv = 0;
 		if (valid[k]) {
 			const int64_t v = voxel_index(g, gx, gy, oz + k);
@@ -326,7 +351,7 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_discharge(GridView gv, const 
 			rv = g.rt[v];
 #pragma unroll
 			for (int d = 0; d < K; ++d) {
-				rl[d][i] = g.r[(int64_t)d * g.nvox + v];
+				rl[d][i] = (RT)g.r[(int64_t)d * g.nvox + v];
 			}
 		} else {
 #pragma unroll
@@ -422,14 +447,14 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_discharge(GridView gv, const 
 						const int32_t rv = rl[d][i];
 						if (rv > 0 && hl[p + T::poff(d)] < level) {
 							const int32_t f = ex < rv ? ex : rv;
-							rl[d][i] = rv - f;
+							rl[d][i] = (RT)(rv - f);
 							ex -= f;
-							atomicOr(&dm[i], 1u << d);
+							Dirty::set(dm, i, d);
 							const int nx = lx + D::dx(d), ny = ly + D::dy(d), nz = k + D::dz(d);
 							if (nx >= 0 && nx < T::TX && ny >= 0 && ny < T::TY && nz >= 0 && nz < T::TZ) {
 								const int j = i + T::voff(d);
-								rl[opposite<kConn>(d)][j] += f;
-								atomicOr(&dm[j], 1u << opposite<kConn>(d));
+								rl[opposite<kConn>(d)][j] = (RT)(rl[opposite<kConn>(d)][j] + f);
+								Dirty::set(dm, j, opposite<kConn>(d));
 								atomicAdd(&el[j], f);
 								next = level - 1 > next ? level - 1 : next;
 							}
@@ -466,7 +491,7 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_discharge(GridView gv, const 
 		if (lab != h0[k]) {
 			g.h[v] = lab;
 		}
-		const uint32_t dirty = dm[i];
+		const uint32_t dirty = Dirty::get(dm, i);
 		if (dirty) {
 #pragma unroll
 			for (int d = 0; d < K; ++d) {

@@ -577,12 +577,20 @@ void GridSolver::run_discharge_phase(int colour, int max_iters)
 		});
 		return;
 	}
-	// reads and writes e, rt, h and r[K]
+	// reads and writes e, rt, h and r[K]; residuals staged in LDS as int16 unless some edge's
+	// capacity pair exceeds 32767
 	launch(kDischarge, -1, 8.0 * (mConn + 3), [&] {
 		dispatch_shape(mConn, mShape.tx, mShape.ty, mShape.tz, [&](auto tag) {
 			using S = decltype(tag);
-			hipLaunchKernelGGL((k_grid_discharge<S::conn, S::tx, S::ty, S::tz>), dim3(blocks), dim3(S::tx * S::ty), 0,
-			                   mStream, g, mList.get(), mCount.get(), max_iters, mActive.get());
+			if (mWideResiduals) {
+				hipLaunchKernelGGL((k_grid_discharge<S::conn, S::tx, S::ty, S::tz, int32_t>), dim3(blocks),
+				                   dim3(S::tx * S::ty), 0, mStream, g, mList.get(), mCount.get(), max_iters,
+				                   mActive.get());
+			} else {
+				hipLaunchKernelGGL((k_grid_discharge<S::conn, S::tx, S::ty, S::tz, int16_t>), dim3(blocks),
+				                   dim3(S::tx * S::ty), 0, mStream, g, mList.get(), mCount.get(), max_iters,
+				                   mActive.get());
+			}
 		});
 	});
 }
@@ -703,6 +711,7 @@ int64_t GridSolver::solve(const cgp_policy &policy, cgp_stats *stats_out, const 
 			fail(CGP_ERR_OVERFLOW, "each voxel's source capacity plus incoming capacities, and each edge's two "
 			                       "capacities, must sum below 2^31");
 		}
+		mWideResiduals = (pinned[0] & 4) != 0;
 	}
 	stats.init_ms = ms_since(t_start);
 	trace.emit(CGP_TRACE_STARTED, 0, 0, 0);

@@ -102,6 +102,7 @@ private:
 	bool mSolved = false;
 	bool mStarted = false;  // a solve has initialised the residual state
 	bool mHasBack = false;
+	bool mWideResiduals = true;  // some edge's capacity pair exceeds 32767: stage residuals as int32
 
 	DeviceBuffer<int32_t> mCs, mCt, mCn, mCnb;
 	DeviceBuffer<int32_t> mE, mRt, mH, mR;
