@@ -102,9 +102,13 @@ def main(argv=None):
     ap.add_argument("--repeats", type=int, default=3)
     ap.add_argument("--out", default=None, help="write all timings as JSON")
     ap.add_argument("--write-header", action="store_true", help="rewrite csrc/tuned_policy.h")
+    ap.add_argument("--from-json", default=None, help="take the timings of an earlier --out file")
     args = ap.parse_args(argv)
     tuned = []
-    for name in args.workloads.split(","):
+    if args.from_json:
+        with open(args.from_json) as f:
+            tuned = json.load(f)
+    for name in ([] if args.from_json else args.workloads.split(",")):
         tuned.append(tune_workload(name, args.repeats, log=lambda *a: print(*a, flush=True)))
         b = tuned[-1]["best"]
         print("best %s: %s" % (name, json.dumps(b)), flush=True)

@@ -25,6 +25,13 @@
 
 namespace cgp {
 
+// Per-voxel state word (GridView::omask): bit d set while the residual in direction d is positive,
+// plus whether the residual sink link and the excess are positive. The init kernel writes it,
+// the discharge keeps it current (its own voxels at write-back, atomic ORs into the halo voxels it
+// pushes to), and the relabel and closure kernels read it instead of the K residual planes.
+constexpr uint32_t kStateSink = 1u << 29;
+constexpr uint32_t kStateExcess = 1u << 30;
+
 // ---------------------------------------------------------------------------------------------
 // init: residual planes, cancellation, excess, residual sink link. Sets status bit 0 for a negative
 // capacity, bit 1 when a voxel's inflow or an edge's capacity pair does not fit int32, and bit 2
@@ -60,6 +67,7 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_init(GridView gv, int ntiles)
 			const int32_t m = cs < ct ? cs : ct;
 			cancelled += m;
 			long long incoming = cs - m;
+			uint32_t state = (cs - m > 0 ? kStateExcess : 0u) | (ct - m > 0 ? kStateSink : 0u);
 #pragma unroll
 			for (int d = 0; d < K; ++d) {
 				const int ux = x + D::dx(d), uy = y + D::dy(d), uz = z + D::dz(d);
@@ -79,7 +87,9 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_init(GridView gv, int ntiles)
 					incoming += in;
 				}
 				g.r[(int64_t)d * g.nvox + v] = out;
+				state |= out > 0 ? (1u << d) : 0u;
 			}
+			g.omask[v] = state;
 			overflow |= incoming > 0x7fffffffLL;
 			g.e[v] = cs - m;
 			g.rt[v] = ct - m;
@@ -229,12 +239,22 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_bfs(GridView gv, const int32_
 		before[k] = hl[p];
 		if (valid[k]) {
 			const int64_t v = voxel_index(g, gx, gy, oz + k);
+			bool sink_open;
+			if (threshold == 0) {
+				// exact relabel: the state word has every open direction
+				const uint32_t st = g.omask[v];
+				open[k] = st & ((1u << T::K) - 1u);
+				has_excess[k] = (st & kStateExcess) != 0;
+				sink_open = (st & kStateSink) != 0;
+			} else {
 #pragma unroll
-			for (int d = 0; d < T::K; ++d) {
-				open[k] |= (g.r[(int64_t)d * g.nvox + v] > threshold) ? (1u << d) : 0u;
+				for (int d = 0; d < T::K; ++d) {
+					open[k] |= (g.r[(int64_t)d * g.nvox + v] > threshold) ? (1u << d) : 0u;
+				}
+				has_excess[k] = g.e[v] > 0;
+				sink_open = g.rt[v] > threshold;
 			}
-			has_excess[k] = g.e[v] > 0;
-			if (g.rt[v] > threshold && hl[p] > 1) {
+			if (sink_open && hl[p] > 1) {
 				hl[p] = 1;
 			}
 		}
@@ -492,6 +512,14 @@ __global__ __launch_bounds__(kTX *kTY, 6) void k_grid_discharge(GridView gv, con
 			g.h[v] = lab;
 		}
 		const uint32_t dirty = Dirty::get(dm, i);
+		if (dirty || ev != e0[k] || rtr[k] != rt0[k]) {
+			uint32_t st = (ev > 0 ? kStateExcess : 0u) | (rtr[k] > 0 ? kStateSink : 0u);
+#pragma unroll
+			for (int d = 0; d < K; ++d) {
+				st |= rl[d][i] > 0 ? (1u << d) : 0u;
+			}
+			g.omask[v] = st;
+		}
 		if (dirty) {
 #pragma unroll
 			for (int d = 0; d < K; ++d) {
@@ -508,6 +536,7 @@ __global__ __launch_bounds__(kTX *kTY, 6) void k_grid_discharge(GridView gv, con
 						const int64_t u = voxel_index(g, gx + D::dx(d), gy + D::dy(d), oz + k + D::dz(d));
 						atomicAdd(&g.e[u], f);
 						g.r[(int64_t)opposite<kConn>(d) * g.nvox + u] += f;
+						atomicOr(&g.omask[u], kStateExcess | (1u << opposite<kConn>(d)));
 						marks[slot] = 1;
 					}
 				}
@@ -532,8 +561,8 @@ __global__ __launch_bounds__(kTX *kTY, 6) void k_grid_discharge(GridView gv, con
 // ---------------------------------------------------------------------------------------------
 // S_min closure pass: s(u) = 1 if u has excess or some v with s(v) = 1 has residual v -> u > 0.
 // s starts at 0 (a fill before the first pass) and bits only rise, so concurrently running tiles
-// may set bits in each other's voxels. first_pass seeds from the excess and records the
-// open-direction bits of every voxel.
+// may set bits in each other's voxels. first_pass seeds from the excess; the open directions come
+// from the state word.
 template <int kConn, int kTX, int kTY, int kTZ>
 __global__ __launch_bounds__(kTX *kTY) void k_grid_source_closure(GridView gv, const int32_t *__restrict__ list,
                                                                   const int32_t *__restrict__ count, int first_pass,
@@ -573,17 +602,9 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_source_closure(GridView gv, c
 			const int64_t v = voxel_index(g, gx, gy, oz + k);
 			const uint8_t stored = g.s[v];
 			before[k] = stored;
-			if (first_pass) {
-#pragma unroll
-				for (int d = 0; d < T::K; ++d) {
-					o |= g.r[(int64_t)d * g.nvox + v] > 0 ? (1u << d) : 0u;
-				}
-				g.omask[v] = o;
-				sv = (g.e[v] > 0 || stored) ? 1 : 0;
-			} else {
-				o = g.omask[v];
-				sv = stored;
-			}
+			const uint32_t st = g.omask[v];
+			o = st & ((1u << T::K) - 1u);
+			sv = ((first_pass && (st & kStateExcess)) || stored) ? 1 : 0;
 		}
 		open[k] = o;
 		sl[i] = sv;

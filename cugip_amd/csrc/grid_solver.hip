@@ -473,8 +473,8 @@ static GridView batch_view(GridView g, const InstanceDesc *inst, const int32_t *
 void GridSolver::launch_init()
 {
 	GridView g = CGP_VIEW();
-	// reads cs, ct and the n-link planes, writes e, rt and the K residual planes
-	const double bytes = 4.0 * (2 + (mConn / 2) * (mHasBack ? 2 : 1)) + 4.0 * (2 + mConn);
+	// reads cs, ct and the n-link planes, writes e, rt, the K residual planes and the state word
+	const double bytes = 4.0 * (2 + (mConn / 2) * (mHasBack ? 2 : 1)) + 4.0 * (3 + mConn);
 	if (mWave) {
 		launch(kInit, mVox, bytes + 1.0, [&] {
 			hipLaunchKernelGGL(wave::k_wave_init, dim3(mTiles), dim3(64), 0, mStream, g, mBits.get(), mTiles);
@@ -542,8 +542,8 @@ int GridSolver::run_global_relabel(int32_t threshold, cgp_stats &stats)
 				                   count, threshold, mDirtyA.get(), mActive.get(), (int32_t *)nullptr);
 			});
 		} else {
-			// reads r[K], rt, e and h, writes h
-			launch(kBfs, (int64_t)blocks * tile_voxels, 4.0 * (mConn + 4), [&] {
+			// exact passes read the state word and h and write h; threshold passes read r[K], rt, e, h
+			launch(kBfs, (int64_t)blocks * tile_voxels, threshold == 0 ? 12.0 : 4.0 * (mConn + 4), [&] {
 				dispatch_shape(mConn, mShape.tx, mShape.ty, mShape.tz, [&](auto tag) {
 					using S = decltype(tag);
 					hipLaunchKernelGGL((k_grid_bfs<S::conn, S::tx, S::ty, S::tz>), dim3(blocks), dim3(S::tx * S::ty),
@@ -577,9 +577,9 @@ void GridSolver::run_discharge_phase(int colour, int max_iters)
 		});
 		return;
 	}
-	// reads and writes e, rt, h and r[K]; residuals staged in LDS as int16 unless some edge's
-	// capacity pair exceeds 32767
-	launch(kDischarge, -1, 8.0 * (mConn + 3), [&] {
+	// reads and writes e, rt, h and r[K], writes the state word; residuals staged in LDS as int16
+	// unless some edge's capacity pair exceeds 32767
+	launch(kDischarge, -1, 8.0 * (mConn + 3) + 4.0, [&] {
 		dispatch_shape(mConn, mShape.tx, mShape.ty, mShape.tz, [&](auto tag) {
 			using S = decltype(tag);
 			if (mWideResiduals) {
@@ -627,8 +627,8 @@ void GridSolver::run_source_closure(cgp_stats &stats)
 				                   mSBits.get(), mTiles, list, count, first, mDirtyB.get());
 			});
 		} else {
-			// first pass: reads r[K], e and s, writes the open mask and s; later passes: mask and s
-			const double bytes = first ? 4.0 * (mConn + 1) + 1 + 4 + 1 : 4.0 + 1 + 1;
+			// reads the state word and s, writes s
+			const double bytes = 4.0 + 1 + 1;
 			launch(kClosure, (int64_t)blocks * tile_voxels, bytes, [&] {
 				dispatch_shape(mConn, mShape.tx, mShape.ty, mShape.tz, [&](auto tag) {
 					using S = decltype(tag);

@@ -83,7 +83,7 @@ struct GridView {
 	int32_t *h;              // label: BFS distance to t, kLabelInf if t is unreachable
 	int32_t *r;              // K residual planes, r[d * nvox + v] = residual v -> v + o_d
 	uint8_t *s;              // S_min byte mask
-	uint32_t *omask;         // open-direction bits (r > 0) used by the S_min closure
+	uint32_t *omask;         // state word: open directions (r > 0), residual sink link > 0, excess > 0
 	const int32_t *cs, *ct;  // input t-link capacities
 	const int32_t *cn, *cnb; // input n-link capacities, H planes (cnb may be null)
 	unsigned long long *flow;
