@@ -680,8 +680,10 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_source_closure(GridView gv, c
 // Compact the indices of flagged tiles (optionally of one colour) with a wave64 ballot and one
 // atomic per wave. colours[t] is the colour of tile t inside its instance: 6-connected
 // (tx+ty+tz) & 1, 26-connected (tx&1) | (ty&1)<<1 | (tz&1)<<2.
-__global__ __launch_bounds__(256) void k_compact_tiles(const uint8_t *__restrict__ flags, int ntiles,
-                                                       const uint8_t *__restrict__ colours, int colour,
+// With clear set, the selected flags are reset (the pass that takes the list consumes them); with
+// list null, only the count is taken.
+__global__ __launch_bounds__(256) void k_compact_tiles(uint8_t *__restrict__ flags, int ntiles,
+                                                       const uint8_t *__restrict__ colours, int colour, int clear,
                                                        int32_t *__restrict__ list, int32_t *__restrict__ count)
 {
 	const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -694,7 +696,12 @@ __global__ __launch_bounds__(256) void k_compact_tiles(const uint8_t *__restrict
 	}
 	base = __shfl(base, 0, 64);
 	if (pred) {
-		list[base + (int)__popcll(ballot & ((1ull << lane) - 1ull))] = t;
+		if (list) {
+			list[base + (int)__popcll(ballot & ((1ull << lane) - 1ull))] = t;
+		}
+		if (clear) {
+			flags[t] = 0;
+		}
 	}
 }
 

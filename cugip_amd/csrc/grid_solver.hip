@@ -94,8 +94,11 @@ void GridSolver::launch(Kern kind, int64_t voxels, double bytes_per_voxel, F &&f
 	}
 	unsigned long long *pinned = reinterpret_cast<unsigned long long *>(mHostScratch.get());
 	unsigned long long before = 0;
+	// tile-visit counter of the kernel kind: [0] discharge, [1] relabel, [2] closure
+	const int counter = kind == kBfs ? 1 : (kind == kClosure ? 2 : 0);
 	if (voxels < 0) {
-		CGP_HIP(hipMemcpyAsync(pinned, mCounters.get(), sizeof(unsigned long long), hipMemcpyDeviceToHost, mStream));
+		CGP_HIP(hipMemcpyAsync(pinned, mCounters.get() + counter, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+		                       mStream));
 		CGP_HIP(hipStreamSynchronize(mStream));
 		before = pinned[0];
 	}
@@ -107,7 +110,8 @@ void GridSolver::launch(Kern kind, int64_t voxels, double bytes_per_voxel, F &&f
 	float ms = 0.0f;
 	CGP_HIP(hipEventElapsedTime(&ms, mEvStart, mEvStop));
 	if (voxels < 0) {
-		CGP_HIP(hipMemcpyAsync(pinned, mCounters.get(), sizeof(unsigned long long), hipMemcpyDeviceToHost, mStream));
+		CGP_HIP(hipMemcpyAsync(pinned, mCounters.get() + counter, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+		                       mStream));
 		CGP_HIP(hipStreamSynchronize(mStream));
 		voxels = (int64_t)(pinned[0] - before) * mShape.tx * mShape.ty * mShape.tz;
 	}
@@ -246,7 +250,7 @@ void GridSolver::allocate_grids(int conn, const std::vector<InstanceDesc> &inst,
 	mFlow.allocate(count);
 	mCounters.allocate(4);
 	mStatus.allocate(1);
-	mCount.allocate(1);
+	mCount.allocate(3);
 	mHostScratch.allocate(count > 8 ? count : 8);
 }
 
@@ -402,6 +406,27 @@ void GridSolver::configure_tiles(const Shape &shape)
 	all[mTiles] = mTiles;  // count slot for the all-tiles list
 	CGP_HIP(hipMemcpyAsync(mAllTiles.get(), all.data(), all.size() * sizeof(int32_t), hipMemcpyHostToDevice, mStream));
 	CGP_HIP(hipMemcpyAsync(mTileColour.get(), colour.data(), colour.size(), hipMemcpyHostToDevice, mStream));
+	// tiles grouped by colour (the first sweep of a relabel or closure visits them colour by colour)
+	mColours = ncolours;
+	mColourStart.assign(ncolours, 0);
+	mColourSize.assign(per_colour.begin(), per_colour.end());
+	for (int c = 1; c < ncolours; ++c) {
+		mColourStart[c] = mColourStart[c - 1] + mColourSize[c - 1];
+	}
+	{
+		std::vector<int32_t> grouped((size_t)mTiles + ncolours);
+		std::vector<int> fill(mColourStart);
+		for (int t = 0; t < mTiles; ++t) {
+			grouped[(size_t)fill[colour[(size_t)t]]++] = t;
+		}
+		for (int c = 0; c < ncolours; ++c) {
+			grouped[(size_t)mTiles + c] = mColourSize[c];
+		}
+		mColourTiles.allocate(grouped.size());
+		CGP_HIP(hipMemcpyAsync(mColourTiles.get(), grouped.data(), grouped.size() * sizeof(int32_t),
+		                       hipMemcpyHostToDevice, mStream));
+		CGP_HIP(hipStreamSynchronize(mStream));
+	}
 	if (mInst.size() > 1) {
 		mInstDev.allocate(mInst.size());
 		mTileInst.allocate(mTiles);
@@ -490,20 +515,77 @@ void GridSolver::launch_init()
 	});
 }
 
-void GridSolver::launch_compact(const uint8_t *flags, int colour)
+void GridSolver::launch_compact(uint8_t *flags, int colour, bool clear)
 {
 	launch(kFill, 0, 0.0, [&] {
 		hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, mStream, mCount.get(), (int64_t)1, 0);
 	});
 	launch(kCompact, 0, 0.0, [&] {
 		hipLaunchKernelGGL(k_compact_tiles, dim3(blocks_for(mTiles, 256, 1 << 20)), dim3(256), 0, mStream, flags,
-		                   mTiles, mTileColour.get(), colour, mList.get(), mCount.get());
+		                   mTiles, mTileColour.get(), colour, clear ? 1 : 0, mList.get(), mCount.get());
 	});
+}
+
+// Counts the set flags of two arrays with one host read.
+void GridSolver::count_two(uint8_t *a, uint8_t *b, int &na, int &nb)
+{
+	launch(kFill, 0, 0.0, [&] {
+		hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, mStream, mCount.get() + 1, (int64_t)2, 0);
+	});
+	launch(kCompact, 0, 0.0, [&] {
+		const int blocks = blocks_for(mTiles, 256, 1 << 20);
+		hipLaunchKernelGGL(k_compact_tiles, dim3(blocks), dim3(256), 0, mStream, a, mTiles, mTileColour.get(), -1, 0,
+		                   (int32_t *)nullptr, mCount.get() + 1);
+		if (b) {
+			hipLaunchKernelGGL(k_compact_tiles, dim3(blocks), dim3(256), 0, mStream, b, mTiles, mTileColour.get(), -1,
+			                   0, (int32_t *)nullptr, mCount.get() + 2);
+		}
+	});
+	int32_t *pinned = reinterpret_cast<int32_t *>(mHostScratch.get());
+	CGP_HIP(hipMemcpyAsync(pinned, mCount.get() + 1, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, mStream));
+	CGP_HIP(hipStreamSynchronize(mStream));
+	na = pinned[0];
+	nb = pinned[1];
+}
+
+// Visits every tile once, colour by colour, then keeps visiting the dirty tiles colour by colour
+// until a check finds none. Tiles of one colour never touch, so each pass reads the final state
+// of the neighbours it depends on; the passes between two checks are chained on the device (each
+// reads its tile count from device memory, blocks beyond it exit), so a relabel costs one or two
+// host reads instead of one per pass. pass(list, count, grid, first) launches one pass.
+// Returns the number of pass launches; `also` receives the flag count of also_count (if given)
+// at the final check.
+template <typename Pass>
+int GridSolver::colour_sweeps(uint8_t *dirty, uint8_t *also_count, int &also, Pass &&pass)
+{
+	constexpr int kCyclesPerCheck = 2;
+	int launches = 0;
+	for (int c = 0; c < mColours; ++c) {
+		if (mColourSize[c] == 0) {
+			continue;  // e.g. a grid one tile thick has no tile of the odd colours
+		}
+		pass(mColourTiles.get() + mColourStart[c], mColourTiles.get() + mTiles + c, mColourSize[c], true);
+		++launches;
+	}
+	for (;;) {
+		int pending = 0;
+		count_two(dirty, also_count, pending, also);
+		if (pending == 0) {
+			return launches;
+		}
+		for (int cycle = 0; cycle < kCyclesPerCheck; ++cycle) {
+			for (int c = 0; c < mColours; ++c) {
+				launch_compact(dirty, c, true);
+				pass(mList.get(), mCount.get(), mMaxColourTiles, false);
+				++launches;
+			}
+		}
+	}
 }
 
 int GridSolver::count_flags(const uint8_t *flags, int colour)
 {
-	launch_compact(flags, colour);
+	launch_compact(const_cast<uint8_t *>(flags), colour);
 	int32_t *pinned = reinterpret_cast<int32_t *>(mHostScratch.get());
 	CGP_HIP(hipMemcpyAsync(pinned, mCount.get(), sizeof(int32_t), hipMemcpyDeviceToHost, mStream));
 	CGP_HIP(hipStreamSynchronize(mStream));
@@ -513,44 +595,52 @@ int GridSolver::count_flags(const uint8_t *flags, int colour)
 int GridSolver::run_global_relabel(int32_t threshold, cgp_stats &stats)
 {
 	GridView g = CGP_VIEW();
-	if (mWave) {
-		const int64_t n = (int64_t)mTiles * wave::kFaces;
-		launch(kFill, mVox, 4.0 * n / mVox, [&] {
-			hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(n, 256, 8192)), dim3(256), 0, mStream, mFaces.get(), n,
-			                   kLabelInf);
-		});
-	} else {
+	const int64_t tile_voxels = (int64_t)mShape.tx * mShape.ty * mShape.tz;
+	if (!mWave) {
 		launch(kFill, mVox, 4.0, [&] {
 			hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(mVox, 256, 8192)), dim3(256), 0, mStream, mH.get(), mVox,
 			                   kLabelInf);
 		});
+		launch(kFill, 0, 0.0, [&] {
+			hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(mTiles, 256)), dim3(256), 0, mStream, mDirtyA.get(),
+			                   (int64_t)mTiles, (uint8_t)0);
+		});
+		int active = 0;
+		// exact passes read the state word and h and write h; threshold passes read r[K], rt, e, h
+		const double bytes = threshold == 0 ? 12.0 : 4.0 * (mConn + 4);
+		stats.bfs_passes += colour_sweeps(mDirtyA.get(), mActive.get(), active,
+		                                  [&](const int32_t *list, const int32_t *count, int grid, bool first) {
+			                                  launch(kBfs, first ? grid * tile_voxels : -1, bytes, [&] {
+				                                  dispatch_shape(mConn, mShape.tx, mShape.ty, mShape.tz, [&](auto tag) {
+					                                  using S = decltype(tag);
+					                                  hipLaunchKernelGGL((k_grid_bfs<S::conn, S::tx, S::ty, S::tz>),
+					                                                     dim3(grid), dim3(S::tx * S::ty), 0, mStream, g,
+					                                                     list, count, threshold, mDirtyA.get(),
+					                                                     mActive.get());
+				                                  });
+			                                  });
+		                                  });
+		return active;
 	}
+	const int64_t n = (int64_t)mTiles * wave::kFaces;
+	launch(kFill, mVox, 4.0 * n / mVox, [&] {
+		hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(n, 256, 8192)), dim3(256), 0, mStream, mFaces.get(), n,
+		                   kLabelInf);
+	});
 	const int32_t *list = mAllTiles.get();
 	const int32_t *count = mAllTiles.get() + mTiles;
 	int blocks = mTiles;
-	const int64_t tile_voxels = (int64_t)mShape.tx * mShape.ty * mShape.tz;
 	for (;;) {
 		launch(kFill, 0, 0.0, [&] {
 			hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(mTiles, 256)), dim3(256), 0, mStream, mDirtyA.get(),
 			                   (int64_t)mTiles, (uint8_t)0);
 		});
-		if (mWave) {
-			// reads the tile's bit planes (8 bits/voxel) and its own face labels (832 per 1024 voxels),
-			// writes face labels and the reach plane
-			launch(kBfs, (int64_t)blocks * tile_voxels, 1.0 + 2.0 * 4.0 * wave::kFaces / wave::NV + 0.125, [&] {
-				hipLaunchKernelGGL(wave::k_wave_bfs, dim3(blocks), dim3(64), 0, mStream, g, mBits.get(), mTiles, list,
-				                   count, threshold, mDirtyA.get(), mActive.get(), (int32_t *)nullptr);
-			});
-		} else {
-			// exact passes read the state word and h and write h; threshold passes read r[K], rt, e, h
-			launch(kBfs, (int64_t)blocks * tile_voxels, threshold == 0 ? 12.0 : 4.0 * (mConn + 4), [&] {
-				dispatch_shape(mConn, mShape.tx, mShape.ty, mShape.tz, [&](auto tag) {
-					using S = decltype(tag);
-					hipLaunchKernelGGL((k_grid_bfs<S::conn, S::tx, S::ty, S::tz>), dim3(blocks), dim3(S::tx * S::ty),
-					                   0, mStream, g, list, count, threshold, mDirtyA.get(), mActive.get());
-				});
-			});
-		}
+		// reads the tile's bit planes (8 bits/voxel) and its own face labels (832 per 1024 voxels),
+		// writes face labels and the reach plane
+		launch(kBfs, (int64_t)blocks * tile_voxels, 1.0 + 2.0 * 4.0 * wave::kFaces / wave::NV + 0.125, [&] {
+			hipLaunchKernelGGL(wave::k_wave_bfs, dim3(blocks), dim3(64), 0, mStream, g, mBits.get(), mTiles, list,
+			                   count, threshold, mDirtyA.get(), mActive.get(), (int32_t *)nullptr);
+		});
 		++stats.bfs_passes;
 		const int dirty = count_flags(mDirtyA.get(), -1);
 		if (dirty == 0) {
@@ -598,45 +688,51 @@ void GridSolver::run_discharge_phase(int colour, int max_iters)
 void GridSolver::run_source_closure(cgp_stats &stats)
 {
 	GridView g = CGP_VIEW();
-	if (mWave) {
-		launch(kFill, mVox, 0.125, [&] {
-			hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for((int64_t)mTiles * wave::R * 4, 256, 8192)), dim3(256), 0,
-			                   mStream, reinterpret_cast<uint8_t *>(mSBits.get()), (int64_t)mTiles * wave::R * 4,
-			                   (uint8_t)0);
-		});
-	} else {
+	const int64_t tile_voxels = (int64_t)mShape.tx * mShape.ty * mShape.tz;
+	if (!mWave) {
 		launch(kFill, mVox, 1.0, [&] {
 			hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(mVox, 256, 8192)), dim3(256), 0, mStream, mS.get(), mVox,
 			                   (uint8_t)0);
 		});
+		launch(kFill, 0, 0.0, [&] {
+			hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(mTiles, 256)), dim3(256), 0, mStream, mDirtyB.get(),
+			                   (int64_t)mTiles, (uint8_t)0);
+		});
+		int unused = 0;
+		// reads the state word and s, writes s
+		stats.cut_passes += colour_sweeps(mDirtyB.get(), nullptr, unused,
+		                                  [&](const int32_t *list, const int32_t *count, int grid, bool first) {
+			                                  launch(kClosure, first ? grid * tile_voxels : -1, 6.0, [&] {
+				                                  dispatch_shape(mConn, mShape.tx, mShape.ty, mShape.tz, [&](auto tag) {
+					                                  using S = decltype(tag);
+					                                  hipLaunchKernelGGL(
+					                                      (k_grid_source_closure<S::conn, S::tx, S::ty, S::tz>),
+					                                      dim3(grid), dim3(S::tx * S::ty), 0, mStream, g, list, count,
+					                                      first ? 1 : 0, mDirtyB.get());
+				                                  });
+			                                  });
+		                                  });
+		return;
 	}
+	launch(kFill, mVox, 0.125, [&] {
+		hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for((int64_t)mTiles * wave::R * 4, 256, 8192)), dim3(256), 0,
+		                   mStream, reinterpret_cast<uint8_t *>(mSBits.get()), (int64_t)mTiles * wave::R * 4,
+		                   (uint8_t)0);
+	});
 	const int32_t *list = mAllTiles.get();
 	const int32_t *count = mAllTiles.get() + mTiles;
 	int blocks = mTiles;
 	int first = 1;
-	const int64_t tile_voxels = (int64_t)mShape.tx * mShape.ty * mShape.tz;
 	for (;;) {
 		launch(kFill, 0, 0.0, [&] {
 			hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(mTiles, 256)), dim3(256), 0, mStream, mDirtyB.get(),
 			                   (int64_t)mTiles, (uint8_t)0);
 		});
-		if (mWave) {
-			// reads 7 bit planes and the S bits, writes the S bits
-			launch(kClosure, (int64_t)blocks * tile_voxels, 1.0, [&] {
-				hipLaunchKernelGGL(wave::k_wave_closure, dim3(blocks), dim3(64), 0, mStream, g, mBits.get(),
-				                   mSBits.get(), mTiles, list, count, first, mDirtyB.get());
-			});
-		} else {
-			// reads the state word and s, writes s
-			const double bytes = 4.0 + 1 + 1;
-			launch(kClosure, (int64_t)blocks * tile_voxels, bytes, [&] {
-				dispatch_shape(mConn, mShape.tx, mShape.ty, mShape.tz, [&](auto tag) {
-					using S = decltype(tag);
-					hipLaunchKernelGGL((k_grid_source_closure<S::conn, S::tx, S::ty, S::tz>), dim3(blocks),
-					                   dim3(S::tx * S::ty), 0, mStream, g, list, count, first, mDirtyB.get());
-				});
-			});
-		}
+		// reads 7 bit planes and the S bits, writes the S bits
+		launch(kClosure, (int64_t)blocks * tile_voxels, 1.0, [&] {
+			hipLaunchKernelGGL(wave::k_wave_closure, dim3(blocks), dim3(64), 0, mStream, g, mBits.get(), mSBits.get(),
+			                   mTiles, list, count, first, mDirtyB.get());
+		});
 		++stats.cut_passes;
 		first = 0;
 		const int dirty = count_flags(mDirtyB.get(), -1);

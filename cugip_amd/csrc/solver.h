@@ -81,7 +81,10 @@ private:
 	void launch_init();
 	int run_global_relabel(int32_t threshold, cgp_stats &stats);
 	int count_flags(const uint8_t *flags, int colour);
-	void launch_compact(const uint8_t *flags, int colour);
+	void launch_compact(uint8_t *flags, int colour, bool clear = false);
+	void count_two(uint8_t *a, uint8_t *b, int &na, int &nb);
+	template <typename Pass>
+	int colour_sweeps(uint8_t *dirty, uint8_t *also_count, int &also, Pass &&pass);
 	void run_discharge_phase(int colour, int max_iters);
 	void run_source_closure(cgp_stats &stats);
 	int64_t read_flow();
@@ -97,6 +100,9 @@ private:
 	DeviceBuffer<InstanceDesc> mInstDev;       // batches only
 	DeviceBuffer<int32_t> mTileInst;           // batches only: instance of every tile
 	DeviceBuffer<uint8_t> mTileColour;         // colour of every tile inside its instance
+	DeviceBuffer<int32_t> mColourTiles;        // tile ids grouped by colour, then one count per colour
+	std::vector<int> mColourStart, mColourSize;
+	int mColours = 0;
 	Shape mShape{0, 0, 0};
 	int mNtx = 0, mNty = 0, mNtz = 0, mTiles = 0, mMaxColourTiles = 0;
 	bool mSolved = false;
