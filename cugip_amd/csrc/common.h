@@ -141,6 +141,16 @@ __device__ inline long long wave_sum_i64(long long v)
 	return v;
 }
 
+__device__ inline int wave_max_i32(int v)
+{
+#pragma unroll
+	for (int off = 32; off > 0; off >>= 1) {
+		const int o = __shfl_xor(v, off, 64);
+		v = o > v ? o : v;
+	}
+	return v;
+}
+
 // Block-wide int64 sum, result valid in thread 0. `scratch` holds one slot per wave.
 template <int kThreads>
 __device__ inline long long block_sum_i64(long long v, long long *scratch)

@@ -310,6 +310,9 @@ struct DirtyBits {
 			atomicOr(&dm[i], 1u << d);
 		}
 	}
+	// voxel i pushed into t (its excess and residual sink link changed)
+	static constexpr int kSinkBit = kConn <= 8 ? 6 : 30;
+	__device__ static void touch(uint32_t *dm, int i) { set(dm, i, kSinkBit); }
 	__device__ static uint32_t get(const uint32_t *dm, int i)
 	{
 		return kConn <= 8 ? (dm[i >> 2] >> (8 * (i & 3))) & 0xffu : dm[i];
@@ -359,7 +362,10 @@ __global__ __launch_bounds__(kTX *kTY, 6) void k_grid_discharge(GridView gv, con
 
 	bool valid[T::TZ];
 	int32_t rtr[T::TZ];
-	int32_t e0[T::TZ], rt0[T::TZ], h0[T::TZ];
+	int32_t h0[T::TZ];
+	// 6-connected: flow pushed across the tile faces during the visit. The z-column of thread
+	// (lx, ly) touches at most one x face and one y face, and the two z faces at its ends.
+	int32_t out_x[T::TZ], out_y[T::TZ], out_zlo = 0, out_zhi = 0;
 #pragma unroll
 	for (int k = 0; k < T::TZ; ++k) {
 		valid[k] = gx < g.nx && gy < g.ny && oz + k < g.nz;
@@ -381,8 +387,8 @@ __global__ __launch_bounds__(kTX *kTY, 6) void k_grid_discharge(GridView gv, con
 		}
 		el[i] = ev;
 		rtr[k] = rv;
-		e0[k] = ev;
-		rt0[k] = rv;
+		out_x[k] = 0;
+		out_y[k] = 0;
 	}
 	__syncthreads();
 #pragma unroll
@@ -425,7 +431,10 @@ __global__ __launch_bounds__(kTX *kTY, 6) void k_grid_discharge(GridView gv, con
 			}
 		}
 		++step;
-		atomicMax(&key, ((unsigned long long)step << 32) | (unsigned)top);
+		top = wave_max_i32(top);
+		if ((tid & 63) == 0) {
+			atomicMax(&key, ((unsigned long long)step << 32) | (unsigned)top);
+		}
 		__syncthreads();
 		int level = (int)(key & 0xffffffffull);
 		if (level == 0) {
@@ -460,6 +469,7 @@ __global__ __launch_bounds__(kTX *kTY, 6) void k_grid_discharge(GridView gv, con
 					rtr[k] -= f;
 					ex -= f;
 					sink += f;
+					Dirty::touch(dm, i);
 				}
 #pragma unroll
 				for (int d = 0; d < K; ++d) {
@@ -477,6 +487,16 @@ __global__ __launch_bounds__(kTX *kTY, 6) void k_grid_discharge(GridView gv, con
 								Dirty::set(dm, j, opposite<kConn>(d));
 								atomicAdd(&el[j], f);
 								next = level - 1 > next ? level - 1 : next;
+							} else if (kConn == 6) {
+								if (d == 0 || d == 3) {
+									out_x[k] += f;
+								} else if (d == 1 || d == 4) {
+									out_y[k] += f;
+								} else if (d == 2) {
+									out_zlo += f;
+								} else {
+									out_zhi += f;
+								}
 							}
 						}
 					}
@@ -484,13 +504,16 @@ __global__ __launch_bounds__(kTX *kTY, 6) void k_grid_discharge(GridView gv, con
 				el[i] = ex;
 			}
 			++step;
-			atomicMax(&key, ((unsigned long long)step << 32) | (unsigned)next);
+			next = wave_max_i32(next);
+			if ((tid & 63) == 0) {
+				atomicMax(&key, ((unsigned long long)step << 32) | (unsigned)next);
+			}
 			__syncthreads();
 			level = (int)(key & 0xffffffffull);
 		}
 	}
 
-	// ---- write back (only what changed); pushes that left the tile go to the halo voxels
+	// ---- write back what changed; pushes that left the tile go to the neighbouring voxels
 	int active = 0;
 #pragma unroll
 	for (int k = 0; k < T::TZ; ++k) {
@@ -502,47 +525,64 @@ __global__ __launch_bounds__(kTX *kTY, 6) void k_grid_discharge(GridView gv, con
 		const int64_t v = voxel_index(g, gx, gy, oz + k);
 		const int32_t ev = el[i], lab = hl[p];
 		active |= ev > 0 && lab < kLabelInf;
-		if (ev != e0[k]) {
-			g.e[v] = ev;
-		}
-		if (rtr[k] != rt0[k]) {
-			g.rt[v] = rtr[k];
-		}
 		if (lab != h0[k]) {
 			g.h[v] = lab;
 		}
+		// every change of e, rt or r[d] set a bit of the voxel's dirty byte / word
 		const uint32_t dirty = Dirty::get(dm, i);
-		if (dirty || ev != e0[k] || rtr[k] != rt0[k]) {
-			uint32_t st = (ev > 0 ? kStateExcess : 0u) | (rtr[k] > 0 ? kStateSink : 0u);
-#pragma unroll
-			for (int d = 0; d < K; ++d) {
-				st |= rl[d][i] > 0 ? (1u << d) : 0u;
-			}
-			g.omask[v] = st;
+		if (!dirty) {
+			continue;
 		}
-		if (dirty) {
+		g.e[v] = ev;
+		g.rt[v] = rtr[k];
+		uint32_t st = (ev > 0 ? kStateExcess : 0u) | (rtr[k] > 0 ? kStateSink : 0u);
 #pragma unroll
-			for (int d = 0; d < K; ++d) {
-				if (!(dirty & (1u << d))) {
-					continue;
-				}
-				const int32_t nr = rl[d][i];
+		for (int d = 0; d < K; ++d) {
+			st |= rl[d][i] > 0 ? (1u << d) : 0u;
+		}
+		g.omask[v] = st;
+#pragma unroll
+		for (int d = 0; d < K; ++d) {
+			if (!(dirty & (1u << d))) {
+				continue;
+			}
+			const int32_t nr = rl[d][i];
+			if (kConn != 6) {
 				const int slot = leaving_slot<T>(lx, ly, k, d);
 				if (slot >= 0) {
 					// only this voxel pushes along this edge during the phase: the drop is the flow sent
-					const int32_t old = g.r[(int64_t)d * g.nvox + v];
-					const int32_t f = old - nr;
+					const int32_t f = g.r[(int64_t)d * g.nvox + v] - nr;
 					if (f > 0) {
 						const int64_t u = voxel_index(g, gx + D::dx(d), gy + D::dy(d), oz + k + D::dz(d));
 						atomicAdd(&g.e[u], f);
-						g.r[(int64_t)opposite<kConn>(d) * g.nvox + u] += f;
+						atomicAdd(&g.r[(int64_t)opposite<kConn>(d) * g.nvox + u], f);
 						atomicOr(&g.omask[u], kStateExcess | (1u << opposite<kConn>(d)));
 						marks[slot] = 1;
 					}
 				}
-				g.r[(int64_t)d * g.nvox + v] = nr;
 			}
+			g.r[(int64_t)d * g.nvox + v] = nr;
 		}
+	}
+	if (kConn == 6) {
+		// flow that crossed the faces goes to the neighbouring voxels (atomics without return)
+		auto send = [&](int k, int d, int32_t f) {
+			if (f <= 0) {
+				return;
+			}
+			const int64_t u = voxel_index(g, gx + D::dx(d), gy + D::dy(d), oz + k + D::dz(d));
+			atomicAdd(&g.e[u], f);
+			atomicAdd(&g.r[(int64_t)opposite<kConn>(d) * g.nvox + u], f);
+			atomicOr(&g.omask[u], kStateExcess | (1u << opposite<kConn>(d)));
+			marks[leaving_slot<T>(lx, ly, k, d)] = 1;
+		};
+#pragma unroll
+		for (int k = 0; k < T::TZ; ++k) {
+			send(k, lx == 0 ? 0 : 3, out_x[k]);
+			send(k, ly == 0 ? 1 : 4, out_y[k]);
+		}
+		send(0, 2, out_zlo);
+		send(T::TZ - 1, 5, out_zhi);
 	}
 	const long long total = block_sum_i64<T::NT>(sink, red);
 	active = __syncthreads_or(active);
