@@ -155,19 +155,19 @@ __device__ inline bool relax_voxel(int32_t *hl, int p, uint32_t open)
 
 // Fixpoint of relax_voxel over the tile (own column up then down per sweep).
 template <typename T>
-__device__ inline void relax_tile(int32_t *hl, const uint32_t (&open)[T::TZ], const bool (&valid)[T::TZ], int lx, int ly)
+__device__ inline void relax_tile(int32_t *hl, const uint32_t (&open)[T::TZ], uint32_t valid, int lx, int ly)
 {
 	for (;;) {
 		int changed = 0;
 #pragma unroll
 		for (int k = 0; k < T::TZ; ++k) {
-			if (valid[k]) {
+			if (valid & (1u << k)) {
 				changed |= relax_voxel<T>(hl, T::pidx(lx, ly, k), open[k]);
 			}
 		}
 #pragma unroll
 		for (int k = T::TZ - 2; k >= 0; --k) {
-			if (valid[k]) {
+			if (valid & (1u << k)) {
 				changed |= relax_voxel<T>(hl, T::pidx(lx, ly, k), open[k]);
 			}
 		}
@@ -226,18 +226,18 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_bfs(GridView gv, const int32_
 	load_label_tile<T>(g, hl, ox, oy, oz);
 	__syncthreads();
 
-	bool valid[T::TZ];
+	uint32_t valid = 0;
 	uint32_t open[T::TZ];
 	int32_t before[T::TZ];
 	bool has_excess[T::TZ];
 #pragma unroll
 	for (int k = 0; k < T::TZ; ++k) {
-		valid[k] = gx < g.nx && gy < g.ny && oz + k < g.nz;
+		valid |= (gx < g.nx && gy < g.ny && oz + k < g.nz) ? (1u << k) : 0u;
 		open[k] = 0;
 		has_excess[k] = false;
 		const int p = T::pidx(lx, ly, k);
 		before[k] = hl[p];
-		if (valid[k]) {
+		if (valid & (1u << k)) {
 			const int64_t v = voxel_index(g, gx, gy, oz + k);
 			bool sink_open;
 			if (threshold == 0) {
@@ -265,7 +265,7 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_bfs(GridView gv, const int32_
 	int active = 0;
 #pragma unroll
 	for (int k = 0; k < T::TZ; ++k) {
-		if (!valid[k]) {
+		if (!(valid & (1u << k))) {
 			continue;
 		}
 		const int p = T::pidx(lx, ly, k);
@@ -360,18 +360,17 @@ __global__ __launch_bounds__(kTX *kTY, 6) void k_grid_discharge(GridView gv, con
 	}
 	load_label_tile<T>(g, hl, ox, oy, oz);
 
-	bool valid[T::TZ];
+	uint32_t valid = 0;  // bit k: voxel k of the column lies inside the volume
 	int32_t rtr[T::TZ];
-	int32_t h0[T::TZ];
 	// 6-connected: flow pushed across the tile faces during the visit. The z-column of thread
 	// (lx, ly) touches at most one x face and one y face, and the two z faces at its ends.
 	int32_t out_x[T::TZ], out_y[T::TZ], out_zlo = 0, out_zhi = 0;
 #pragma unroll
 	for (int k = 0; k < T::TZ; ++k) {
-		valid[k] = gx < g.nx && gy < g.ny && oz + k < g.nz;
+		valid |= (gx < g.nx && gy < g.ny && oz + k < g.nz) ? (1u << k) : 0u;
 		const int i = T::vidx(lx, ly, k);
 		int32_t ev = 0, rv = 0;
-		if (valid[k]) {
+		if (valid & (1u << k)) {
 			const int64_t v = voxel_index(g, gx, gy, oz + k);
 			ev = g.e[v];
 			rv = g.rt[v];
@@ -391,10 +390,6 @@ __global__ __launch_bounds__(kTX *kTY, 6) void k_grid_discharge(GridView gv, con
 		out_y[k] = 0;
 	}
 	__syncthreads();
-#pragma unroll
-	for (int k = 0; k < T::TZ; ++k) {
-		h0[k] = hl[T::pidx(lx, ly, k)];
-	}
 
 	unsigned int step = 0;
 	long long sink = 0;
@@ -414,7 +409,7 @@ __global__ __launch_bounds__(kTX *kTY, 6) void k_grid_discharge(GridView gv, con
 		__syncthreads();  // all reads of the previous sweep's labels are done
 #pragma unroll
 		for (int k = 0; k < T::TZ; ++k) {
-			if (valid[k]) {
+			if (valid & (1u << k)) {
 				hl[T::pidx(lx, ly, k)] = rtr[k] > 0 ? 1 : kLabelInf;
 			}
 		}
@@ -426,7 +421,7 @@ __global__ __launch_bounds__(kTX *kTY, 6) void k_grid_discharge(GridView gv, con
 #pragma unroll
 		for (int k = 0; k < T::TZ; ++k) {
 			const int32_t lab = hl[T::pidx(lx, ly, k)];
-			if (valid[k] && el[T::vidx(lx, ly, k)] > 0 && lab < kLabelInf && lab > top) {
+			if ((valid & (1u << k)) && el[T::vidx(lx, ly, k)] > 0 && lab < kLabelInf && lab > top) {
 				top = lab;
 			}
 		}
@@ -448,7 +443,7 @@ __global__ __launch_bounds__(kTX *kTY, 6) void k_grid_discharge(GridView gv, con
 			int next = 0;
 #pragma unroll
 			for (int k = 0; k < T::TZ; ++k) {
-				if (!valid[k]) {
+				if (!(valid & (1u << k))) {
 					continue;
 				}
 				const int p = T::pidx(lx, ly, k);
@@ -517,7 +512,7 @@ __global__ __launch_bounds__(kTX *kTY, 6) void k_grid_discharge(GridView gv, con
 	int active = 0;
 #pragma unroll
 	for (int k = 0; k < T::TZ; ++k) {
-		if (!valid[k]) {
+		if (!(valid & (1u << k))) {
 			continue;
 		}
 		const int p = T::pidx(lx, ly, k);
@@ -525,9 +520,7 @@ __global__ __launch_bounds__(kTX *kTY, 6) void k_grid_discharge(GridView gv, con
 		const int64_t v = voxel_index(g, gx, gy, oz + k);
 		const int32_t ev = el[i], lab = hl[p];
 		active |= ev > 0 && lab < kLabelInf;
-		if (lab != h0[k]) {
-			g.h[v] = lab;
-		}
+		g.h[v] = lab;  // (a store is cheaper than keeping the loaded labels in registers)
 		// every change of e, rt or r[d] set a bit of the voxel's dirty byte / word
 		const uint32_t dirty = Dirty::get(dm, i);
 		if (!dirty) {
