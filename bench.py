@@ -8,7 +8,10 @@ Prints one JSON line with the throughput (Mvoxels/s), the roofline of the domina
 (HIP events on the solver's stream, algorithmic bytes from the solver's voxel-visit counters)
 and the CPU baseline (Boykov-Kolmogorov on one host core, maxflow() only).
 
-usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 256] [--seed 0] [--no-cpu]
+Also reported (config.batched_vs_plain): 8 grids of the same size solved as one batch against the
+plain solver run on each in turn, bit-exact per grid.
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 256] [--seed 0] [--no-cpu] [--batch 8]
 """
 import argparse
 import hashlib
@@ -59,6 +62,104 @@ def cpu_baseline(nx, ny, nz, conn, cs, ct, cn, runs):
                 seconds=round(med, 4), flow=flow, host_cpu=cpu_model(), host_nproc=os.cpu_count())
 
 
+def batched_vs_plain(cugip_amd, n, conn, seed, count, repeats=3):
+    """`count` grids of n^3 (instances 0..count-1 of `seed`): one batched solve (cgp_solve_batch) against
+    the plain solver run on each grid in turn; both from HBM-resident capacities, per-grid flows and
+    S_min compared bit for bit."""
+    import numpy as np
+    grids = []
+    for i in range(count):
+        _, cs, ct, cn = cugip_amd.make_instance(n, n, n, seed, i, conn)
+        grids.append((cs, ct, cn))
+    vox = count * n ** 3
+    with cugip_amd.GridBatch() as b:
+        b.set_batch(grids, conn)
+        b.solve()
+        ts = []
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            flows = b.solve()
+            ts.append(time.perf_counter() - t0)
+        batch_s = min(ts)
+        smin = [np.packbits(b.batch_source_set(i)) for i in range(count)]
+    plain_s = 0.0
+    same = True
+    for i, (cs, ct, cn) in enumerate(grids):
+        with cugip_amd.GridGraph() as g:
+            g.set_grid(cs, ct, cn, conn=conn)
+            g.max_flow()
+            ts = []
+            for _ in range(repeats):
+                t0 = time.perf_counter()
+                f = g.max_flow()
+                ts.append(time.perf_counter() - t0)
+            plain_s += min(ts)
+            same &= f == flows[i] and np.array_equal(np.packbits(g.source_set()), smin[i])
+    return dict(workload=f"{count} grids of {n}^3 {conn}-connected (seed {seed}, instances 0-{count - 1})",
+                batched_mvox_s=round(vox / batch_s / 1e6, 1), plain_mvox_s=round(vox / plain_s / 1e6, 1),
+                speedup=round(plain_s / batch_s, 3), batched_ms=round(batch_s * 1e3, 3),
+                plain_ms=round(plain_s * 1e3, 3), bit_exact=bool(same))
+
+
+def calibration_copies(cugip_amd, cs, ct, cn):
+    """Device-to-device copies of known size (64, 64 and 192 MiB for 256^3) through cgp_set_grid_device:
+    under rocprofv3 --pmc their FETCH_SIZE / WRITE_SIZE give the counters' scale."""
+    from tests.hip_rt import DeviceArray
+    dev = [DeviceArray(a) for a in (cs, ct, cn)]
+    with cugip_amd.GridGraph() as g:
+        nz, ny, nx = cs.shape
+        g.set_grid_device((nz, ny, nx), 6, dev[0].address, dev[1].address, dev[2].address)
+    for d in dev:
+        d.free()
+    return [a.nbytes for a in (cs, ct, cn)]
+
+
+def traffic_from_counters(fetch_csv, write_csv, out_path, copy_bytes):
+    """HBM bytes per launch of each grid kernel of the last solve, from two rocprofv3 --pmc runs
+    (FETCH_SIZE, WRITE_SIZE, in KiB per dispatch) of `bench.py --calibration-copies`, scaled by what the
+    counters read for the calibration copies (the first three dispatches)."""
+    import csv
+    import collections
+
+    def per_dispatch(path):
+        d = collections.OrderedDict()
+        for r in csv.DictReader(open(path)):
+            k = int(r["Dispatch_Id"])
+            d.setdefault(k, [r["Kernel_Name"], 0.0])
+            d[k][1] += float(r["Counter_Value"])
+        return d
+
+    fa, wa = per_dispatch(fetch_csv), per_dispatch(write_csv)
+    ids = list(fa)
+    known_kib = sum(copy_bytes) / 1024.0
+    fcal = known_kib / sum(fa[i][1] for i in ids[:3])
+    wcal = known_kib / sum(wa[i][1] for i in ids[:3])
+    last_init = [i for i in ids if "k_grid_init" in fa[i][0]][-1]
+    names = {"k_grid_discharge": "grid_discharge", "k_grid_bfs": "grid_bfs", "k_grid_init": "grid_init",
+             "k_grid_source_closure": "grid_source_closure"}
+    out = dict(workload="last solve of `bench.py --calibration-copies --steps 1 --warmup 1 --no-cpu --batch 0`",
+               calibration=dict(fetch_factor=round(fcal, 4), write_factor=round(wcal, 4),
+                                how="bytes of the three device-to-device copies / their FETCH_SIZE and WRITE_SIZE"))
+    for i in ids:
+        if i < last_init:
+            continue
+        base = fa[i][0].split("(")[0].replace("void ", "").replace("cgp::", "").split("<")[0]
+        if base not in names:
+            continue
+        rec = out.setdefault(names[base], dict(launches=0, fetch_bytes=0.0, write_bytes=0.0))
+        rec["launches"] += 1
+        rec["fetch_bytes"] += fa[i][1] * fcal * 1024
+        rec["write_bytes"] += wa.get(i, ["", 0.0])[1] * wcal * 1024
+    for k in names.values():
+        if k in out:
+            r = out[k]
+            r["bytes_per_launch"] = round((r["fetch_bytes"] + r["write_bytes"]) / r["launches"])
+            r["fetch_bytes"], r["write_bytes"] = round(r["fetch_bytes"]), round(r["write_bytes"])
+    with open(out_path, "w") as f:
+        json.dump(out, f, indent=1)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,7 +170,20 @@ def main():
     ap.add_argument("--conn", type=int, default=6)
     ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch", type=int, default=8, help="grids of the batched-vs-plain comparison (0: skip)")
+    ap.add_argument("--calibration-copies", action="store_true",
+                    help="start with device-to-device copies of known size (for rocprofv3 --pmc)")
+    ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="only turn two --pmc counter_collection.csv files into profiles/hbm_traffic.json")
     args = ap.parse_args()
+
+    if args.traffic_from:
+        n = args.size
+        copy = [4 * n ** 3, 4 * n ** 3, 12 * n ** 3]
+        res = traffic_from_counters(args.traffic_from[0], args.traffic_from[1],
+                                    os.path.join(ROOT, "profiles", "hbm_traffic.json"), copy)
+        print(json.dumps(res))
+        return
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -90,6 +204,8 @@ def main():
     vox = nx * ny * nz
     log(f"generating {nx}x{ny}x{nz} {args.conn}-connected instance seed {args.seed}")
     vol, cs, ct, cn = cugip_amd.make_instance(nx, ny, nz, seed=args.seed, conn=args.conn)
+    if args.calibration_copies:
+        log(f"calibration copies: {calibration_copies(cugip_amd, cs, ct, cn)} bytes")
 
     g = cugip_amd.GridGraph()
     t0 = time.perf_counter()
@@ -149,6 +265,11 @@ def main():
                     alg_bytes_per_launch=round(dom["algorithmic_bytes"] / max(1, dom["launches"])),
                     bytes_per_voxel_visit=dom["bytes_per_voxel"])
 
+    batch = None
+    if args.batch > 0:
+        batch = batched_vs_plain(cugip_amd, n, args.conn, args.seed, args.batch)
+        log(f"batched vs plain: {batch}")
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(nx, ny, nz, args.conn, cs, ct, cn, args.cpu_runs)
@@ -182,6 +303,7 @@ def main():
             "flow": flow,
             "verified_bit_exact": verified,
             "h2d_setup_ms": round(h2d_ms, 2),
+            "batched_vs_plain": batch,
             "learned_batched_speedup": None,
         },
         "roofline": roofline,
