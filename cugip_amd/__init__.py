@@ -148,6 +148,7 @@ _lib.cgp_set_grid_volume.argtypes = [_vp] + [ctypes.c_int32] * 4 + [_vp, _vp, ct
 _lib.cgp_grid_capacities.argtypes = [_vp, ctypes.c_int32, _i32p, _i32p, _i32p, _i32p]
 _lib.cgp_grid_state.argtypes = [_vp, ctypes.c_int32, _i32p, _i32p, ctypes.POINTER(ctypes.c_uint32)]
 _lib.cgp_save_graphml.argtypes = [_vp, ctypes.c_char_p]
+_lib.cgp_cut_capacity.argtypes = [_vp, ctypes.c_int32, _u8p, ctypes.POINTER(ctypes.c_int64)]
 _lib.cgp_set_grid_batch.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(GridInstance),
                                     ctypes.c_int32]
 _lib.cgp_batch_size.argtypes = [_vp, _i32p]
@@ -285,6 +286,16 @@ class _Handle:
         out = np.empty(self._n, np.int32)
         self._check(_lib.cgp_labels(self._h, _p32(out)))
         return out
+
+    def cut_capacity(self, source_side, index=0):
+        """Exact capacity of the cut with the given source-side mask (computed on the device)."""
+        side = np.ascontiguousarray(source_side, dtype=np.uint8).reshape(-1)
+        want = int(np.prod(self.shapes[index])) if hasattr(self, "shapes") else self._n
+        if side.size != want:
+            raise CgpError(CGP_ERR_INVALID, "mask has %d entries, the grid %d voxels" % (side.size, want))
+        cap = ctypes.c_int64(0)
+        self._check(_lib.cgp_cut_capacity(self._h, index, side.ctypes.data_as(_u8p), ctypes.byref(cap)))
+        return int(cap.value)
 
     def save_graphml(self, path):
         """GraphML dump of the graph (Graph::save_to_graphml)."""

@@ -331,6 +331,16 @@ int cgp_grid_state(cgp_graph g, int32_t index, int32_t *excess, int32_t *labels,
 	return guarded(g, [&] { grid_of(g).state(index, excess, labels, saturated); });
 }
 
+int cgp_cut_capacity(cgp_graph g, int32_t index, const uint8_t *source_side, int64_t *capacity)
+{
+	return guarded(g, [&] {
+		if (!source_side || !capacity) {
+			cgp::fail(CGP_ERR_INVALID, "null argument");
+		}
+		*capacity = grid_of(g).cut_capacity(index, source_side);
+	});
+}
+
 int cgp_save_graphml(cgp_graph g, const char *path)
 {
 	return guarded(g, [&] {

@@ -825,6 +825,46 @@ __global__ __launch_bounds__(256) void k_saturated_bits(int nx, int ny, int nz, 
 	}
 }
 
+// Capacity of a cut given by its source-side mask, over one grid (int64 block sums, one atomic per
+// block).
+template <int kConn>
+__global__ __launch_bounds__(256) void k_cut_capacity(int nx, int ny, int nz, const uint8_t *__restrict__ side,
+                                                      const int32_t *__restrict__ cs, const int32_t *__restrict__ ct,
+                                                      const int32_t *__restrict__ cn,
+                                                      const int32_t *__restrict__ cnb,
+                                                      unsigned long long *__restrict__ out)
+{
+	using D = Dirs<kConn>;
+	__shared__ long long red[4];
+	const int64_t n = (int64_t)nx * ny * nz;
+	const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+	long long sum = 0;
+	for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+		const int x = (int)(v % nx);
+		const int y = (int)((v / nx) % ny);
+		const int z = (int)(v / ((int64_t)nx * ny));
+		const bool in = side[v] != 0;
+		sum += in ? ct[v] : cs[v];
+#pragma unroll
+		for (int j = 0; j < D::H; ++j) {
+			const int ux = x + D::dx(j), uy = y + D::dy(j), uz = z + D::dz(j);
+			if (ux < 0 || ux >= nx || uy < 0 || uy >= ny || uz < 0 || uz >= nz) {
+				continue;
+			}
+			const bool uin = side[((int64_t)uz * ny + uy) * nx + ux] != 0;
+			if (in && !uin) {
+				sum += cn[(int64_t)j * n + v];            // v -> v + o_j leaves the source side
+			} else if (uin && !in) {
+				sum += (cnb ? cnb : cn)[(int64_t)j * n + v];  // v + o_j -> v
+			}
+		}
+	}
+	const long long total = block_sum_i64<256>(sum, red);
+	if (threadIdx.x == 0 && total) {
+		atomicAdd(out, (unsigned long long)total);
+	}
+}
+
 __global__ __launch_bounds__(256) void k_fill_u8(uint8_t *__restrict__ p, int64_t n, uint8_t value)
 {
 	const int64_t stride = (int64_t)gridDim.x * blockDim.x;

@@ -331,6 +331,35 @@ void GridSolver::set_grid_volume(int nx, int ny, int nz, int conn, const uint8_t
 	CGP_HIP(hipStreamSynchronize(mStream));
 }
 
+int64_t GridSolver::cut_capacity(int index, const uint8_t *host_side)
+{
+	if (mInst.empty()) {
+		fail(CGP_ERR_STATE, "no grid set");
+	}
+	const InstanceDesc &d = instance(index);
+	const int half = mConn / 2;
+	DeviceBuffer<uint8_t> side(d.nvox);
+	DeviceBuffer<unsigned long long> sum(1);
+	CGP_HIP(hipMemcpyAsync(side.get(), host_side, d.nvox, hipMemcpyHostToDevice, mStream));
+	hipLaunchKernelGGL(k_fill_u8, dim3(1), dim3(64), 0, mStream, reinterpret_cast<uint8_t *>(sum.get()), (int64_t)8,
+	                   (uint8_t)0);
+	const int32_t *cn = mCn.get() + d.vbase * half;
+	const int32_t *cnb = mHasBack ? mCnb.get() + d.vbase * half : nullptr;
+	const int blocks = blocks_for(d.nvox, 256, 8192);
+	if (mConn == 6) {
+		hipLaunchKernelGGL(k_cut_capacity<6>, dim3(blocks), dim3(256), 0, mStream, d.nx, d.ny, d.nz, side.get(),
+		                   mCs.get() + d.vbase, mCt.get() + d.vbase, cn, cnb, sum.get());
+	} else {
+		hipLaunchKernelGGL(k_cut_capacity<26>, dim3(blocks), dim3(256), 0, mStream, d.nx, d.ny, d.nz, side.get(),
+		                   mCs.get() + d.vbase, mCt.get() + d.vbase, cn, cnb, sum.get());
+	}
+	CGP_LAUNCH_CHECK();
+	unsigned long long host = 0;
+	CGP_HIP(hipMemcpyAsync(&host, sum.get(), sizeof(host), hipMemcpyDeviceToHost, mStream));
+	CGP_HIP(hipStreamSynchronize(mStream));
+	return (int64_t)host;
+}
+
 void GridSolver::capacities(int index, int32_t *cs, int32_t *ct, int32_t *cn, int32_t *cnb)
 {
 	if (mInst.empty()) {

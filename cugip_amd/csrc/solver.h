@@ -44,6 +44,7 @@ public:
 	                     const cgp_volume_weights &weights, bool device_pointers);
 	void capacities(int index, int32_t *cs, int32_t *ct, int32_t *cn, int32_t *cnb);
 	void state(int index, int32_t *excess, int32_t *labels, uint32_t *saturated);
+	int64_t cut_capacity(int index, const uint8_t *host_side);
 	int conn() const { return mConn; }
 	const InstanceDesc &grid(int index) const { return instance(index); }
 	// Returns the total flow of all instances; per-instance values in flows().

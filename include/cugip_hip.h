@@ -178,6 +178,11 @@ int cgp_labels(cgp_graph graph, int32_t *out);
    neighbour in residual direction d has capacity > 0 and residual 0; d < conn/2 are the offsets of
    cap_n, d >= conn/2 their mirrors). Any output may be NULL. */
 int cgp_grid_state(cgp_graph graph, int32_t index, int32_t *excess, int32_t *labels, uint32_t *saturated);
+/* Capacity of the cut whose source side is {s} + {v : source_side[v] != 0} of grid `index`
+   (host buffer of its voxel count): the sum of cs over voxels outside, ct over voxels inside,
+   and the capacities of the n-links from inside to outside. Exact int64. A cut whose capacity
+   equals the max flow is a minimum cut. */
+int cgp_cut_capacity(cgp_graph graph, int32_t index, const uint8_t *source_side, int64_t *capacity);
 /* Writes the graph (generic or single grid) as GraphML: nodes "source", "sink", "n<i>"; one
    directed edge per positive capacity (both directions of every n-link, source -> n<i>,
    n<i> -> sink), capacity in the "capacity" key. */

@@ -378,3 +378,47 @@ def test_grid_state_and_graphml(tmp_path):
     assert len(root.findall(".//g:node", ns)) == nx * ny * nz + 2
     ref = oracle.dinic(nx * ny * nz, np.array(ea), np.array(eb), np.array(cap), np.zeros(len(cap)), src, snk)[0]
     assert ref == flow
+
+
+def test_cut_capacity_entry():
+    """cgp_cut_capacity against the numpy cut capacity on random masks (asymmetric capacities,
+    both connectivities), and on S_min / V minus T_min, whose capacity is the flow."""
+    say("grid cut capacity entry")
+    rng = np.random.default_rng(77)
+    for conn in (6, 26):
+        shape = (13, 17, 19)
+        half = conn // 2
+        cs = rng.integers(0, 300, shape).astype(np.int32)
+        ct = rng.integers(0, 300, shape).astype(np.int32)
+        cn = rng.integers(0, 90, (half,) + shape).astype(np.int32)
+        cnb = rng.integers(0, 90, (half,) + shape).astype(np.int32)
+        with cugip_amd.GridGraph() as g:
+            g.set_grid(cs, ct, cn, cnb, conn=conn)
+            for _ in range(3):
+                mask = (rng.random(shape) < 0.5).astype(np.uint8)
+                assert g.cut_capacity(mask) == grid_cut_capacity(mask, cs, ct, cn, cnb, conn)
+            flow = g.max_flow()
+            assert g.cut_capacity(g.source_set()) == flow
+            assert g.cut_capacity(1 - g.fill_segments(1, 0)) == flow
+
+
+def test_full_size_26c_512x512x384():
+    """The 26-connected 512x512x384 workload (about 100 M voxels, capacities computed on the device
+    from the volume): the S_min and the V minus T_min cuts both have the flow's capacity (so both are
+    minimum cuts and the flow is maximum), they are nested, and a second solve gives the same bits."""
+    import time
+    nx, ny, nz = 512, 512, 384
+    say("grid 26C 512x512x384")
+    vol = cugip_amd.generate_volume(nx, ny, nz, seed=0)
+    with cugip_amd.GridGraph() as g:
+        g.set_grid_volume(vol, conn=26)
+        t0 = time.perf_counter()
+        flow = g.max_flow()
+        say(f"  solve {1e3 * (time.perf_counter() - t0):.1f} ms, flow {flow}, stats {g.stats.as_dict()}")
+        smin = g.source_set()
+        tmin = g.fill_segments(1, 0)
+        assert g.cut_capacity(smin) == flow
+        assert g.cut_capacity(1 - tmin) == flow
+        assert not (smin.astype(bool) & tmin.astype(bool)).any()
+        assert g.max_flow() == flow
+        assert digest(g.source_set()) == digest(smin)
