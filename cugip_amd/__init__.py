@@ -149,6 +149,18 @@ _lib.cgp_grid_capacities.argtypes = [_vp, ctypes.c_int32, _i32p, _i32p, _i32p, _
 _lib.cgp_grid_state.argtypes = [_vp, ctypes.c_int32, _i32p, _i32p, ctypes.POINTER(ctypes.c_uint32)]
 _lib.cgp_save_graphml.argtypes = [_vp, ctypes.c_char_p]
 _lib.cgp_cut_capacity.argtypes = [_vp, ctypes.c_int32, _u8p, ctypes.POINTER(ctypes.c_int64)]
+_lib.cgp_cnn_create.argtypes = [ctypes.POINTER(_vp)]
+_lib.cgp_cnn_destroy.argtypes = [_vp]
+_lib.cgp_cnn_last_error.argtypes = [_vp]
+_lib.cgp_cnn_last_error.restype = ctypes.c_char_p
+_fp = ctypes.POINTER(ctypes.c_float)
+_lib.cgp_cnn_set_params.argtypes = [_vp, _fp, ctypes.c_int32]
+_lib.cgp_cnn_get_params.argtypes = [_vp, _fp, ctypes.c_int32]
+_lib.cgp_cnn_predict.argtypes = [_vp, _vp, _u8p, _u8p, _fp, ctypes.POINTER(ctypes.c_int64)]
+_lib.cgp_cnn_gradients.argtypes = [_vp, _vp, _u8p, ctypes.POINTER(ctypes.c_double), _fp]
+_lib.cgp_cnn_train_step.argtypes = [_vp, _vp, _u8p, ctypes.c_float, ctypes.POINTER(ctypes.c_double)]
+_lib.cgp_max_flow_certified.argtypes = [_vp, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(Stats),
+                                        ctypes.POINTER(ctypes.c_int32)]
 _lib.cgp_set_grid_batch.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(GridInstance),
                                     ctypes.c_int32]
 _lib.cgp_batch_size.argtypes = [_vp, _i32p]
@@ -296,6 +308,17 @@ class _Handle:
         cap = ctypes.c_int64(0)
         self._check(_lib.cgp_cut_capacity(self._h, index, side.ctypes.data_as(_u8p), ctypes.byref(cap)))
         return int(cap.value)
+
+    def max_flow_certified(self, certificate):
+        """Solve that stops once the flow into t reaches `certificate` (a cut capacity); returns
+        (flow, stopped_early)."""
+        flow = ctypes.c_int64(0)
+        early = ctypes.c_int32(0)
+        st = Stats()
+        self._check(_lib.cgp_max_flow_certified(self._h, int(certificate), ctypes.byref(flow), ctypes.byref(st),
+                                                ctypes.byref(early)))
+        self.stats = st
+        return int(flow.value), bool(early.value)
 
     def save_graphml(self, path):
         """GraphML dump of the graph (Graph::save_to_graphml)."""
@@ -524,6 +547,96 @@ def write_mfi(path, cap_source, cap_sink, cap_n, cap_n_back=None, conn=6, maxflo
         raise CgpError(CGP_ERR_INVALID, "capacity shapes do not match the volume")
     _global_check(_lib.cgp_write_mfi(os.fsencode(path), nx, ny, nz, conn, _p32(cs), _p32(ct), _p32(cn),
                                      _p32(cb) if cb is not None else None, int(maxflow), metadata.encode()))
+
+
+CNN_PARAMS = 22545
+
+
+class Cnn:
+    """Learned cut proposal (cgp_cnn_*): a 3D CNN run on the matrix cores. Parameter layout: see
+    cugip_hip.h. Prediction, gradients and training act on grid 0 of a GridGraph plus its volume."""
+
+    def __init__(self):
+        h = _vp()
+        rc = _lib.cgp_cnn_create(ctypes.byref(h))
+        if rc != CGP_OK:
+            raise CgpError(rc, (_lib.cgp_last_error(None) or b"").decode())
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.cgp_cnn_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc):
+        if rc != CGP_OK:
+            raise CgpError(rc, _lib.cgp_cnn_last_error(self._h).decode())
+
+    @staticmethod
+    def init_params(seed=0):
+        """He-initialised parameters (numpy float32), the padding input channel zero."""
+        rng = np.random.default_rng(seed)
+        p = np.zeros(CNN_PARAMS, np.float32)
+        w1 = rng.normal(0, np.sqrt(2.0 / (27 * 3)), (27, 4, 16)).astype(np.float32)
+        w1[:, 3, :] = 0
+        p[:27 * 4 * 16] = w1.reshape(-1)
+        off = 27 * 4 * 16 + 16
+        for _ in range(3):
+            p[off:off + 27 * 256] = rng.normal(0, np.sqrt(2.0 / (27 * 16)), 27 * 256)
+            off += 27 * 256 + 16
+        p[off:off + 16] = rng.normal(0, np.sqrt(1.0 / 16), 16)
+        return p
+
+    def set_params(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        self._check(_lib.cgp_cnn_set_params(self._h, p.ctypes.data_as(_fp), p.size))
+
+    def get_params(self):
+        p = np.empty(CNN_PARAMS, np.float32)
+        self._check(_lib.cgp_cnn_get_params(self._h, p.ctypes.data_as(_fp), p.size))
+        return p
+
+    @staticmethod
+    def _volume(graph, volume):
+        vol = np.ascontiguousarray(volume, dtype=np.uint8).reshape(-1)
+        if vol.size != graph._n:
+            raise CgpError(CGP_ERR_INVALID, "volume does not match the graph's grid")
+        return vol
+
+    def predict(self, graph, volume):
+        """(side 0/1 bytes, probabilities, certificate = capacity of the predicted cut)."""
+        vol = self._volume(graph, volume)
+        side = np.empty(vol.size, np.uint8)
+        prob = np.empty(vol.size, np.float32)
+        cert = ctypes.c_int64(0)
+        self._check(_lib.cgp_cnn_predict(self._h, graph._h, vol.ctypes.data_as(_u8p), side.ctypes.data_as(_u8p),
+                                         prob.ctypes.data_as(_fp), ctypes.byref(cert)))
+        return side, prob, int(cert.value)
+
+    def gradients(self, graph, volume):
+        """(loss, gradient) against the graph's S_min (the graph must be solved)."""
+        vol = self._volume(graph, volume)
+        grad = np.empty(CNN_PARAMS, np.float32)
+        loss = ctypes.c_double(0)
+        self._check(_lib.cgp_cnn_gradients(self._h, graph._h, vol.ctypes.data_as(_u8p), ctypes.byref(loss),
+                                           grad.ctypes.data_as(_fp)))
+        return float(loss.value), grad
+
+    def train_step(self, graph, volume, learning_rate=1e-3):
+        vol = self._volume(graph, volume)
+        loss = ctypes.c_double(0)
+        self._check(_lib.cgp_cnn_train_step(self._h, graph._h, vol.ctypes.data_as(_u8p), float(learning_rate),
+                                            ctypes.byref(loss)))
+        return float(loss.value)
 
 
 def generate_volume(nx, ny, nz, seed, instance=0):

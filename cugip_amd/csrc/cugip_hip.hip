@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "common.h"
+#include "cnn.h"
 #include "grid_dirs.h"
 #include "mfi.h"
 #include "solver.h"
@@ -23,6 +24,13 @@ struct cgp_graph_s {
 	cgp::TraceHook trace;
 	std::unique_ptr<cgp::CsrSolver> csr;
 	std::unique_ptr<cgp::GridSolver> grid;
+};
+
+struct cgp_cnn_s {
+	std::string last_error;
+	hipStream_t stream = nullptr;
+	int device = 0;
+	std::unique_ptr<cgp::CnnModel> model;
 };
 
 namespace {
@@ -96,6 +104,37 @@ cgp::GridSolver &single_grid(cgp_graph g)
 		                               " grids: use the cgp_batch_* accessors");
 	}
 	return grid;
+}
+
+template <typename Fn>
+int cnn_guarded(cgp_cnn c, Fn &&fn)
+{
+	if (!c) {
+		return CGP_ERR_INVALID;
+	}
+	try {
+		CGP_HIP(hipSetDevice(c->device));
+		fn();
+		c->last_error.clear();
+		return CGP_OK;
+	} catch (const cgp::Error &e) {
+		c->last_error = e.what();
+		return e.code;
+	} catch (const std::bad_alloc &) {
+		c->last_error = "host allocation failed";
+		return CGP_ERR_NOMEM;
+	} catch (const std::exception &e) {
+		c->last_error = e.what();
+		return CGP_ERR_DEVICE;
+	}
+}
+
+cgp::GridSolver &cnn_grid(cgp_graph g)
+{
+	if (!g) {
+		cgp::fail(CGP_ERR_INVALID, "null graph");
+	}
+	return single_grid(g);
 }
 
 cgp::CsrSolver &generic(cgp_graph g)
@@ -414,6 +453,123 @@ int cgp_save_graphml(cgp_graph g, const char *path)
 		const bool ok = std::fprintf(f, "</graph>\n</graphml>\n") > 0;
 		if (std::fclose(f) != 0 || !ok) {
 			cgp::fail(CGP_ERR_INVALID, std::string(path) + ": write failed");
+		}
+	});
+}
+
+int cgp_cnn_create(cgp_cnn *out)
+{
+	if (!out) {
+		return CGP_ERR_INVALID;
+	}
+	*out = nullptr;
+	return guarded_global([&] {
+		std::unique_ptr<cgp_cnn_s> c(new cgp_cnn_s());
+		int count = 0;
+		CGP_HIP(hipGetDeviceCount(&count));
+		if (count < 1) {
+			cgp::fail(CGP_ERR_DEVICE, "no HIP device");
+		}
+		CGP_HIP(hipGetDevice(&c->device));
+		CGP_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+		c->model.reset(new cgp::CnnModel(c->stream));
+		*out = c.release();
+	});
+}
+
+int cgp_cnn_destroy(cgp_cnn c)
+{
+	if (!c) {
+		return CGP_OK;
+	}
+	(void)hipSetDevice(c->device);
+	c->model.reset();
+	if (c->stream) {
+		(void)hipStreamDestroy(c->stream);
+	}
+	delete c;
+	return CGP_OK;
+}
+
+const char *cgp_cnn_last_error(cgp_cnn c)
+{
+	return c ? c->last_error.c_str() : "null handle";
+}
+
+int cgp_cnn_set_params(cgp_cnn c, const float *params, int32_t count)
+{
+	return cnn_guarded(c, [&] {
+		if (!params || count != cgp::kCnnParams) {
+			cgp::fail(CGP_ERR_INVALID, "expected " + std::to_string(cgp::kCnnParams) + " parameters");
+		}
+		c->model->set_params(params);
+	});
+}
+
+int cgp_cnn_get_params(cgp_cnn c, float *params, int32_t count)
+{
+	return cnn_guarded(c, [&] {
+		if (!params || count != cgp::kCnnParams) {
+			cgp::fail(CGP_ERR_INVALID, "expected " + std::to_string(cgp::kCnnParams) + " parameters");
+		}
+		c->model->get_params(params);
+	});
+}
+
+int cgp_cnn_predict(cgp_cnn c, cgp_graph g, const uint8_t *volume, uint8_t *side, float *prob, int64_t *certificate)
+{
+	return cnn_guarded(c, [&] {
+		cgp::GridSolver &grid = cnn_grid(g);
+		c->model->forward(grid, volume, false, false);
+		c->model->prediction_to_host(side, prob);
+		if (certificate) {
+			*certificate = grid.cut_capacity_device(0, c->model->predicted_side());
+		}
+	});
+}
+
+int cgp_cnn_gradients(cgp_cnn c, cgp_graph g, const uint8_t *volume, double *loss, float *grad)
+{
+	return cnn_guarded(c, [&] {
+		const double l = c->model->gradients(cnn_grid(g), volume, false, grad);
+		if (loss) {
+			*loss = l;
+		}
+	});
+}
+
+int cgp_cnn_train_step(cgp_cnn c, cgp_graph g, const uint8_t *volume, float learning_rate, double *loss)
+{
+	return cnn_guarded(c, [&] {
+		const double l = c->model->train_step(cnn_grid(g), volume, false, learning_rate);
+		if (loss) {
+			*loss = l;
+		}
+	});
+}
+
+int cgp_max_flow_certified(cgp_graph g, int64_t certificate, int64_t *flow, cgp_stats *stats, int32_t *stopped_early)
+{
+	return guarded(g, [&] {
+		if (certificate <= 0) {
+			cgp::fail(CGP_ERR_INVALID, "certificate must be positive");
+		}
+		cgp::GridSolver &grid = single_grid(g);
+		grid.set_profiling(g->profiling);
+		grid.set_certificate(certificate);
+		int64_t f = 0;
+		try {
+			f = grid.solve(g->policy, stats, g->trace);
+		} catch (...) {
+			grid.set_certificate(0);
+			throw;
+		}
+		grid.set_certificate(0);
+		if (flow) {
+			*flow = f;
+		}
+		if (stopped_early) {
+			*stopped_early = grid.stopped_early() ? 1 : 0;
 		}
 	});
 }

@@ -337,20 +337,27 @@ int64_t GridSolver::cut_capacity(int index, const uint8_t *host_side)
 		fail(CGP_ERR_STATE, "no grid set");
 	}
 	const InstanceDesc &d = instance(index);
-	const int half = mConn / 2;
 	DeviceBuffer<uint8_t> side(d.nvox);
-	DeviceBuffer<unsigned long long> sum(1);
 	CGP_HIP(hipMemcpyAsync(side.get(), host_side, d.nvox, hipMemcpyHostToDevice, mStream));
+	return cut_capacity_device(index, side.get());
+}
+
+int64_t GridSolver::cut_capacity_device(int index, const uint8_t *device_side)
+{
+	const InstanceDesc &d = instance(index);
+	const int half = mConn / 2;
+	const uint8_t *side = device_side;
+	DeviceBuffer<unsigned long long> sum(1);
 	hipLaunchKernelGGL(k_fill_u8, dim3(1), dim3(64), 0, mStream, reinterpret_cast<uint8_t *>(sum.get()), (int64_t)8,
 	                   (uint8_t)0);
 	const int32_t *cn = mCn.get() + d.vbase * half;
 	const int32_t *cnb = mHasBack ? mCnb.get() + d.vbase * half : nullptr;
 	const int blocks = blocks_for(d.nvox, 256, 8192);
 	if (mConn == 6) {
-		hipLaunchKernelGGL(k_cut_capacity<6>, dim3(blocks), dim3(256), 0, mStream, d.nx, d.ny, d.nz, side.get(),
+		hipLaunchKernelGGL(k_cut_capacity<6>, dim3(blocks), dim3(256), 0, mStream, d.nx, d.ny, d.nz, side,
 		                   mCs.get() + d.vbase, mCt.get() + d.vbase, cn, cnb, sum.get());
 	} else {
-		hipLaunchKernelGGL(k_cut_capacity<26>, dim3(blocks), dim3(256), 0, mStream, d.nx, d.ny, d.nz, side.get(),
+		hipLaunchKernelGGL(k_cut_capacity<26>, dim3(blocks), dim3(256), 0, mStream, d.nx, d.ny, d.nz, side,
 		                   mCs.get() + d.vbase, mCt.get() + d.vbase, cn, cnb, sum.get());
 	}
 	CGP_LAUNCH_CHECK();
@@ -804,6 +811,11 @@ int64_t GridSolver::solve(const cgp_policy &policy, cgp_stats *stats_out, const 
 	std::memset(&stats, 0, sizeof(stats));
 	const auto t_start = std::chrono::steady_clock::now();
 	mSolved = false;
+	mStoppedEarly = false;
+	mLabelsExact = false;
+	if (mCertificate > 0 && mInst.size() != 1) {
+		fail(CGP_ERR_INVALID, "a certificate applies to a single grid, not a batch");
+	}
 	for (KernelRecord &rec : mProf) {
 		rec = KernelRecord();
 	}
@@ -864,12 +876,20 @@ int64_t GridSolver::solve(const cgp_policy &policy, cgp_stats *stats_out, const 
 		for (int p = 0; p < phases; ++p, ++phase) {
 			run_discharge_phase((int)(phase % ncolours), max_iters);
 			++stats.discharge_phases;
+			if (mCertificate > 0 && read_flow() >= mCertificate) {
+				// the flow into t equals the capacity of a cut: the preflow is maximum
+				mStoppedEarly = true;
+				break;
+			}
 		}
 		if (trace.fn) {
 			trace.emit(CGP_TRACE_AFTER_PUSH, round, 0, read_flow());
 		}
 		stats.discharge_ms += ms_since(t0);
 		++round;
+		if (mStoppedEarly) {
+			break;
+		}
 	}
 	auto t0 = std::chrono::steady_clock::now();
 	run_source_closure(stats);
@@ -891,6 +911,7 @@ int64_t GridSolver::solve(const cgp_policy &policy, cgp_stats *stats_out, const 
 	stats.tile_z = mShape.tz;
 	stats.conn = mConn;
 	mSolved = true;
+	mLabelsExact = !mStoppedEarly;
 	trace.emit(CGP_TRACE_AFTER_PUSH, round, 1, flow);
 	trace.emit(CGP_TRACE_FINISHED, round, 1, flow);
 	if (stats_out) {
@@ -899,11 +920,24 @@ int64_t GridSolver::solve(const cgp_policy &policy, cgp_stats *stats_out, const 
 	return flow;
 }
 
+// After a solve that stopped at a certificate the labels are those of an earlier relabel: one exact
+// relabel of the final residual graph gives the vertices that still reach t.
+void GridSolver::ensure_exact_labels()
+{
+	if (!mLabelsExact) {
+		cgp_stats scratch;
+		std::memset(&scratch, 0, sizeof(scratch));
+		run_global_relabel(0, scratch);
+		mLabelsExact = true;
+	}
+}
+
 void GridSolver::sink_side(int index, uint8_t *host_out, uint8_t mask, uint8_t background)
 {
 	if (!mSolved) {
 		fail(CGP_ERR_STATE, "fill_segments before max_flow");
 	}
+	ensure_exact_labels();
 	const InstanceDesc &d = instance(index);
 	DeviceBuffer<uint8_t> tmp(d.nvox);
 	if (mWave) {
@@ -940,6 +974,7 @@ void GridSolver::labels(int index, int32_t *host_out)
 	if (!mSolved) {
 		fail(CGP_ERR_STATE, "labels before max_flow");
 	}
+	ensure_exact_labels();
 	const InstanceDesc &d = instance(index);
 	if (mWave) {
 		// the face labels are final: one pass over every tile yields every voxel's exact label
@@ -989,6 +1024,31 @@ void GridSolver::state(int index, int32_t *excess, int32_t *labels, uint32_t *sa
 		CGP_HIP(hipMemcpyAsync(labels, mH.get() + d.vbase, d.nvox * sizeof(int32_t), hipMemcpyDeviceToHost, mStream));
 	}
 	CGP_HIP(hipStreamSynchronize(mStream));
+}
+
+GridSolver::DeviceGrid GridSolver::device_grid(int index) const
+{
+	if (mInst.empty()) {
+		fail(CGP_ERR_STATE, "no grid set");
+	}
+	const InstanceDesc &d = instance(index);
+	return DeviceGrid{d.nx, d.ny, d.nz, mCs.get() + d.vbase, mCt.get() + d.vbase};
+}
+
+const uint8_t *GridSolver::device_source_side(int index)
+{
+	if (!mSolved) {
+		fail(CGP_ERR_STATE, "the grid is not solved");
+	}
+	const InstanceDesc &d = instance(index);
+	if (mWave) {
+		GridView g = CGP_VIEW();
+		hipLaunchKernelGGL(wave::k_wave_plane_bytes, dim3(mTiles), dim3(64), 0, mStream, g, mSBits.get(), mS.get(),
+		                   (uint8_t)1, (uint8_t)0);
+		CGP_LAUNCH_CHECK();
+		CGP_HIP(hipStreamSynchronize(mStream));
+	}
+	return mS.get() + d.vbase;
 }
 
 }  // namespace cgp

@@ -45,6 +45,18 @@ public:
 	void capacities(int index, int32_t *cs, int32_t *ct, int32_t *cn, int32_t *cnb);
 	void state(int index, int32_t *excess, int32_t *labels, uint32_t *saturated);
 	int64_t cut_capacity(int index, const uint8_t *host_side);
+	int64_t cut_capacity_device(int index, const uint8_t *device_side);
+	// Device views for the learned cut proposal.
+	struct DeviceGrid {
+		int nx, ny, nz;
+		const int32_t *cs, *ct;
+	};
+	DeviceGrid device_grid(int index) const;
+	const uint8_t *device_source_side(int index);
+	// A certificate > 0 (the capacity of some cut) lets the next solve stop as soon as the flow into
+	// t reaches it; 0 disables it. Single grids only.
+	void set_certificate(int64_t certificate) { mCertificate = certificate; }
+	bool stopped_early() const { return mStoppedEarly; }
 	int conn() const { return mConn; }
 	const InstanceDesc &grid(int index) const { return instance(index); }
 	// Returns the total flow of all instances; per-instance values in flows().
@@ -108,6 +120,10 @@ private:
 	int mNtx = 0, mNty = 0, mNtz = 0, mTiles = 0, mMaxColourTiles = 0;
 	bool mSolved = false;
 	bool mStarted = false;  // a solve has initialised the residual state
+	int64_t mCertificate = 0;
+	bool mStoppedEarly = false;
+	bool mLabelsExact = false;  // labels (T_min) come from an exact relabel of the final state
+	void ensure_exact_labels();
 	bool mHasBack = false;
 	bool mWideResiduals = true;  // some edge's capacity pair exceeds 32767: stage residuals as int32
 

@@ -266,6 +266,37 @@ int cgp_set_grid_volume(cgp_graph graph, int32_t nx, int32_t ny, int32_t nz, int
 int cgp_grid_capacities(cgp_graph graph, int32_t index, int32_t *cap_source, int32_t *cap_sink, int32_t *cap_n,
                         int32_t *cap_n_back);
 
+/* ---- learned cut proposal (3D CNN on the matrix cores) --------------------------------------
+   Four 3x3x3 convolutions with 16 channels and ReLU, then a 1x1x1 sigmoid head; BF16 operands,
+   FP32 accumulation (v_mfma_f32_16x16x16_bf16). Input per voxel: intensity / 255, source and sink
+   t-link / 1000. Parameters (fp32, CGP_CNN_PARAMS): W1[27][4][16] (tap, input channel, output
+   channel; input channel 3 unused), b1[16], then W2, W3, W4 [27][16][16] each followed by its bias
+   [16], then w_head[16], b_head; tap t = (dz+1)*9 + (dy+1)*3 + (dx+1). The network reads grid 0 of
+   a graph handle (its t-links, already on the device) plus the volume. The thresholded output is
+   a candidate source side whose cut capacity bounds the max flow from above: a certificate.
+   No counterpart in the reference. */
+#define CGP_CNN_PARAMS 22545
+typedef struct cgp_cnn_s *cgp_cnn;
+int cgp_cnn_create(cgp_cnn *out);
+int cgp_cnn_destroy(cgp_cnn cnn);
+const char *cgp_cnn_last_error(cgp_cnn cnn);
+int cgp_cnn_set_params(cgp_cnn cnn, const float *params, int32_t count);
+int cgp_cnn_get_params(cgp_cnn cnn, float *params, int32_t count);
+/* Prediction for the graph's grid from its volume (host bytes): side (host, 0/1 bytes) and prob
+   (host floats) may be NULL; certificate (may be NULL) receives the exact capacity of the cut. */
+int cgp_cnn_predict(cgp_cnn cnn, cgp_graph graph, const uint8_t *volume, uint8_t *side, float *prob,
+                    int64_t *certificate);
+/* Mean binary cross-entropy against the graph's S_min (the graph must be solved) and its gradient
+   (grad: CGP_CNN_PARAMS floats, may be NULL). */
+int cgp_cnn_gradients(cgp_cnn cnn, cgp_graph graph, const uint8_t *volume, double *loss, float *grad);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on that gradient; loss before the step. */
+int cgp_cnn_train_step(cgp_cnn cnn, cgp_graph graph, const uint8_t *volume, float learning_rate, double *loss);
+/* Solve with a certificate (the capacity of some cut): stops as soon as the flow into t reaches
+   it, runs to completion otherwise; stopped_early (may be NULL) says which. Results are exact
+   either way; fill_segments after an early stop runs one exact relabel first. */
+int cgp_max_flow_certified(cgp_graph graph, int64_t certificate, int64_t *flow, cgp_stats *stats,
+                           int32_t *stopped_early);
+
 /* ---- synthetic instances ------------------------------------------------------------------
    Blob volume (3-8 ellipsoids at 180 on 60, N(0, 25^2) noise, clamped to 0-255) and the int32
    capacities derived from it (Boykov-Jolly t-links, contrast-sensitive n-links). Integer-only:

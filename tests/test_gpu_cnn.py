@@ -1,0 +1,119 @@
+"""Learned cut proposal on the GPU (cgp_cnn_*, cgp_max_flow_certified): the matrix-core forward and
+backward passes against the numpy oracle (oracle/cnn.py, same bfloat16 rounding points, float64
+sums), training on the solver's own cuts, and the certified solve, whose results are exact whether
+or not the certificate stops it early."""
+import time
+
+import numpy as np
+import pytest
+
+import cugip_amd
+from oracle import cnn as ocnn
+from tests.gpu_util import grid_cut_capacity, say
+
+pytestmark = pytest.mark.gpu
+
+
+def params(seed):
+    p = cugip_amd.Cnn.init_params(seed)
+    rng = np.random.default_rng(seed + 100)
+    W, b, wh, bh = ocnn.unpack(p)
+    b = [rng.normal(0, 0.05, 16) for _ in range(4)]
+    return ocnn.pack(W, b, wh, 0.1).astype(np.float32)
+
+
+def test_forward_matches_oracle():
+    """Odd dimensions (partial bricks on every axis). Tolerance: the GPU sums in float32 and the
+    oracle in float64, so a stored bfloat16 can differ by one unit in the last place."""
+    nx, ny, nz = 21, 13, 11
+    say(f"cnn forward {nx}x{ny}x{nz}")
+    vol, cs, ct, cn = cugip_amd.make_instance(nx, ny, nz, 5)
+    p = params(3)
+    with cugip_amd.GridGraph() as g, cugip_amd.Cnn() as net:
+        g.set_grid(cs, ct, cn)
+        net.set_params(p)
+        side, prob, cert = net.predict(g, vol)
+    z, _ = ocnn.forward(p, vol, cs, ct, bf=True)
+    ref = 1.0 / (1.0 + np.exp(-z.reshape(-1)))
+    err = np.abs(prob - ref)
+    assert err.max() < 2e-2, err.max()
+    assert np.quantile(err, 0.999) < 2e-3, np.quantile(err, 0.999)
+    zf = z.reshape(-1)
+    differ = side != (zf > 0)
+    assert not differ[np.abs(zf) > 2e-2].any()
+    assert cert == grid_cut_capacity(side, cs, ct, cn)
+
+
+def test_gradients_match_oracle():
+    nx, ny, nz = 19, 14, 10
+    say(f"cnn gradients {nx}x{ny}x{nz}")
+    vol, cs, ct, cn = cugip_amd.make_instance(nx, ny, nz, 8)
+    p = params(4)
+    with cugip_amd.GridGraph() as g, cugip_amd.Cnn() as net:
+        g.set_grid(cs, ct, cn)
+        g.max_flow()
+        label = g.source_set()
+        net.set_params(p)
+        loss, grad = net.gradients(g, vol)
+    ref_loss, ref_grad = ocnn.loss_and_grad(p, vol, cs, ct, label.reshape(nz, ny, nx), bf=True)
+    assert abs(loss - ref_loss) <= 1e-3 * abs(ref_loss)
+    bounds = [0, ocnn.N_W1, ocnn.N_W1 + 16]
+    for _ in range(3):
+        bounds += [bounds[-1] + ocnn.N_W, bounds[-1] + ocnn.N_W + 16]
+    bounds += [bounds[-1] + 16, bounds[-1] + 17]
+    for lo, hi in zip(bounds[:-1], bounds[1:]):
+        a, b = grad[lo:hi].astype(np.float64), ref_grad[lo:hi]
+        nb = np.linalg.norm(b)
+        if nb == 0:
+            assert np.linalg.norm(a) == 0
+            continue
+        rel = np.linalg.norm(a - b) / nb
+        cos = float(a @ b / (np.linalg.norm(a) * nb))
+        assert rel < 3e-2 and cos > 0.999, (lo, hi, rel, cos)
+
+
+def test_training_tightens_the_certificate_and_certified_solves_are_exact():
+    """Training on the solver's S_min lowers the loss and brings the certificate (capacity of the
+    predicted cut) closer to the max flow than the untrained network's. The certified solve
+    returns the plain solve's flow, S_min and T_min, and stops early when the certificate is the
+    max flow itself."""
+    say("cnn training and certificate")
+    n = 40
+    train = [cugip_amd.make_instance(n, n, n, 300 + i) for i in range(6)]
+    test = cugip_amd.make_instance(n, n, n, 400)
+    vol_t, cs_t, ct_t, cn_t = test
+    with cugip_amd.Cnn() as net, cugip_amd.GridGraph() as gt:
+        gt.set_grid(cs_t, ct_t, cn_t)
+        flow = gt.max_flow()
+        smin = gt.source_set()
+        tmin = gt.fill_segments(1, 0)
+        net.set_params(cugip_amd.Cnn.init_params(0))
+        _, _, cert0 = net.predict(gt, vol_t)
+        graphs = []
+        for vol, cs, ct, cn in train:
+            g = cugip_amd.GridGraph()
+            g.set_grid(cs, ct, cn)
+            g.max_flow()
+            graphs.append((g, vol))
+        losses = []
+        for step in range(90):
+            g, vol = graphs[step % len(graphs)]
+            losses.append(net.train_step(g, vol, 3e-3))
+        for g, _ in graphs:
+            g.close()
+        side, prob, cert1 = net.predict(gt, vol_t)
+        acc = float((side == smin).mean())
+        say(f"  loss {np.mean(losses[:6]):.4f} -> {np.mean(losses[-6:]):.4f}, certificate gap "
+            f"{cert0 - flow} -> {cert1 - flow} (flow {flow}), voxel accuracy {acc:.4f}")
+        assert np.mean(losses[-6:]) < 0.7 * np.mean(losses[:6])
+        assert flow <= cert1 < cert0
+        for certificate in (cert1, flow):
+            t0 = time.perf_counter()
+            f, early = gt.max_flow_certified(certificate)
+            ms = (time.perf_counter() - t0) * 1e3
+            say(f"  certificate {certificate}: flow {f}, stopped early {early}, {ms:.2f} ms")
+            assert f == flow
+            assert np.array_equal(gt.source_set(), smin)
+            assert np.array_equal(gt.fill_segments(1, 0), tmin)
+            assert early == (certificate == flow) or certificate != flow
+        assert early  # the last certificate was the max flow itself
