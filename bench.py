@@ -8,8 +8,11 @@ Prints one JSON line with the throughput (Mvoxels/s), the roofline of the domina
 (HIP events on the solver's stream, algorithmic bytes from the solver's voxel-visit counters)
 and the CPU baseline (Boykov-Kolmogorov on one host core, maxflow() only).
 
-Also reported (config.batched_vs_plain): 8 grids of the same size solved as one batch against the
-plain solver run on each in turn, bit-exact per grid.
+Also reported: config.batched_vs_plain (8 grids of the same size solved as one batch against the
+plain solver run on each in turn, bit-exact per grid), config.learned_vs_plain (the learned cut
+proposal's inference + certified solve against the plain solve on the same grids) and
+config.learned_batched_speedup (inference for every grid + one batched solve, against the plain
+solves).
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 256] [--seed 0] [--no-cpu] [--batch 8]
 """
@@ -99,6 +102,53 @@ def batched_vs_plain(cugip_amd, n, conn, seed, count, repeats=3):
                 batched_mvox_s=round(vox / batch_s / 1e6, 1), plain_mvox_s=round(vox / plain_s / 1e6, 1),
                 speedup=round(plain_s / batch_s, 3), batched_ms=round(batch_s * 1e3, 3),
                 plain_ms=round(plain_s * 1e3, 3), bit_exact=bool(same))
+
+
+def learned_vs_plain(cugip_amd, n, seed, count, weights, repeats=3):
+    """The learned cut proposal on the batched-comparison grids: per grid, inference + certificate
+    (capacity of the predicted cut) + the certified solve, against the plain solve; flows and S_min
+    must be identical. Also the learned+batched pipeline: inference for every grid plus one batched
+    solve (the certificate applies to single grids, so the batch solve runs to completion)."""
+    import numpy as np
+    params = np.load(weights)
+    plain = learned = infer_total = 0.0
+    early = 0
+    exact = True
+    gaps = []
+    with cugip_amd.Cnn() as net:
+        net.set_params(params)
+        for i in range(count):
+            vol = cugip_amd.generate_volume(n, n, n, seed, i)
+            with cugip_amd.GridGraph() as g:
+                g.set_grid_volume(vol)
+                flow = g.max_flow()
+                smin = np.packbits(g.source_set())
+                ts = []
+                for _ in range(repeats):
+                    t0 = time.perf_counter()
+                    g.max_flow()
+                    ts.append(time.perf_counter() - t0)
+                plain += min(ts)
+                net.certificate(g)  # warm-up
+                best = None
+                for _ in range(repeats):
+                    t0 = time.perf_counter()
+                    cert = net.certificate(g)
+                    t1 = time.perf_counter()
+                    f, stopped = g.max_flow_certified(cert)
+                    t2 = time.perf_counter()
+                    if best is None or t2 - t0 < best[0]:
+                        best = (t2 - t0, t1 - t0)
+                learned += best[0]
+                infer_total += best[1]
+                early += int(stopped)
+                gaps.append((cert - flow) / flow)
+                exact &= f == flow and np.array_equal(np.packbits(g.source_set()), smin)
+    return dict(workload=f"{count} grids of {n}^3 6-connected (seed {seed}), weights {os.path.basename(weights)} "
+                         "(trained on 2000 grids of 128^3)",
+                speedup=round(plain / learned, 3), plain_ms=round(plain * 1e3, 3), learned_ms=round(learned * 1e3, 3),
+                inference_ms=round(infer_total * 1e3, 3), stopped_early=early,
+                median_relative_certificate_gap=float(np.median(gaps)), bit_exact=bool(exact))
 
 
 def calibration_copies(cugip_amd, cs, ct, cn):
@@ -266,9 +316,17 @@ def main():
                     bytes_per_voxel_visit=dom["bytes_per_voxel"])
 
     batch = None
+    learned = None
+    learned_batched = None
     if args.batch > 0:
         batch = batched_vs_plain(cugip_amd, n, args.conn, args.seed, args.batch)
         log(f"batched vs plain: {batch}")
+        weights = os.path.join(ROOT, "cugip_amd", "cnn_weights.npy")
+        if args.conn == 6 and os.path.exists(weights):
+            learned = learned_vs_plain(cugip_amd, n, args.seed, args.batch, weights)
+            log(f"learned vs plain: {learned}")
+            # inference for every grid plus one batched solve, against the plain solves
+            learned_batched = round(batch["plain_ms"] / (batch["batched_ms"] + learned["inference_ms"]), 3)
 
     cpu = None
     if not args.no_cpu:
@@ -304,7 +362,8 @@ def main():
             "verified_bit_exact": verified,
             "h2d_setup_ms": round(h2d_ms, 2),
             "batched_vs_plain": batch,
-            "learned_batched_speedup": None,
+            "learned_vs_plain": learned,
+            "learned_batched_speedup": learned_batched,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -552,6 +552,10 @@ def write_mfi(path, cap_source, cap_sink, cap_n, cap_n_back=None, conn=6, maxflo
 CNN_PARAMS = 22545
 
 
+def _vol_ptr(vol):
+    return vol.ctypes.data_as(_u8p) if vol is not None else None
+
+
 class Cnn:
     """Learned cut proposal (cgp_cnn_*): a 3D CNN run on the matrix cores. Parameter layout: see
     cugip_hip.h. Prediction, gradients and training act on grid 0 of a GridGraph plus its volume."""
@@ -607,6 +611,8 @@ class Cnn:
 
     @staticmethod
     def _volume(graph, volume):
+        if volume is None:
+            return None  # the volume the grid was built from (set_grid_volume), on the device
         vol = np.ascontiguousarray(volume, dtype=np.uint8).reshape(-1)
         if vol.size != graph._n:
             raise CgpError(CGP_ERR_INVALID, "volume does not match the graph's grid")
@@ -615,26 +621,33 @@ class Cnn:
     def predict(self, graph, volume):
         """(side 0/1 bytes, probabilities, certificate = capacity of the predicted cut)."""
         vol = self._volume(graph, volume)
-        side = np.empty(vol.size, np.uint8)
-        prob = np.empty(vol.size, np.float32)
+        side = np.empty(graph._n, np.uint8)
+        prob = np.empty(graph._n, np.float32)
         cert = ctypes.c_int64(0)
-        self._check(_lib.cgp_cnn_predict(self._h, graph._h, vol.ctypes.data_as(_u8p), side.ctypes.data_as(_u8p),
+        self._check(_lib.cgp_cnn_predict(self._h, graph._h, _vol_ptr(vol), side.ctypes.data_as(_u8p),
                                          prob.ctypes.data_as(_fp), ctypes.byref(cert)))
         return side, prob, int(cert.value)
+
+    def certificate(self, graph, volume=None):
+        """Capacity of the predicted cut only (nothing copied back but the number)."""
+        vol = self._volume(graph, volume)
+        cert = ctypes.c_int64(0)
+        self._check(_lib.cgp_cnn_predict(self._h, graph._h, _vol_ptr(vol), None, None, ctypes.byref(cert)))
+        return int(cert.value)
 
     def gradients(self, graph, volume):
         """(loss, gradient) against the graph's S_min (the graph must be solved)."""
         vol = self._volume(graph, volume)
         grad = np.empty(CNN_PARAMS, np.float32)
         loss = ctypes.c_double(0)
-        self._check(_lib.cgp_cnn_gradients(self._h, graph._h, vol.ctypes.data_as(_u8p), ctypes.byref(loss),
+        self._check(_lib.cgp_cnn_gradients(self._h, graph._h, _vol_ptr(vol), ctypes.byref(loss),
                                            grad.ctypes.data_as(_fp)))
         return float(loss.value), grad
 
     def train_step(self, graph, volume, learning_rate=1e-3):
         vol = self._volume(graph, volume)
         loss = ctypes.c_double(0)
-        self._check(_lib.cgp_cnn_train_step(self._h, graph._h, vol.ctypes.data_as(_u8p), float(learning_rate),
+        self._check(_lib.cgp_cnn_train_step(self._h, graph._h, _vol_ptr(vol), float(learning_rate),
                                             ctypes.byref(loss)))
         return float(loss.value)
 

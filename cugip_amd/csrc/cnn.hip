@@ -435,6 +435,14 @@ __global__ __launch_bounds__(256) void k_threshold(int64_t n, const float *__res
 	}
 }
 
+__global__ __launch_bounds__(256) void k_sigmoid(int64_t n, const float *__restrict__ logits, float *__restrict__ prob)
+{
+	const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+	for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+		prob[v] = 1.0f / (1.0f + expf(-logits[v]));
+	}
+}
+
 int blocks_of(int64_t n, int cap = 8192)
 {
 	int64_t b = (n + 255) / 256;
@@ -513,7 +521,11 @@ void CnnModel::ensure(int64_t vox)
 void CnnModel::forward(GridSolver &grid, const uint8_t *volume, bool device_volume, bool)
 {
 	if (!volume) {
-		fail(CGP_ERR_INVALID, "volume pointer must not be null");
+		volume = grid.device_volume();  // the grid was built from its volume on the device
+		device_volume = true;
+		if (!volume) {
+			fail(CGP_ERR_INVALID, "no volume: pass one, or build the grid with cgp_set_grid_volume");
+		}
 	}
 	const GridSolver::DeviceGrid d = grid.device_grid(0);
 	ensure((int64_t)d.nx * d.ny * d.nz);
@@ -558,12 +570,11 @@ void CnnModel::prediction_to_host(uint8_t *side, float *prob)
 		CGP_HIP(hipMemcpyAsync(side, mSide.get(), mVox, hipMemcpyDeviceToHost, mStream));
 	}
 	if (prob) {
-		std::vector<float> z((size_t)mVox);
-		CGP_HIP(hipMemcpyAsync(z.data(), mLogit.get(), mVox * sizeof(float), hipMemcpyDeviceToHost, mStream));
+		DeviceBuffer<float> p(mVox);
+		hipLaunchKernelGGL(k_sigmoid, dim3(blocks_of(mVox)), dim3(256), 0, mStream, mVox, mLogit.get(), p.get());
+		CGP_LAUNCH_CHECK();
+		CGP_HIP(hipMemcpyAsync(prob, p.get(), mVox * sizeof(float), hipMemcpyDeviceToHost, mStream));
 		CGP_HIP(hipStreamSynchronize(mStream));
-		for (int64_t i = 0; i < mVox; ++i) {
-			prob[i] = 1.0f / (1.0f + std::exp(-z[(size_t)i]));
-		}
 	}
 	CGP_HIP(hipStreamSynchronize(mStream));
 }

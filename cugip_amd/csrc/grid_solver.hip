@@ -225,6 +225,7 @@ void GridSolver::allocate_grids(int conn, const std::vector<InstanceDesc> &inst,
 	const size_t count = inst.size();
 	mSolved = false;
 	mStarted = false;
+	mVolumeKept = false;
 	mInst = inst;
 	mFlows.assign(count, 0);
 	mNx = inst[0].nx;
@@ -305,13 +306,15 @@ void GridSolver::set_grid_volume(int nx, int ny, int nz, int conn, const uint8_t
 	}
 	allocate_grids(conn, inst, false);
 	const int64_t n = inst[0].nvox;
-	DeviceBuffer<uint8_t> vol, mark;
+	DeviceBuffer<uint8_t> mark;
 	DeviceBuffer<cgp_volume_weights> w(1);
-	const uint8_t *dv = volume, *dm = markers;
+	const uint8_t *dm = markers;
+	// the volume stays on the device (the learned cut proposal reads it)
+	mVolume.allocate(n);
+	CGP_HIP(hipMemcpyAsync(mVolume.get(), volume, n, device_pointers ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+	                       mStream));
+	const uint8_t *dv = mVolume.get();
 	if (!device_pointers) {
-		vol.allocate(n);
-		CGP_HIP(hipMemcpyAsync(vol.get(), volume, n, hipMemcpyHostToDevice, mStream));
-		dv = vol.get();
 		if (markers) {
 			mark.allocate(n);
 			CGP_HIP(hipMemcpyAsync(mark.get(), markers, n, hipMemcpyHostToDevice, mStream));
@@ -329,6 +332,7 @@ void GridSolver::set_grid_volume(int nx, int ny, int nz, int conn, const uint8_t
 	}
 	CGP_LAUNCH_CHECK();
 	CGP_HIP(hipStreamSynchronize(mStream));
+	mVolumeKept = true;
 }
 
 int64_t GridSolver::cut_capacity(int index, const uint8_t *host_side)

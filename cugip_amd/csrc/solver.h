@@ -53,6 +53,8 @@ public:
 	};
 	DeviceGrid device_grid(int index) const;
 	const uint8_t *device_source_side(int index);
+	// The volume of a grid set by set_grid_volume (kept on the device), else null.
+	const uint8_t *device_volume() const { return mVolumeKept ? mVolume.get() : nullptr; }
 	// A certificate > 0 (the capacity of some cut) lets the next solve stop as soon as the flow into
 	// t reaches it; 0 disables it. Single grids only.
 	void set_certificate(int64_t certificate) { mCertificate = certificate; }
@@ -121,6 +123,8 @@ private:
 	bool mSolved = false;
 	bool mStarted = false;  // a solve has initialised the residual state
 	int64_t mCertificate = 0;
+	DeviceBuffer<uint8_t> mVolume;  // volume of set_grid_volume
+	bool mVolumeKept = false;
 	bool mStoppedEarly = false;
 	bool mLabelsExact = false;  // labels (T_min) come from an exact relabel of the final state
 	void ensure_exact_labels();

@@ -282,8 +282,10 @@ int cgp_cnn_destroy(cgp_cnn cnn);
 const char *cgp_cnn_last_error(cgp_cnn cnn);
 int cgp_cnn_set_params(cgp_cnn cnn, const float *params, int32_t count);
 int cgp_cnn_get_params(cgp_cnn cnn, float *params, int32_t count);
-/* Prediction for the graph's grid from its volume (host bytes): side (host, 0/1 bytes) and prob
-   (host floats) may be NULL; certificate (may be NULL) receives the exact capacity of the cut. */
+/* Prediction for the graph's grid from its volume (host bytes; NULL = the volume the grid was built
+   from by cgp_set_grid_volume, already on the device). side (host, 0/1 bytes) and prob (host
+   floats) may be NULL; certificate (may be NULL) receives the exact capacity of the cut. The same
+   volume rule holds for cgp_cnn_gradients and cgp_cnn_train_step. */
 int cgp_cnn_predict(cgp_cnn cnn, cgp_graph graph, const uint8_t *volume, uint8_t *side, float *prob,
                     int64_t *certificate);
 /* Mean binary cross-entropy against the graph's S_min (the graph must be solved) and its gradient
