@@ -106,20 +106,33 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_init(GridView gv, int ntiles)
 }
 
 // ---------------------------------------------------------------------------------------------
-// Stage the label tile with its halo into LDS; voxels outside the volume read as kLabelInf.
+// Stage the label tile with its halo into LDS; voxels outside the volume read as kLabelInf. A fixed
+// number of entries per thread, all loads issued before the first LDS store, so that one memory
+// latency covers them all.
 template <typename T>
 __device__ inline void load_label_tile(const GridView &g, int32_t *hl, int ox, int oy, int oz)
 {
-	for (int p = threadIdx.x; p < T::NP; p += T::NT) {
+	constexpr int kPer = (T::NP + T::NT - 1) / T::NT;
+	int32_t vals[kPer];
+#pragma unroll
+	for (int it = 0; it < kPer; ++it) {
+		const int p = (int)threadIdx.x + it * T::NT;
 		const int px = p % T::PX;
 		const int py = (p / T::PX) % T::PY;
 		const int pz = p / (T::PX * T::PY);
 		const int x = ox + px - 1, y = oy + py - 1, z = oz + pz - 1;
 		int32_t v = kLabelInf;
-		if (inside(g, x, y, z)) {
+		if (p < T::NP && inside(g, x, y, z)) {
 			v = g.h[voxel_index(g, x, y, z)];
 		}
-		hl[p] = v;
+		vals[it] = v;
+	}
+#pragma unroll
+	for (int it = 0; it < kPer; ++it) {
+		const int p = (int)threadIdx.x + it * T::NT;
+		if (p < T::NP) {
+			hl[p] = vals[it];
+		}
 	}
 }
 
