@@ -110,6 +110,42 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_init(GridView gv, int ntiles)
 // number of entries per thread, all loads issued before the first LDS store, so that one memory
 // latency covers them all.
 template <typename T>
+struct HaloLabels {
+	static constexpr int kPer = (T::NP + T::NT - 1) / T::NT;
+	int32_t vals[kPer];
+};
+
+template <typename T>
+__device__ inline void fetch_label_tile(const GridView &g, HaloLabels<T> &halo, int ox, int oy, int oz)
+{
+#pragma unroll
+	for (int it = 0; it < HaloLabels<T>::kPer; ++it) {
+		const int p = (int)threadIdx.x + it * T::NT;
+		const int px = p % T::PX;
+		const int py = (p / T::PX) % T::PY;
+		const int pz = p / (T::PX * T::PY);
+		const int x = ox + px - 1, y = oy + py - 1, z = oz + pz - 1;
+		int32_t v = kLabelInf;
+		if (p < T::NP && inside(g, x, y, z)) {
+			v = g.h[voxel_index(g, x, y, z)];
+		}
+		halo.vals[it] = v;
+	}
+}
+
+template <typename T>
+__device__ inline void store_label_tile(int32_t *hl, const HaloLabels<T> &halo)
+{
+#pragma unroll
+	for (int it = 0; it < HaloLabels<T>::kPer; ++it) {
+		const int p = (int)threadIdx.x + it * T::NT;
+		if (p < T::NP) {
+			hl[p] = halo.vals[it];
+		}
+	}
+}
+
+template <typename T>
 __device__ inline void load_label_tile(const GridView &g, int32_t *hl, int ox, int oy, int oz)
 {
 	constexpr int kPer = (T::NP + T::NT - 1) / T::NT;
@@ -218,10 +254,10 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_bfs(GridView gv, const int32_
                                                        uint8_t *__restrict__ tile_active)
 {
 	using T = Tile<kConn, kTX, kTY, kTZ>;
+	const int tile = list[blockIdx.x];  // in bounds: the grid never exceeds the list's capacity
 	if ((int)blockIdx.x >= *count) {
 		return;
 	}
-	const int tile = list[blockIdx.x];
 	int local;
 	const GridView g = instance_view(gv, tile, local);
 	int tx, ty, tz;
@@ -236,16 +272,24 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_bfs(GridView gv, const int32_
 	if (tid < 27) {
 		marks[tid] = 0;
 	}
-	load_label_tile<T>(g, hl, ox, oy, oz);
+	// every global load of the visit first (halo labels, state words), then the LDS stores
+	HaloLabels<T> halo;
+	fetch_label_tile<T>(g, halo, ox, oy, oz);
+	uint32_t valid = 0;
+	uint32_t state[T::TZ];
+#pragma unroll
+	for (int k = 0; k < T::TZ; ++k) {
+		valid |= (gx < g.nx && gy < g.ny && oz + k < g.nz) ? (1u << k) : 0u;
+		state[k] = (threshold == 0 && (valid & (1u << k))) ? g.omask[voxel_index(g, gx, gy, oz + k)] : 0u;
+	}
+	store_label_tile<T>(hl, halo);
 	__syncthreads();
 
-	uint32_t valid = 0;
 	uint32_t open[T::TZ];
 	int32_t before[T::TZ];
 	bool has_excess[T::TZ];
 #pragma unroll
 	for (int k = 0; k < T::TZ; ++k) {
-		valid |= (gx < g.nx && gy < g.ny && oz + k < g.nz) ? (1u << k) : 0u;
 		open[k] = 0;
 		has_excess[k] = false;
 		const int p = T::pidx(lx, ly, k);
@@ -255,7 +299,7 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_bfs(GridView gv, const int32_
 			bool sink_open;
 			if (threshold == 0) {
 				// exact relabel: the state word has every open direction
-				const uint32_t st = g.omask[v];
+				const uint32_t st = state[k];
 				open[k] = st & ((1u << T::K) - 1u);
 				has_excess[k] = (st & kStateExcess) != 0;
 				sink_open = (st & kStateSink) != 0;
@@ -341,10 +385,10 @@ __global__ __launch_bounds__(kTX *kTY, 6) void k_grid_discharge(GridView gv, con
 	using D = typename T::D;
 	using Dirty = DirtyBits<kConn>;
 	constexpr int K = T::K;
+	const int tile = list[blockIdx.x];  // in bounds: the grid never exceeds the list's capacity
 	if ((int)blockIdx.x >= *count) {
 		return;
 	}
-	const int tile = list[blockIdx.x];
 	int local;
 	const GridView g = instance_view(gv, tile, local);
 	int tx, ty, tz;
@@ -616,10 +660,10 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_source_closure(GridView gv, c
 {
 	using T = Tile<kConn, kTX, kTY, kTZ>;
 	using D = typename T::D;
+	const int tile = list[blockIdx.x];  // in bounds: the grid never exceeds the list's capacity
 	if ((int)blockIdx.x >= *count) {
 		return;
 	}
-	const int tile = list[blockIdx.x];
 	int local;
 	const GridView g = instance_view(gv, tile, local);
 	int tx, ty, tz;
