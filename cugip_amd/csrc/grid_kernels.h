@@ -350,6 +350,25 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_bfs(GridView gv, const int32_
 	}
 }
 
+// Admissible direction masks of a thread's TZ voxels: packed into one word when K * TZ <= 32.
+template <int K, int TZ>
+struct AdmissibleDirs {
+	static constexpr bool kPacked = K * TZ <= 32;
+	uint32_t w[kPacked ? 1 : TZ] = {};
+	__device__ void set(int k, uint32_t m)
+	{
+		if (kPacked) {
+			w[0] = (w[0] & ~(((1u << K) - 1u) << (K * k))) | (m << (K * k));
+		} else {
+			w[kPacked ? 0 : k] = m;
+		}
+	}
+	__device__ uint32_t get(int k) const
+	{
+		return kPacked ? (w[0] >> (K * k)) & ((1u << K) - 1u) : w[kPacked ? 0 : k];
+	}
+};
+
 // ---------------------------------------------------------------------------------------------
 // Discharge one tile: repeated {region relabel, top-down level push} in LDS, then write back.
 // Runs only on tiles of one colour at a time, so no neighbouring tile is running.
@@ -473,13 +492,33 @@ __global__ __launch_bounds__(kTX *kTY, 6) void k_grid_discharge(GridView gv, con
 		__syncthreads();
 		relax_tile<T>(hl, open, valid, lx, ly);
 
-		// ---- active voxels and the top level
+		// ---- per own voxel, fixed for the push sweep below: its label, its admissible directions
+		// (open, neighbour one level lower; a push never opens one, since it goes one level down)
+		// and whether it held excess when the sweep began; and the top level holding excess
+		int32_t lab[T::TZ];
+		AdmissibleDirs<K, T::TZ> adm;
+		uint32_t held = 0;
 		int top = 0;
 #pragma unroll
 		for (int k = 0; k < T::TZ; ++k) {
-			const int32_t lab = hl[T::pidx(lx, ly, k)];
-			if ((valid & (1u << k)) && el[T::vidx(lx, ly, k)] > 0 && lab < kLabelInf && lab > top) {
-				top = lab;
+			const int p = T::pidx(lx, ly, k);
+			lab[k] = kLabelInf;
+			if (!(valid & (1u << k))) {
+				continue;
+			}
+			const int32_t l = hl[p];
+			lab[k] = l;
+			uint32_t a = 0;
+#pragma unroll
+			for (int d = 0; d < K; ++d) {
+				if ((open[k] & (1u << d)) && hl[p + T::poff(d)] == l - 1) {
+					a |= 1u << d;
+				}
+			}
+			adm.set(k, a);
+			if (el[T::vidx(lx, ly, k)] > 0 && l < kLabelInf) {
+				held |= 1u << k;
+				top = l > top ? l : top;
 			}
 		}
 		++step;
@@ -495,23 +534,19 @@ __global__ __launch_bounds__(kTX *kTY, 6) void k_grid_discharge(GridView gv, con
 		// ---- top-down push sweep over the levels that hold excess. The labels are exact BFS
 		// distances inside the tile, so a voxel of level L pushes only to voxels of level L - 1:
 		// the next level to visit is L - 1 if this step pushed inside the tile, else the highest
-		// lower level whose voxels already hold excess (their excess does not change in this step).
+		// lower level whose voxels held excess when the sweep began (a voxel below the current
+		// level has received nothing yet).
 		while (level >= 1) {
 			int next = 0;
 #pragma unroll
 			for (int k = 0; k < T::TZ; ++k) {
-				if (!(valid & (1u << k))) {
-					continue;
-				}
-				const int p = T::pidx(lx, ly, k);
-				const int i = T::vidx(lx, ly, k);
-				const int32_t lab = hl[p];
-				if (lab != level) {
-					if (lab < level && lab > next && el[i] > 0) {
-						next = lab;
+				if (lab[k] != level) {
+					if (lab[k] < level && lab[k] > next && (held & (1u << k))) {
+						next = lab[k];
 					}
 					continue;
 				}
+				const int i = T::vidx(lx, ly, k);
 				int32_t ex = el[i];
 				if (ex <= 0) {
 					continue;
@@ -523,11 +558,12 @@ __global__ __launch_bounds__(kTX *kTY, 6) void k_grid_discharge(GridView gv, con
 					sink += f;
 					Dirty::touch(dm, i);
 				}
+				const uint32_t a = adm.get(k);
 #pragma unroll
 				for (int d = 0; d < K; ++d) {
-					if (ex > 0) {
+					if (ex > 0 && (a & (1u << d))) {
 						const int32_t rv = rl[d][i];
-						if (rv > 0 && hl[p + T::poff(d)] < level) {
+						if (rv > 0) {
 							const int32_t f = ex < rv ? ex : rv;
 							rl[d][i] = (RT)(rv - f);
 							ex -= f;
