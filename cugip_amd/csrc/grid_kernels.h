@@ -182,42 +182,82 @@ __device__ inline void mark_neighbour_tile(const GridView &g, uint8_t *flags, in
 	}
 }
 
-// Relax one own voxel: label = min(label, 1 + label of any neighbour reachable over an open edge).
-template <typename T>
-__device__ inline bool relax_voxel(int32_t *hl, int p, uint32_t open)
+// Index of the direction with offset (x, y, z), -1 if the neighbourhood has none.
+template <typename D>
+__host__ __device__ constexpr int dir_index(int x, int y, int z)
 {
-	const int32_t cur = hl[p];
-	int32_t best = cur;
-#pragma unroll
-	for (int d = 0; d < T::K; ++d) {
-		if (open & (1u << d)) {
-			const int32_t nb = hl[p + T::poff(d)] + 1;
-			best = nb < best ? nb : best;
+	for (int d = 0; d < D::K; ++d) {
+		if (D::dx(d) == x && D::dy(d) == y && D::dz(d) == z) {
+			return d;
 		}
 	}
-	if (best < cur) {
-		hl[p] = best;
-		return true;
-	}
-	return false;
+	return -1;
 }
 
-// Fixpoint of relax_voxel over the tile (own column up then down per sweep).
+// Fixpoint of h(u) = min(h(u), 1 + h(v)) over the open directions u -> v of the tile's valid
+// voxels, the halo labels fixed; hl holds the result when it returns. A round pulls the
+// directions that leave the thread's z-column from LDS and then runs the column up and down in
+// registers; the halo ends of the column never change and are read once. The fixpoint does not
+// depend on the order of the relaxations.
 template <typename T>
-__device__ inline void relax_tile(int32_t *hl, const uint32_t (&open)[T::TZ], uint32_t valid, int lx, int ly)
+__device__ inline void relax_tile(int32_t *hl, uint32_t (&open)[T::TZ], uint32_t valid, int lx, int ly)
 {
+	using D = typename T::D;
+	constexpr int kMZ = dir_index<D>(0, 0, -1), kPZ = dir_index<D>(0, 0, 1);
+	int32_t lab[T::TZ], stored[T::TZ];
+#pragma unroll
+	for (int k = 0; k < T::TZ; ++k) {
+		lab[k] = stored[k] = hl[T::pidx(lx, ly, k)];
+	}
+	const int32_t below = hl[T::pidx(lx, ly, -1)] + 1, above = hl[T::pidx(lx, ly, T::TZ)] + 1;
 	for (;;) {
-		int changed = 0;
+		// open[] is left as it is; the empty asm keeps the optimiser from hoisting per-direction
+		// lane masks out of the round loop into scalar registers
 #pragma unroll
 		for (int k = 0; k < T::TZ; ++k) {
-			if (valid & (1u << k)) {
-				changed |= relax_voxel<T>(hl, T::pidx(lx, ly, k), open[k]);
+			asm volatile("" : "+v"(open[k]));
+		}
+#pragma unroll
+		for (int k = 0; k < T::TZ; ++k) {
+			if (!((valid >> k) & 1u)) {
+				continue;
+			}
+			const int p = T::pidx(lx, ly, k);
+			int32_t best = lab[k];
+#pragma unroll
+			for (int d = 0; d < T::K; ++d) {
+				if ((D::dx(d) != 0 || D::dy(d) != 0) && (open[k] & (1u << d))) {
+					const int32_t nb = hl[p + T::poff(d)] + 1;
+					best = nb < best ? nb : best;
+				}
+			}
+			if (k == 0 && (open[k] & (1u << kMZ)) && below < best) {
+				best = below;
+			}
+			if (k == T::TZ - 1 && (open[k] & (1u << kPZ)) && above < best) {
+				best = above;
+			}
+			lab[k] = best;
+		}
+#pragma unroll
+		for (int k = 1; k < T::TZ; ++k) {
+			if (((valid >> k) & 1u) && ((open[k] >> kMZ) & 1u) && lab[k - 1] + 1 < lab[k]) {
+				lab[k] = lab[k - 1] + 1;
 			}
 		}
 #pragma unroll
 		for (int k = T::TZ - 2; k >= 0; --k) {
-			if (valid & (1u << k)) {
-				changed |= relax_voxel<T>(hl, T::pidx(lx, ly, k), open[k]);
+			if (((valid >> k) & 1u) && ((open[k] >> kPZ) & 1u) && lab[k + 1] + 1 < lab[k]) {
+				lab[k] = lab[k + 1] + 1;
+			}
+		}
+		int changed = 0;
+#pragma unroll
+		for (int k = 0; k < T::TZ; ++k) {
+			if (lab[k] < stored[k]) {
+				hl[T::pidx(lx, ly, k)] = lab[k];
+				stored[k] = lab[k];
+				changed = 1;
 			}
 		}
 		if (!__syncthreads_or(changed)) {
