@@ -106,68 +106,87 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_init(GridView gv, int ntiles)
 }
 
 // ---------------------------------------------------------------------------------------------
-// Stage the label tile with its halo into LDS; voxels outside the volume read as kLabelInf. A fixed
-// number of entries per thread, all loads issued before the first LDS store, so that one memory
-// latency covers them all.
+// Stage the label tile with its halo into LDS; voxels outside the volume read as kLabelInf. Thread
+// (lx, ly) loads its own column with the two z-halo voxels (one z-slice per load, so the slice
+// start is uniform), and the threads share the lateral halo ring of every z-level. All loads are
+// issued before the first LDS store, so that one memory latency covers them all.
 template <typename T>
 struct HaloLabels {
-	static constexpr int kPer = (T::NP + T::NT - 1) / T::NT;
-	int32_t vals[kPer];
+	static constexpr int kRing = 2 * T::PX + 2 * (T::PY - 2);  // lateral halo voxels of one z-level
+	static constexpr int kRingPer = (kRing * T::PZ + T::NT - 1) / T::NT;
+	int32_t col[T::PZ];
+	int32_t ring[kRingPer];
+
+	// halo-box position of ring entry r of a z-level
+	__device__ static void ring_pos(int r, int &px, int &py)
+	{
+		if (r < T::PX) {
+			px = r;
+			py = 0;
+		} else if (r < 2 * T::PX) {
+			px = r - T::PX;
+			py = T::PY - 1;
+		} else {
+			const int q = r - 2 * T::PX;
+			px = (q & 1) ? T::PX - 1 : 0;
+			py = 1 + q / 2;
+		}
+	}
 };
 
 template <typename T>
 __device__ inline void fetch_label_tile(const GridView &g, HaloLabels<T> &halo, int ox, int oy, int oz)
 {
+	const int tid = threadIdx.x;
+	const int gx = ox + tid % T::TX, gy = oy + tid / T::TX;
+	const bool in_xy = gx < g.nx && gy < g.ny;
+	// every halo voxel lies at or above slice zb: 32-bit offsets from there
+	const int zb = oz > 0 ? oz - 1 : 0;
+	const int32_t *base = g.h + slice_start(g, zb);
+	const uint32_t xy = slice_xy(g, gx, gy), sxy = (uint32_t)(g.nx * g.ny);
 #pragma unroll
-	for (int it = 0; it < HaloLabels<T>::kPer; ++it) {
-		const int p = (int)threadIdx.x + it * T::NT;
-		const int px = p % T::PX;
-		const int py = (p / T::PX) % T::PY;
-		const int pz = p / (T::PX * T::PY);
-		const int x = ox + px - 1, y = oy + py - 1, z = oz + pz - 1;
+	for (int pz = 0; pz < T::PZ; ++pz) {
+		const int z = oz + pz - 1;
 		int32_t v = kLabelInf;
-		if (p < T::NP && inside(g, x, y, z)) {
-			v = g.h[voxel_index(g, x, y, z)];
+		if (in_xy && z >= 0 && z < g.nz) {
+			v = at(base, (uint32_t)(z - zb) * sxy + xy);
 		}
-		halo.vals[it] = v;
+		halo.col[pz] = v;
+	}
+#pragma unroll
+	for (int it = 0; it < HaloLabels<T>::kRingPer; ++it) {
+		const int e = tid + it * T::NT;
+		int32_t v = kLabelInf;
+		if (e < HaloLabels<T>::kRing * T::PZ) {
+			const int pz = e / HaloLabels<T>::kRing;
+			int px, py;
+			HaloLabels<T>::ring_pos(e % HaloLabels<T>::kRing, px, py);
+			const int x = ox + px - 1, y = oy + py - 1, z = oz + pz - 1;
+			if (inside(g, x, y, z)) {
+				v = at(base, (uint32_t)(z - zb) * sxy + slice_xy(g, x, y));
+			}
+		}
+		halo.ring[it] = v;
 	}
 }
 
 template <typename T>
 __device__ inline void store_label_tile(int32_t *hl, const HaloLabels<T> &halo)
 {
+	const int tid = threadIdx.x;
+	const int lx = tid % T::TX, ly = tid / T::TX;
 #pragma unroll
-	for (int it = 0; it < HaloLabels<T>::kPer; ++it) {
-		const int p = (int)threadIdx.x + it * T::NT;
-		if (p < T::NP) {
-			hl[p] = halo.vals[it];
-		}
-	}
-}
-
-template <typename T>
-__device__ inline void load_label_tile(const GridView &g, int32_t *hl, int ox, int oy, int oz)
-{
-	constexpr int kPer = (T::NP + T::NT - 1) / T::NT;
-	int32_t vals[kPer];
-#pragma unroll
-	for (int it = 0; it < kPer; ++it) {
-		const int p = (int)threadIdx.x + it * T::NT;
-		const int px = p % T::PX;
-		const int py = (p / T::PX) % T::PY;
-		const int pz = p / (T::PX * T::PY);
-		const int x = ox + px - 1, y = oy + py - 1, z = oz + pz - 1;
-		int32_t v = kLabelInf;
-		if (p < T::NP && inside(g, x, y, z)) {
-			v = g.h[voxel_index(g, x, y, z)];
-		}
-		vals[it] = v;
+	for (int pz = 0; pz < T::PZ; ++pz) {
+		hl[T::pidx(lx, ly, pz - 1)] = halo.col[pz];
 	}
 #pragma unroll
-	for (int it = 0; it < kPer; ++it) {
-		const int p = (int)threadIdx.x + it * T::NT;
-		if (p < T::NP) {
-			hl[p] = vals[it];
+	for (int it = 0; it < HaloLabels<T>::kRingPer; ++it) {
+		const int e = tid + it * T::NT;
+		if (e < HaloLabels<T>::kRing * T::PZ) {
+			const int pz = e / HaloLabels<T>::kRing;
+			int px, py;
+			HaloLabels<T>::ring_pos(e % HaloLabels<T>::kRing, px, py);
+			hl[(pz * T::PY + py) * T::PX + px] = halo.ring[it];
 		}
 	}
 }
@@ -315,12 +334,13 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_bfs(GridView gv, const int32_
 	// every global load of the visit first (halo labels, state words), then the LDS stores
 	HaloLabels<T> halo;
 	fetch_label_tile<T>(g, halo, ox, oy, oz);
+	const uint32_t xy = slice_xy(g, gx, gy), sxy = (uint32_t)(g.nx * g.ny);
 	uint32_t valid = 0;
 	uint32_t state[T::TZ];
 #pragma unroll
 	for (int k = 0; k < T::TZ; ++k) {
 		valid |= (gx < g.nx && gy < g.ny && oz + k < g.nz) ? (1u << k) : 0u;
-		state[k] = (threshold == 0 && (valid & (1u << k))) ? g.omask[voxel_index(g, gx, gy, oz + k)] : 0u;
+		state[k] = (threshold == 0 && (valid & (1u << k))) ? tile_at(g.omask, g, oz, k, xy, sxy) : 0u;
 	}
 	store_label_tile<T>(hl, halo);
 	__syncthreads();
@@ -369,7 +389,7 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_bfs(GridView gv, const int32_
 		const int32_t nl = hl[p];
 		active |= has_excess[k] && nl < kLabelInf;
 		if (nl != before[k]) {
-			g.h[voxel_index(g, gx, gy, oz + k)] = nl;
+			tile_at(g.h, g, oz, k, xy, sxy) = nl;
 #pragma unroll
 			for (int d = 0; d < T::K; ++d) {
 				// a voxel beyond the tile pulls this label; it must be relaxed again if it could improve
@@ -421,6 +441,9 @@ struct DirtyBits {
 	__device__ static void set(uint32_t *dm, int i, int d)
 	{
 		if (kConn <= 8) {
+			// the empty asm keeps the optimiser from hoisting the word address and bit of every
+			// (voxel, direction) pair out of the push loop, where they would not fit in registers
+			asm volatile("" : "+v"(i));
 			atomicOr(&dm[i >> 2], 1u << (8 * (i & 3) + d));
 		} else {
 			atomicOr(&dm[i], 1u << d);
@@ -474,34 +497,60 @@ __global__ __launch_bounds__(kTX *kTY, 6) void k_grid_discharge(GridView gv, con
 	for (int w = tid; w < Dirty::kWords(T::NV); w += T::NT) {
 		dm[w] = 0;
 	}
-	load_label_tile<T>(g, hl, ox, oy, oz);
-
+	// every global load of a group first, then its LDS stores: the halo labels with the excess,
+	// then the residual planes in groups of kGroup directions
+	const uint32_t xy = slice_xy(g, gx, gy), sxy = (uint32_t)(g.nx * g.ny);
 	uint32_t valid = 0;  // bit k: voxel k of the column lies inside the volume
 	int32_t rtr[T::TZ];
+	{
+		HaloLabels<T> halo;
+		fetch_label_tile<T>(g, halo, ox, oy, oz);
+		int32_t ev[T::TZ];
+#pragma unroll
+		for (int k = 0; k < T::TZ; ++k) {
+			valid |= (gx < g.nx && gy < g.ny && oz + k < g.nz) ? (1u << k) : 0u;
+			ev[k] = 0;
+			rtr[k] = 0;
+			if (valid & (1u << k)) {
+				ev[k] = tile_at(g.e, g, oz, k, xy, sxy);
+				rtr[k] = tile_at(g.rt, g, oz, k, xy, sxy);
+			}
+		}
+		store_label_tile<T>(hl, halo);
+#pragma unroll
+		for (int k = 0; k < T::TZ; ++k) {
+			el[T::vidx(lx, ly, k)] = ev[k];
+		}
+	}
+	constexpr int kGroup = 6;
+#pragma unroll
+	for (int d0 = 0; d0 < K; d0 += kGroup) {
+		int32_t rv[kGroup][T::TZ];
+#pragma unroll
+		for (int j = 0; j < kGroup; ++j) {
+#pragma unroll
+			for (int k = 0; k < T::TZ; ++k) {
+				rv[j][k] = 0;
+				if (d0 + j < K && (valid & (1u << k))) {
+					rv[j][k] = tile_at(g.r + (int64_t)(d0 + j) * g.nvox, g, oz, k, xy, sxy);
+				}
+			}
+		}
+#pragma unroll
+		for (int j = 0; j < kGroup; ++j) {
+#pragma unroll
+			for (int k = 0; k < T::TZ; ++k) {
+				if (d0 + j < K) {
+					rl[d0 + j][T::vidx(lx, ly, k)] = (RT)rv[j][k];
+				}
+			}
+		}
+	}
 	// 6-connected: flow pushed across the tile faces during the visit. The z-column of thread
 	// (lx, ly) touches at most one x face and one y face, and the two z faces at its ends.
 	int32_t out_x[T::TZ], out_y[T::TZ], out_zlo = 0, out_zhi = 0;
 #pragma unroll
 	for (int k = 0; k < T::TZ; ++k) {
-		valid |= (gx < g.nx && gy < g.ny && oz + k < g.nz) ? (1u << k) : 0u;
-		const int i = T::vidx(lx, ly, k);
-		int32_t ev = 0, rv = 0;
-		if (valid & (1u << k)) {
-			const int64_t v = voxel_index(g, gx, gy, oz + k);
-			ev = g.e[v];
-			rv = g.rt[v];
-#pragma unroll
-			for (int d = 0; d < K; ++d) {
-				rl[d][i] = (RT)g.r[(int64_t)d * g.nvox + v];
-			}
-		} else {
-#pragma unroll
-			for (int d = 0; d < K; ++d) {
-				rl[d][i] = 0;
-			}
-		}
-		el[i] = ev;
-		rtr[k] = rv;
 		out_x[k] = 0;
 		out_y[k] = 0;
 	}
@@ -650,44 +699,45 @@ __global__ __launch_bounds__(kTX *kTY, 6) void k_grid_discharge(GridView gv, con
 		}
 		const int p = T::pidx(lx, ly, k);
 		const int i = T::vidx(lx, ly, k);
-		const int64_t v = voxel_index(g, gx, gy, oz + k);
 		const int32_t ev = el[i], lab = hl[p];
 		active |= ev > 0 && lab < kLabelInf;
-		g.h[v] = lab;  // (a store is cheaper than keeping the loaded labels in registers)
+		tile_at(g.h, g, oz, k, xy, sxy) = lab;  // (a store is cheaper than keeping the loaded labels in registers)
 		// every change of e, rt or r[d] set a bit of the voxel's dirty byte / word
 		const uint32_t dirty = Dirty::get(dm, i);
 		if (!dirty) {
 			continue;
 		}
-		g.e[v] = ev;
-		g.rt[v] = rtr[k];
+		tile_at(g.e, g, oz, k, xy, sxy) = ev;
+		tile_at(g.rt, g, oz, k, xy, sxy) = rtr[k];
 		uint32_t st = (ev > 0 ? kStateExcess : 0u) | (rtr[k] > 0 ? kStateSink : 0u);
 #pragma unroll
 		for (int d = 0; d < K; ++d) {
 			st |= rl[d][i] > 0 ? (1u << d) : 0u;
 		}
-		g.omask[v] = st;
+		tile_at(g.omask, g, oz, k, xy, sxy) = st;
 #pragma unroll
 		for (int d = 0; d < K; ++d) {
 			if (!(dirty & (1u << d))) {
 				continue;
 			}
 			const int32_t nr = rl[d][i];
+			int32_t *plane = g.r + (int64_t)d * g.nvox;
 			if (kConn != 6) {
 				const int slot = leaving_slot<T>(lx, ly, k, d);
 				if (slot >= 0) {
 					// only this voxel pushes along this edge during the phase: the drop is the flow sent
-					const int32_t f = g.r[(int64_t)d * g.nvox + v] - nr;
+					const int32_t f = tile_at(plane, g, oz, k, xy, sxy) - nr;
 					if (f > 0) {
-						const int64_t u = voxel_index(g, gx + D::dx(d), gy + D::dy(d), oz + k + D::dz(d));
-						atomicAdd(&g.e[u], f);
-						atomicAdd(&g.r[(int64_t)opposite<kConn>(d) * g.nvox + u], f);
-						atomicOr(&g.omask[u], kStateExcess | (1u << opposite<kConn>(d)));
+						const int uk = k + D::dz(d);
+						const uint32_t uxy = slice_xy(g, gx + D::dx(d), gy + D::dy(d));
+						atomicAdd(&tile_at(g.e, g, oz, uk, uxy, sxy), f);
+						atomicAdd(&tile_at(g.r + (int64_t)opposite<kConn>(d) * g.nvox, g, oz, uk, uxy, sxy), f);
+						atomicOr(&tile_at(g.omask, g, oz, uk, uxy, sxy), kStateExcess | (1u << opposite<kConn>(d)));
 						marks[slot] = 1;
 					}
 				}
 			}
-			g.r[(int64_t)d * g.nvox + v] = nr;
+			tile_at(plane, g, oz, k, xy, sxy) = nr;
 		}
 	}
 	if (kConn == 6) {
@@ -696,10 +746,11 @@ __global__ __launch_bounds__(kTX *kTY, 6) void k_grid_discharge(GridView gv, con
 			if (f <= 0) {
 				return;
 			}
-			const int64_t u = voxel_index(g, gx + D::dx(d), gy + D::dy(d), oz + k + D::dz(d));
-			atomicAdd(&g.e[u], f);
-			atomicAdd(&g.r[(int64_t)opposite<kConn>(d) * g.nvox + u], f);
-			atomicOr(&g.omask[u], kStateExcess | (1u << opposite<kConn>(d)));
+			const int uk = k + D::dz(d);
+			const uint32_t uxy = slice_xy(g, gx + D::dx(d), gy + D::dy(d));
+			atomicAdd(&tile_at(g.e, g, oz, uk, uxy, sxy), f);
+			atomicAdd(&tile_at(g.r + (int64_t)opposite<kConn>(d) * g.nvox, g, oz, uk, uxy, sxy), f);
+			atomicOr(&tile_at(g.omask, g, oz, uk, uxy, sxy), kStateExcess | (1u << opposite<kConn>(d)));
 			marks[leaving_slot<T>(lx, ly, k, d)] = 1;
 		};
 #pragma unroll
@@ -754,26 +805,26 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_source_closure(GridView gv, c
 	if (tid < 27) {
 		marks[tid] = 0;
 	}
+	const uint32_t xy = slice_xy(g, gx, gy), sxy = (uint32_t)(g.nx * g.ny);
 	bool valid[T::TZ];
 	uint32_t open[T::TZ];
 	uint8_t before[T::TZ];
+	// all loads first, then the LDS stores
 #pragma unroll
 	for (int k = 0; k < T::TZ; ++k) {
 		valid[k] = gx < g.nx && gy < g.ny && oz + k < g.nz;
-		const int i = T::vidx(lx, ly, k);
-		uint8_t sv = 0;
 		before[k] = 0;
-		uint32_t o = 0;
+		open[k] = 0;
 		if (valid[k]) {
-			const int64_t v = voxel_index(g, gx, gy, oz + k);
-			const uint8_t stored = g.s[v];
-			before[k] = stored;
-			const uint32_t st = g.omask[v];
-			o = st & ((1u << T::K) - 1u);
-			sv = ((first_pass && (st & kStateExcess)) || stored) ? 1 : 0;
+			before[k] = tile_at(g.s, g, oz, k, xy, sxy);
+			open[k] = tile_at(g.omask, g, oz, k, xy, sxy);
 		}
-		open[k] = o;
-		sl[i] = sv;
+	}
+#pragma unroll
+	for (int k = 0; k < T::TZ; ++k) {
+		const uint32_t st = open[k];
+		sl[T::vidx(lx, ly, k)] = ((first_pass && (st & kStateExcess)) || before[k]) ? 1 : 0;
+		open[k] = st & ((1u << T::K) - 1u);
 	}
 	__syncthreads();
 	for (;;) {
@@ -810,9 +861,8 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_source_closure(GridView gv, c
 		if (!valid[k]) {
 			continue;
 		}
-		const int64_t v = voxel_index(g, gx, gy, oz + k);
 		if (sl[i] != before[k]) {
-			g.s[v] = sl[i];
+			tile_at(g.s, g, oz, k, xy, sxy) = sl[i];
 		}
 		if (!sl[i]) {
 			continue;
@@ -826,9 +876,9 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_source_closure(GridView gv, c
 			if (slot < 0) {
 				continue;
 			}
-			const int64_t u = voxel_index(g, gx + D::dx(d), gy + D::dy(d), oz + k + D::dz(d));
-			if (!g.s[u]) {
-				g.s[u] = 1;
+			uint8_t &su = tile_at(g.s, g, oz, k + D::dz(d), slice_xy(g, gx + D::dx(d), gy + D::dy(d)), sxy);
+			if (!su) {
+				su = 1;
 				marks[slot] = 1;
 			}
 		}

@@ -202,6 +202,10 @@ std::vector<InstanceDesc> GridSolver::describe(int conn, const GridSpec *specs, 
 		if (vox >= (int64_t)1 << 31) {
 			fail(CGP_ERR_INVALID, which + "grids of 2^31 voxels or more are not supported");
 		}
+		if ((int64_t)sp.nx * sp.ny >= (int64_t)1 << 26) {
+			// kernels address the voxels of a tile with 32-bit byte offsets from its first z-slice
+			fail(CGP_ERR_INVALID, which + "z-slices of 2^26 voxels or more are not supported");
+		}
 		InstanceDesc &d = inst[(size_t)i];
 		std::memset(&d, 0, sizeof(d));
 		d.nx = sp.nx;

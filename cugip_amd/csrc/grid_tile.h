@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 namespace cgp {
 
@@ -133,6 +134,40 @@ __device__ inline GridView instance_view(const GridView &g, int tile, int &local
 __device__ inline int64_t voxel_index(const GridView &g, int x, int y, int z)
 {
 	return ((int64_t)z * g.ny + y) * g.nx + x;
+}
+
+// Element offset of z-slice z in a per-voxel array. Within a workgroup z is uniform wherever this is
+// used, so the slice start lives in scalar registers.
+__device__ inline int64_t slice_start(const GridView &g, int z)
+{
+	return (int64_t)z * ((int64_t)g.nx * g.ny);
+}
+
+// In-slice index of (x, y).
+__device__ inline uint32_t slice_xy(const GridView &g, int x, int y)
+{
+	return (uint32_t)(y * g.nx + x);
+}
+
+// Element `off` after `base`, addressed as a scalar base + a 32-bit byte offset per lane instead of
+// 64-bit index arithmetic per lane. Callers keep off * sizeof(P) below 2^32: the grid setup limits
+// a z-slice to fewer than 2^26 voxels and the offsets below span at most TZ + 1 slices.
+template <typename P>
+__device__ inline P &at(P *base, uint32_t off)
+{
+	using B = typename std::conditional<std::is_const<P>::value, const char, char>::type;
+	return *reinterpret_cast<P *>(reinterpret_cast<B *>(base) + off * (uint32_t)sizeof(P));
+}
+
+// Voxel xy of z-slice oz + k of a per-voxel array (k a compile-time constant in [-1, TZ] after
+// unrolling): for k >= 0 the tile's first slice is the base, the slice below the tile has its own.
+template <typename P>
+__device__ inline P &tile_at(P *array, const GridView &g, int oz, int k, uint32_t xy, uint32_t sxy)
+{
+	if (k < 0) {
+		return at(array + slice_start(g, oz + k), xy);
+	}
+	return at(array + slice_start(g, oz), xy + (uint32_t)k * sxy);
 }
 
 __device__ inline bool inside(const GridView &g, int x, int y, int z)
