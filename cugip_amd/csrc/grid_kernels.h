@@ -99,7 +99,7 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_init(GridView gv, int ntiles)
 		}
 		const long long total = block_sum_i64<T::NT>(cancelled, red);
 		if (tid == 0 && total) {
-			atomicAdd(g.flow, (unsigned long long)total);
+			atomicAdd(flow_slot(g, tile), (unsigned long long)total);
 		}
 		__syncthreads();  // red is reused by the next tile
 	}
@@ -406,7 +406,7 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_bfs(GridView gv, const int32_
 	}
 	if (tid == 0) {
 		tile_active[tile] = active ? 1 : 0;
-		atomicAdd(&g.counters[1], 1ull);
+		atomicAdd(counter_slot(g, 1, tile), 1ull);
 	}
 }
 
@@ -765,10 +765,10 @@ __global__ __launch_bounds__(kTX *kTY, 6) void k_grid_discharge(GridView gv, con
 	active = __syncthreads_or(active);
 	if (tid == 0) {
 		if (total) {
-			atomicAdd(g.flow, (unsigned long long)total);
+			atomicAdd(flow_slot(g, tile), (unsigned long long)total);
 		}
 		tile_active[tile] = active ? 1 : 0;
-		atomicAdd(&g.counters[0], 1ull);
+		atomicAdd(counter_slot(g, 0, tile), 1ull);
 	}
 	if (tid < 27 && marks[tid]) {
 		mark_neighbour_tile(g, tile_active, tx, ty, tz, tid);
@@ -888,8 +888,25 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_source_closure(GridView gv, c
 		mark_neighbour_tile(g, dirty_next, tx, ty, tz, tid);
 	}
 	if (tid == 0) {
-		atomicAdd(&g.counters[2], 1ull);
+		atomicAdd(counter_slot(g, 2, tile), 1ull);
 	}
+}
+
+// ---------------------------------------------------------------------------------------------
+// total[i] = sum of the kSlots slots of entry i (an instance's flow, or one visit counter).
+__global__ __launch_bounds__(256) void k_sum_slots(const unsigned long long *__restrict__ slots, int64_t n,
+                                                   unsigned long long *__restrict__ total)
+{
+	const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+	if (i >= n) {
+		return;
+	}
+	unsigned long long sum = 0;
+#pragma unroll
+	for (int j = 0; j < kSlots; ++j) {
+		sum += slots[i * kSlotWords + j * kSlotStride];
+	}
+	total[i] = sum;
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -92,15 +92,11 @@ void GridSolver::launch(Kern kind, int64_t voxels, double bytes_per_voxel, F &&f
 		CGP_HIP(hipEventCreate(&mEvStart));
 		CGP_HIP(hipEventCreate(&mEvStop));
 	}
-	unsigned long long *pinned = reinterpret_cast<unsigned long long *>(mHostScratch.get());
 	unsigned long long before = 0;
 	// tile-visit counter of the kernel kind: [0] discharge, [1] relabel, [2] closure
 	const int counter = kind == kBfs ? 1 : (kind == kClosure ? 2 : 0);
 	if (voxels < 0) {
-		CGP_HIP(hipMemcpyAsync(pinned, mCounters.get() + counter, sizeof(unsigned long long), hipMemcpyDeviceToHost,
-		                       mStream));
-		CGP_HIP(hipStreamSynchronize(mStream));
-		before = pinned[0];
+		before = read_counter(counter);
 	}
 	CGP_HIP(hipEventRecord(mEvStart, mStream));
 	fn();
@@ -110,10 +106,7 @@ void GridSolver::launch(Kern kind, int64_t voxels, double bytes_per_voxel, F &&f
 	float ms = 0.0f;
 	CGP_HIP(hipEventElapsedTime(&ms, mEvStart, mEvStop));
 	if (voxels < 0) {
-		CGP_HIP(hipMemcpyAsync(pinned, mCounters.get() + counter, sizeof(unsigned long long), hipMemcpyDeviceToHost,
-		                       mStream));
-		CGP_HIP(hipStreamSynchronize(mStream));
-		voxels = (int64_t)(pinned[0] - before) * mShape.tx * mShape.ty * mShape.tz;
+		voxels = (int64_t)(read_counter(counter) - before) * mShape.tx * mShape.ty * mShape.tz;
 	}
 	KernelRecord &rec = mProf[kind];
 	rec.launches += 1;
@@ -252,8 +245,10 @@ void GridSolver::allocate_grids(int conn, const std::vector<InstanceDesc> &inst,
 	mR.allocate((size_t)total * conn);
 	mS.allocate(total);
 	mOmask.allocate(total);
-	mFlow.allocate(count);
-	mCounters.allocate(4);
+	mFlow.allocate(count * kSlotWords);
+	mFlowTotal.allocate(count);
+	mCounters.allocate(3 * kSlotWords);
+	mCounterTotal.allocate(1);
 	mStatus.allocate(1);
 	mCount.allocate(3);
 	mHostScratch.allocate(count > 8 ? count : 8);
@@ -789,11 +784,26 @@ void GridSolver::run_source_closure(cgp_stats &stats)
 	}
 }
 
+// Sum of the slots of visit counter `kind` (0 discharge, 1 relabel, 2 closure).
+unsigned long long GridSolver::read_counter(int kind)
+{
+	unsigned long long *pinned = reinterpret_cast<unsigned long long *>(mHostScratch.get());
+	hipLaunchKernelGGL(k_sum_slots, dim3(1), dim3(64), 0, mStream, mCounters.get() + (int64_t)kind * kSlotWords,
+	                   (int64_t)1, mCounterTotal.get());
+	CGP_LAUNCH_CHECK();
+	CGP_HIP(hipMemcpyAsync(pinned, mCounterTotal.get(), sizeof(unsigned long long), hipMemcpyDeviceToHost, mStream));
+	CGP_HIP(hipStreamSynchronize(mStream));
+	return pinned[0];
+}
+
 int64_t GridSolver::read_flow()
 {
 	const size_t n = mInst.size();
 	unsigned long long *pinned = reinterpret_cast<unsigned long long *>(mHostScratch.get());
-	CGP_HIP(hipMemcpyAsync(pinned, mFlow.get(), n * sizeof(unsigned long long), hipMemcpyDeviceToHost, mStream));
+	hipLaunchKernelGGL(k_sum_slots, dim3(blocks_for((int64_t)n, 256)), dim3(256), 0, mStream, mFlow.get(), (int64_t)n,
+	                   mFlowTotal.get());
+	CGP_LAUNCH_CHECK();
+	CGP_HIP(hipMemcpyAsync(pinned, mFlowTotal.get(), n * sizeof(unsigned long long), hipMemcpyDeviceToHost, mStream));
 	CGP_HIP(hipStreamSynchronize(mStream));
 	int64_t total = 0;
 	for (size_t i = 0; i < n; ++i) {
@@ -839,10 +849,13 @@ int64_t GridSolver::solve(const cgp_policy &policy, cgp_stats *stats_out, const 
 	}
 
 	hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, mStream, mStatus.get(), (int64_t)1, 0);
-	hipLaunchKernelGGL(k_fill_u8, dim3(1), dim3(256), 0, mStream, reinterpret_cast<uint8_t *>(mFlow.get()),
-	                   (int64_t)(8 * mInst.size()), (uint8_t)0);
-	hipLaunchKernelGGL(k_fill_u8, dim3(1), dim3(64), 0, mStream, reinterpret_cast<uint8_t *>(mCounters.get()),
-	                   (int64_t)32, (uint8_t)0);
+	{
+		const int64_t flow_bytes = 8 * (int64_t)mInst.size() * kSlotWords;
+		hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(flow_bytes, 256)), dim3(256), 0, mStream,
+		                   reinterpret_cast<uint8_t *>(mFlow.get()), flow_bytes, (uint8_t)0);
+		hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(8 * 3 * kSlotWords, 256)), dim3(256), 0, mStream,
+		                   reinterpret_cast<uint8_t *>(mCounters.get()), (int64_t)(8 * 3 * kSlotWords), (uint8_t)0);
+	}
 	launch_init();
 	mStarted = true;
 	{
@@ -903,14 +916,9 @@ int64_t GridSolver::solve(const cgp_policy &policy, cgp_stats *stats_out, const 
 	run_source_closure(stats);
 	const int64_t flow = read_flow();
 	stats.cut_ms = ms_since(t0);
-	{
-		unsigned long long *pinned = reinterpret_cast<unsigned long long *>(mHostScratch.get());
-		CGP_HIP(hipMemcpyAsync(pinned, mCounters.get(), 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, mStream));
-		CGP_HIP(hipStreamSynchronize(mStream));
-		stats.discharge_tile_visits = (int64_t)pinned[0];
-		stats.bfs_tile_visits = (int64_t)pinned[1];
-		stats.cut_tile_visits = (int64_t)pinned[2];
-	}
+	stats.discharge_tile_visits = (int64_t)read_counter(0);
+	stats.bfs_tile_visits = (int64_t)read_counter(1);
+	stats.cut_tile_visits = (int64_t)read_counter(2);
 	stats.flow = flow;
 	stats.total_ms = ms_since(t_start);
 	stats.voxels = mVox;

@@ -87,15 +87,33 @@ struct GridView {
 	uint32_t *omask;         // state word: open directions (r > 0), residual sink link > 0, excess > 0
 	const int32_t *cs, *ct;  // input t-link capacities
 	const int32_t *cn, *cnb; // input n-link capacities, H planes (cnb may be null)
-	unsigned long long *flow;
+	unsigned long long *flow;  // kSlots slots of this instance's flow into t
 	int32_t *status;         // nonzero: input error detected on the device
-	unsigned long long *counters; // [0] discharge visits, [1] bfs visits, [2] cut visits
+	unsigned long long *counters; // visit counters, kSlots slots per kind (see counter_slot)
 	int32_t *faces;          // wave engine: labels of each tile's face voxels
 	uint32_t *reach;         // wave engine: "reaches t" bit plane of the last global relabel
 	const InstanceDesc *inst;  // batch: instance table (null for a single grid)
 	const int32_t *tile_inst;  // batch: instance of every global tile
 	int32_t tbase;             // global id of this view's first tile
 };
+
+// The flow of an instance and the visit counters are sums over kSlots slots, each on its own
+// 128-byte line, picked by tile id: thousands of workgroups adding into one address serialise in
+// the L2 and hold up every other request queued behind them.
+constexpr int kSlots = 32;
+constexpr int kSlotStride = 16;  // unsigned long longs per slot (128 bytes)
+constexpr int kSlotWords = kSlots * kSlotStride;  // per instance (flow) or per counter kind
+
+__device__ inline unsigned long long *flow_slot(const GridView &g, int tile)
+{
+	return g.flow + (tile & (kSlots - 1)) * kSlotStride;
+}
+
+// kind: 0 discharge visits, 1 relabel visits, 2 closure visits
+__device__ inline unsigned long long *counter_slot(const GridView &g, int kind, int tile)
+{
+	return g.counters + kind * kSlotWords + (tile & (kSlots - 1)) * kSlotStride;
+}
 
 // The view of the instance that owns global tile `tile`, and the tile's id inside it. For a single
 // grid this is the view itself.
@@ -126,7 +144,7 @@ __device__ inline GridView instance_view(const GridView &g, int tile, int &local
 	v.r = g.r + d.vbase * d.conn;
 	v.cn = g.cn + d.vbase * (d.conn / 2);
 	v.cnb = g.cnb ? g.cnb + d.vbase * (d.conn / 2) : nullptr;
-	v.flow = g.flow + i;
+	v.flow = g.flow + (int64_t)i * kSlotWords;
 	local = tile - d.tbase;
 	return v;
 }

@@ -504,7 +504,7 @@ __global__ __launch_bounds__(64) void k_wave_init(GridView g, uint32_t *__restri
 	}
 	const long long total = wave_sum_i64(cancelled);
 	if (lane == 0 && total) {
-		atomicAdd(g.flow, (unsigned long long)total);
+		atomicAdd(flow_slot(g, tile), (unsigned long long)total);
 	}
 }
 
@@ -634,7 +634,7 @@ __global__ __launch_bounds__(64) void k_wave_bfs(GridView g, const uint32_t *__r
 	}
 	if (lane == 0) {
 		tile_active[tile] = active ? 1 : 0;
-		atomicAdd(&g.counters[1], 1ull);
+		atomicAdd(counter_slot(g, 1, tile), 1ull);
 	}
 }
 
@@ -904,10 +904,10 @@ __global__ __launch_bounds__(64) void k_wave_discharge(GridView g, uint32_t *__r
 	}
 	if (lane == 0) {
 		if (total) {
-			atomicAdd(g.flow, (unsigned long long)total);
+			atomicAdd(flow_slot(g, tile), (unsigned long long)total);
 		}
 		tile_active[tile] = active ? 1 : 0;
-		atomicAdd(&g.counters[0], 1ull);
+		atomicAdd(counter_slot(g, 0, tile), 1ull);
 	}
 }
 
@@ -1003,7 +1003,7 @@ __global__ __launch_bounds__(64) void k_wave_closure(GridView g, const uint32_t 
 		mark_neighbour_tile(g, dirty_next, tx, ty, tz, lane);
 	}
 	if (lane == 0) {
-		atomicAdd(&g.counters[2], 1ull);
+		atomicAdd(counter_slot(g, 2, tile), 1ull);
 	}
 }
 

@@ -103,6 +103,7 @@ private:
 	void run_discharge_phase(int colour, int max_iters);
 	void run_source_closure(cgp_stats &stats);
 	int64_t read_flow();
+	unsigned long long read_counter(int kind);
 	const InstanceDesc &instance(int index) const;
 	std::vector<InstanceDesc> describe(int conn, const GridSpec *specs, int count);
 	void allocate_grids(int conn, const std::vector<InstanceDesc> &inst, bool has_back);
@@ -135,7 +136,8 @@ private:
 	DeviceBuffer<int32_t> mE, mRt, mH, mR;
 	DeviceBuffer<uint8_t> mS;
 	DeviceBuffer<uint32_t> mOmask;
-	DeviceBuffer<unsigned long long> mFlow, mCounters;
+	DeviceBuffer<unsigned long long> mFlow, mCounters;  // slotted (kSlots per instance / per counter kind)
+	DeviceBuffer<unsigned long long> mFlowTotal, mCounterTotal;
 	DeviceBuffer<int32_t> mStatus, mCount, mList, mAllTiles;
 	DeviceBuffer<uint8_t> mActive, mDirtyA, mDirtyB;
 	DeviceBuffer<uint32_t> mBits, mSBits, mReach;  // wave engine: bit planes, S_min and reach planes
