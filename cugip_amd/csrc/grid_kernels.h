@@ -938,6 +938,111 @@ __global__ __launch_bounds__(256) void k_compact_tiles(uint8_t *__restrict__ fla
 	}
 }
 
+// The same compaction by one workgroup, for up to kCompactBlockTiles tiles: thread t takes the 32
+// tiles [32t, 32t + 32) with two 16-byte loads of each array (the flag arrays are padded to whole
+// 32-byte chunks), and a workgroup scan places the chunks in order, so the list comes out sorted
+// and the count is written rather than accumulated (no fill beforehand). With flags2 set, a second
+// flag array (no colour filter, no list) is counted into count2 in the same pass.
+constexpr int kCompactThreads = 1024;
+constexpr int kCompactBlockTiles = 32 * kCompactThreads;
+
+__device__ inline int block_exclusive_scan_1024(int v, int *wave_sums, int &total)
+{
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	int inc = v;
+#pragma unroll
+	for (int o = 1; o < 64; o <<= 1) {
+		const int t = __shfl_up(inc, (unsigned)o, 64);
+		if (lane >= o) {
+			inc += t;
+		}
+	}
+	if (lane == 63) {
+		wave_sums[wave] = inc;
+	}
+	__syncthreads();
+	int before = 0;
+	total = 0;
+#pragma unroll
+	for (int w = 0; w < kCompactThreads / 64; ++w) {
+		const int s = wave_sums[w];
+		before += w < wave ? s : 0;
+		total += s;
+	}
+	return before + inc - v;
+}
+
+// bit j set where byte j of the 16 bytes q is nonzero (colour < 0) or equals colour
+__device__ inline uint32_t byte_mask16(uint4 q, int colour, bool nonzero)
+{
+	const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+	uint32_t m = 0;
+#pragma unroll
+	for (int j = 0; j < 16; ++j) {
+		const uint32_t b = (w[j >> 2] >> (8 * (j & 3))) & 0xffu;
+		m |= ((nonzero ? b != 0u : b == (uint32_t)colour) ? 1u : 0u) << j;
+	}
+	return m;
+}
+
+__global__ __launch_bounds__(kCompactThreads) void k_compact_block(uint8_t *__restrict__ flags, int ntiles,
+                                                                   const uint8_t *__restrict__ colours, int colour,
+                                                                   int clear, int32_t *__restrict__ list,
+                                                                   int32_t *__restrict__ count,
+                                                                   const uint8_t *__restrict__ flags2,
+                                                                   int32_t *__restrict__ count2)
+{
+	__shared__ int sums[2][kCompactThreads / 64];
+	const int lo = (int)threadIdx.x * 32;
+	uint32_t sel = 0, sel2 = 0;
+	if (lo < ntiles) {
+		const uint4 *f = reinterpret_cast<const uint4 *>(flags + lo);
+		const uint4 *c = reinterpret_cast<const uint4 *>(colours + lo);
+		const uint4 f0 = f[0], f1 = f[1];
+		uint4 c0 = make_uint4(0, 0, 0, 0), c1 = c0, g0 = c0, g1 = c0;
+		if (colour >= 0) {
+			c0 = c[0];
+			c1 = c[1];
+		}
+		if (flags2) {
+			const uint4 *f2 = reinterpret_cast<const uint4 *>(flags2 + lo);
+			g0 = f2[0];
+			g1 = f2[1];
+		}
+		sel = byte_mask16(f0, 0, true) | (byte_mask16(f1, 0, true) << 16);
+		if (colour >= 0) {
+			sel &= byte_mask16(c0, colour, false) | (byte_mask16(c1, colour, false) << 16);
+		}
+		sel2 = byte_mask16(g0, 0, true) | (byte_mask16(g1, 0, true) << 16);
+		const int n = ntiles - lo;
+		if (n < 32) {
+			sel &= (1u << n) - 1u;
+			sel2 &= (1u << n) - 1u;
+		}
+	}
+	int total = 0;
+	int pos = block_exclusive_scan_1024(__popc(sel), sums[0], total);
+	if (threadIdx.x == 0) {
+		*count = total;
+	}
+	if (flags2) {
+		int total2 = 0;
+		block_exclusive_scan_1024(__popc(sel2), sums[1], total2);
+		if (threadIdx.x == 0) {
+			*count2 = total2;
+		}
+	}
+	for (uint32_t m = sel; m; m &= m - 1) {
+		const int i = lo + __ffs(m) - 1;
+		if (list) {
+			list[pos++] = i;
+		}
+		if (clear) {
+			flags[i] = 0;
+		}
+	}
+}
+
 // Image -> graph front end: t-links by intensity, then marker and border forcing; n-links by the
 // absolute intensity difference per neighbour class (the caller-side construction of
 // benchmarks/graph_cut/graph_cut.cpp:32-98, as table lookups).

@@ -432,12 +432,14 @@ void GridSolver::configure_tiles(const Shape &shape)
 	}
 	// largest colour class: the grid of a discharge phase must cover every tile of one colour
 	mMaxColourTiles = *std::max_element(per_colour.begin(), per_colour.end());
-	mActive.allocate(mTiles);
-	mDirtyA.allocate(mTiles);
-	mDirtyB.allocate(mTiles);
+	// flag arrays padded to whole 32-byte chunks: the one-workgroup compaction reads them 16 bytes at a time
+	const int64_t padded = ((int64_t)mTiles + 31) / 32 * 32;
+	mActive.allocate(padded);
+	mDirtyA.allocate(padded);
+	mDirtyB.allocate(padded);
 	mList.allocate(mTiles);
 	mAllTiles.allocate(mTiles + 1);
-	mTileColour.allocate(mTiles);
+	mTileColour.allocate(padded);
 	std::vector<int32_t> all(mTiles + 1);
 	for (int t = 0; t < mTiles; ++t) {
 		all[t] = t;
@@ -556,6 +558,14 @@ void GridSolver::launch_init()
 
 void GridSolver::launch_compact(uint8_t *flags, int colour, bool clear)
 {
+	if (mTiles <= kCompactBlockTiles) {
+		launch(kCompact, 0, 0.0, [&] {
+			hipLaunchKernelGGL(k_compact_block, dim3(1), dim3(kCompactThreads), 0, mStream, flags, mTiles,
+			                   mTileColour.get(), colour, clear ? 1 : 0, mList.get(), mCount.get(),
+			                   (const uint8_t *)nullptr, (int32_t *)nullptr);
+		});
+		return;
+	}
 	launch(kFill, 0, 0.0, [&] {
 		hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, mStream, mCount.get(), (int64_t)1, 0);
 	});
@@ -568,6 +578,19 @@ void GridSolver::launch_compact(uint8_t *flags, int colour, bool clear)
 // Counts the set flags of two arrays with one host read.
 void GridSolver::count_two(uint8_t *a, uint8_t *b, int &na, int &nb)
 {
+	int32_t *pinned = reinterpret_cast<int32_t *>(mHostScratch.get());
+	if (mTiles <= kCompactBlockTiles) {
+		launch(kCompact, 0, 0.0, [&] {
+			hipLaunchKernelGGL(k_compact_block, dim3(1), dim3(kCompactThreads), 0, mStream, a, mTiles,
+			                   mTileColour.get(), -1, 0, (int32_t *)nullptr, mCount.get() + 1, (const uint8_t *)b,
+			                   mCount.get() + 2);
+		});
+		CGP_HIP(hipMemcpyAsync(pinned, mCount.get() + 1, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, mStream));
+		CGP_HIP(hipStreamSynchronize(mStream));
+		na = pinned[0];
+		nb = b ? pinned[1] : 0;
+		return;
+	}
 	launch(kFill, 0, 0.0, [&] {
 		hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, mStream, mCount.get() + 1, (int64_t)2, 0);
 	});
@@ -580,11 +603,10 @@ void GridSolver::count_two(uint8_t *a, uint8_t *b, int &na, int &nb)
 			                   0, (int32_t *)nullptr, mCount.get() + 2);
 		}
 	});
-	int32_t *pinned = reinterpret_cast<int32_t *>(mHostScratch.get());
 	CGP_HIP(hipMemcpyAsync(pinned, mCount.get() + 1, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, mStream));
 	CGP_HIP(hipStreamSynchronize(mStream));
 	na = pinned[0];
-	nb = pinned[1];
+	nb = b ? pinned[1] : 0;
 }
 
 // Visits every tile once, colour by colour, then keeps visiting the dirty tiles colour by colour
