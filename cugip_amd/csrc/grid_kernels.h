@@ -52,50 +52,108 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_init(GridView gv, int ntiles)
 		const GridView g = instance_view(gv, tile, local);
 		int tx, ty, tz;
 		tile_coords(g, local, tx, ty, tz);
-		const int x = tx * T::TX + lx, y = ty * T::TY + ly;
+		const int x = tx * T::TX + lx, y = ty * T::TY + ly, oz = tz * T::TZ;
+		const uint32_t xy = slice_xy(g, x, y), sxy = (uint32_t)(g.nx * g.ny);
+		uint32_t valid = 0;
+		int32_t ex[T::TZ], rt[T::TZ];
+		long long incoming[T::TZ];
+		uint32_t state[T::TZ];
 		long long cancelled = 0;
 		bool negative = false, overflow = false, wide = false;
+		// loads of a group first, then its arithmetic and stores: t-links, then the n-links in
+		// groups of kGroup directions
 #pragma unroll
 		for (int k = 0; k < T::TZ; ++k) {
-			const int z = tz * T::TZ + k;
-			if (x >= g.nx || y >= g.ny || z >= g.nz) {
-				continue;
+			valid |= (x < g.nx && y < g.ny && oz + k < g.nz) ? (1u << k) : 0u;
+			ex[k] = rt[k] = 0;
+			if (valid & (1u << k)) {
+				ex[k] = tile_at(g.cs, g, oz, k, xy, sxy);
+				rt[k] = tile_at(g.ct, g, oz, k, xy, sxy);
 			}
-			const int64_t v = voxel_index(g, x, y, z);
-			const int32_t cs = g.cs[v], ct = g.ct[v];
+		}
+#pragma unroll
+		for (int k = 0; k < T::TZ; ++k) {
+			const int32_t cs = ex[k], ct = rt[k];
 			negative |= (cs < 0) | (ct < 0);
 			const int32_t m = cs < ct ? cs : ct;
 			cancelled += m;
-			long long incoming = cs - m;
-			uint32_t state = (cs - m > 0 ? kStateExcess : 0u) | (ct - m > 0 ? kStateSink : 0u);
-#pragma unroll
-			for (int d = 0; d < K; ++d) {
-				const int ux = x + D::dx(d), uy = y + D::dy(d), uz = z + D::dz(d);
-				int32_t out = 0, in = 0;
-				if (inside(g, ux, uy, uz)) {
-					if (d < H) {
-						out = g.cn[(int64_t)d * g.nvox + v];
-						in = g.cnb ? g.cnb[(int64_t)d * g.nvox + v] : out;
-					} else {
-						const int64_t u = voxel_index(g, ux, uy, uz);
-						in = g.cn[(int64_t)(d - H) * g.nvox + u];
-						out = g.cnb ? g.cnb[(int64_t)(d - H) * g.nvox + u] : in;
-					}
-					negative |= (out < 0) | (in < 0);
-					overflow |= ((long long)out + (long long)in) > 0x7fffffffLL;
-					wide |= ((long long)out + (long long)in) > 0x7fffLL;
-					incoming += in;
-				}
-				g.r[(int64_t)d * g.nvox + v] = out;
-				state |= out > 0 ? (1u << d) : 0u;
-			}
-			g.omask[v] = state;
-			overflow |= incoming > 0x7fffffffLL;
-			g.e[v] = cs - m;
-			g.rt[v] = ct - m;
+			ex[k] = cs - m;
+			rt[k] = ct - m;
+			incoming[k] = ex[k];
+			state[k] = (ex[k] > 0 ? kStateExcess : 0u) | (rt[k] > 0 ? kStateSink : 0u);
 		}
-		if (negative || overflow || wide) {
-			atomicOr(g.status, (negative ? 1 : 0) | (overflow ? 2 : 0) | (wide ? 4 : 0));
+		constexpr int kGroup = 3;
+#pragma unroll
+		for (int d0 = 0; d0 < K; d0 += kGroup) {
+			int32_t out[kGroup][T::TZ], in[kGroup][T::TZ];
+#pragma unroll
+			for (int j = 0; j < kGroup; ++j) {
+				const int d = d0 + j;
+#pragma unroll
+				for (int k = 0; k < T::TZ; ++k) {
+					out[j][k] = in[j][k] = 0;
+					if (d >= K || !(valid & (1u << k)) || !inside(g, x + D::dx(d), y + D::dy(d), oz + k + D::dz(d))) {
+						continue;
+					}
+					if (d < H) {
+						out[j][k] = tile_at(g.cn + (int64_t)d * g.nvox, g, oz, k, xy, sxy);
+						if (g.cnb) {
+							in[j][k] = tile_at(g.cnb + (int64_t)d * g.nvox, g, oz, k, xy, sxy);
+						}
+					} else {
+						const int uk = k + D::dz(d);
+						const uint32_t uxy = slice_xy(g, x + D::dx(d), y + D::dy(d));
+						in[j][k] = tile_at(g.cn + (int64_t)(d - H) * g.nvox, g, oz, uk, uxy, sxy);
+						if (g.cnb) {
+							out[j][k] = tile_at(g.cnb + (int64_t)(d - H) * g.nvox, g, oz, uk, uxy, sxy);
+						}
+					}
+				}
+			}
+#pragma unroll
+			for (int j = 0; j < kGroup; ++j) {
+				const int d = d0 + j;
+				if (d >= K) {
+					continue;
+				}
+#pragma unroll
+				for (int k = 0; k < T::TZ; ++k) {
+					if (!(valid & (1u << k))) {
+						continue;
+					}
+					int32_t o = out[j][k], i = in[j][k];
+					if (!g.cnb) {
+						// symmetric capacities: one plane holds both directions
+						if (d < H) {
+							i = o;
+						} else {
+							o = i;
+						}
+					}
+					negative |= (o < 0) | (i < 0);
+					overflow |= ((long long)o + (long long)i) > 0x7fffffffLL;
+					wide |= ((long long)o + (long long)i) > 0x7fffLL;
+					incoming[k] += i;
+					tile_at(g.r + (int64_t)d * g.nvox, g, oz, k, xy, sxy) = o;
+					state[k] |= o > 0 ? (1u << d) : 0u;
+				}
+			}
+		}
+#pragma unroll
+		for (int k = 0; k < T::TZ; ++k) {
+			if (!(valid & (1u << k))) {
+				continue;
+			}
+			overflow |= incoming[k] > 0x7fffffffLL;
+			tile_at(g.omask, g, oz, k, xy, sxy) = state[k];
+			tile_at(g.e, g, oz, k, xy, sxy) = ex[k];
+			tile_at(g.rt, g, oz, k, xy, sxy) = rt[k];
+		}
+		const int bits = (negative ? 1 : 0) | (overflow ? 2 : 0) | (wide ? 4 : 0);
+		// (a plain read first: when every tile sets a bit, one atomic per tile on one word would
+		// serialise in the L2)
+		if (bits && (*(volatile int32_t *)g.status & bits) != bits) {
+			atomicOr(g.status, bits);
 		}
 		const long long total = block_sum_i64<T::NT>(cancelled, red);
 		if (tid == 0 && total) {

@@ -550,7 +550,8 @@ void GridSolver::launch_init()
 	launch(kInit, mVox, bytes, [&] {
 		dispatch_shape(mConn, mShape.tx, mShape.ty, mShape.tz, [&](auto tag) {
 			using S = decltype(tag);
-			hipLaunchKernelGGL((k_grid_init<S::conn, S::tx, S::ty, S::tz>), dim3(std::min(mTiles, 8192)),
+			// one workgroup per tile: a grid-stride loop would chain the per-tile latencies
+			hipLaunchKernelGGL((k_grid_init<S::conn, S::tx, S::ty, S::tz>), dim3(mTiles),
 			                   dim3(S::tx * S::ty), 0, mStream, g, mTiles);
 		});
 	});
