@@ -1224,7 +1224,7 @@ __global__ __launch_bounds__(256) void k_cut_capacity(int nx, int ny, int nz, co
 	}
 	const long long total = block_sum_i64<256>(sum, red);
 	if (threadIdx.x == 0 && total) {
-		atomicAdd(out, (unsigned long long)total);
+		atomicAdd(out + (blockIdx.x & (kSlots - 1)) * kSlotStride, (unsigned long long)total);  // slotted: see kSlots
 	}
 }
 

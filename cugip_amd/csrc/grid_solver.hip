@@ -350,24 +350,29 @@ int64_t GridSolver::cut_capacity_device(int index, const uint8_t *device_side)
 	const InstanceDesc &d = instance(index);
 	const int half = mConn / 2;
 	const uint8_t *side = device_side;
-	DeviceBuffer<unsigned long long> sum(1);
-	hipLaunchKernelGGL(k_fill_u8, dim3(1), dim3(64), 0, mStream, reinterpret_cast<uint8_t *>(sum.get()), (int64_t)8,
-	                   (uint8_t)0);
+	if (!mCutSlots.get()) {
+		mCutSlots.allocate(kSlotWords + 1);  // kSlots slots, then their sum
+	}
+	unsigned long long *slots = mCutSlots.get();
+	hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(8 * kSlotWords, 256)), dim3(256), 0, mStream,
+	                   reinterpret_cast<uint8_t *>(slots), (int64_t)(8 * kSlotWords), (uint8_t)0);
 	const int32_t *cn = mCn.get() + d.vbase * half;
 	const int32_t *cnb = mHasBack ? mCnb.get() + d.vbase * half : nullptr;
 	const int blocks = blocks_for(d.nvox, 256, 8192);
 	if (mConn == 6) {
 		hipLaunchKernelGGL(k_cut_capacity<6>, dim3(blocks), dim3(256), 0, mStream, d.nx, d.ny, d.nz, side,
-		                   mCs.get() + d.vbase, mCt.get() + d.vbase, cn, cnb, sum.get());
+		                   mCs.get() + d.vbase, mCt.get() + d.vbase, cn, cnb, slots);
 	} else {
 		hipLaunchKernelGGL(k_cut_capacity<26>, dim3(blocks), dim3(256), 0, mStream, d.nx, d.ny, d.nz, side,
-		                   mCs.get() + d.vbase, mCt.get() + d.vbase, cn, cnb, sum.get());
+		                   mCs.get() + d.vbase, mCt.get() + d.vbase, cn, cnb, slots);
 	}
 	CGP_LAUNCH_CHECK();
-	unsigned long long host = 0;
-	CGP_HIP(hipMemcpyAsync(&host, sum.get(), sizeof(host), hipMemcpyDeviceToHost, mStream));
+	hipLaunchKernelGGL(k_sum_slots, dim3(1), dim3(64), 0, mStream, slots, (int64_t)1, slots + kSlotWords);
+	CGP_LAUNCH_CHECK();
+	unsigned long long *pinned = reinterpret_cast<unsigned long long *>(mHostScratch.get());
+	CGP_HIP(hipMemcpyAsync(pinned, slots + kSlotWords, sizeof(unsigned long long), hipMemcpyDeviceToHost, mStream));
 	CGP_HIP(hipStreamSynchronize(mStream));
-	return (int64_t)host;
+	return (int64_t)pinned[0];
 }
 
 void GridSolver::capacities(int index, int32_t *cs, int32_t *ct, int32_t *cn, int32_t *cnb)

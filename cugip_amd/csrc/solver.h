@@ -138,6 +138,7 @@ private:
 	DeviceBuffer<uint32_t> mOmask;
 	DeviceBuffer<unsigned long long> mFlow, mCounters;  // slotted (kSlots per instance / per counter kind)
 	DeviceBuffer<unsigned long long> mFlowTotal, mCounterTotal;
+	DeviceBuffer<unsigned long long> mCutSlots;  // cut capacity: kSlots slots and their sum
 	DeviceBuffer<int32_t> mStatus, mCount, mList, mAllTiles;
 	DeviceBuffer<uint8_t> mActive, mDirtyA, mDirtyB;
 	DeviceBuffer<uint32_t> mBits, mSBits, mReach;  // wave engine: bit planes, S_min and reach planes
