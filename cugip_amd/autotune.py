@@ -28,7 +28,7 @@ ITERATIONS = [1, 2, 4, 16]
 # not fitted to one instance.
 WORKLOADS = {
     "grid6_small": (6, False, 1 << 21, lambda: [cugip_amd.make_instance(96, 96, 96, s)[1:] for s in (11, 12, 13)]),
-    "grid6_large": (6, False, None, lambda: [cugip_amd.make_instance(256, 256, 256, s)[1:] for s in (0, 1, 2)]),
+    "grid6_large": (6, False, None, lambda: [cugip_amd.make_instance(256, 256, 256, s)[1:] for s in (1, 2, 3)]),
     "batch6": (6, True, None, lambda: [cugip_amd.make_instance(nx, ny, nz, 1, i, 6)[1:]
                                        for i, (nz, ny, nx) in enumerate(cugip_amd.batch_shapes(128, seed=1))]),
     "grid26": (26, False, None, lambda: [cugip_amd.make_instance(160, 160, 128, 3, 0, 26)[1:]]),
