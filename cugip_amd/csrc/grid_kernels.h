@@ -996,13 +996,15 @@ __global__ __launch_bounds__(256) void k_compact_tiles(uint8_t *__restrict__ fla
 	}
 }
 
-// The same compaction by one workgroup, for up to kCompactBlockTiles tiles: thread t takes the 32
-// tiles [32t, 32t + 32) with two 16-byte loads of each array (the flag arrays are padded to whole
-// 32-byte chunks), and a workgroup scan places the chunks in order, so the list comes out sorted
-// and the count is written rather than accumulated (no fill beforehand). With flags2 set, a second
-// flag array (no colour filter, no list) is counted into count2 in the same pass.
+// The same compaction by one workgroup, for up to kCompactBlockTiles tiles: thread t takes the
+// chunk [t·32m, (t+1)·32m) (m = ceil(ntiles / 32768)) in pieces of 32 tiles, each read with two
+// 16-byte loads per array (the flag arrays are padded to whole 32-byte pieces); a workgroup scan of
+// the chunk counts places the chunks in order, so the list comes out sorted and the count is
+// written rather than accumulated (no fill beforehand). With flags2 set, a second flag array (no
+// colour filter, no list) is counted into count2 in the same pass.
 constexpr int kCompactThreads = 1024;
-constexpr int kCompactBlockTiles = 32 * kCompactThreads;
+constexpr int kCompactMaxPieces = 1;  // more pieces per thread measured slower than the multi-workgroup path
+constexpr int kCompactBlockTiles = 32 * kCompactThreads * kCompactMaxPieces;
 
 __device__ inline int block_exclusive_scan_1024(int v, int *wave_sums, int &total)
 {
@@ -1030,7 +1032,7 @@ __device__ inline int block_exclusive_scan_1024(int v, int *wave_sums, int &tota
 	return before + inc - v;
 }
 
-// bit j set where byte j of the 16 bytes q is nonzero (colour < 0) or equals colour
+// bit j set where byte j of the 16 bytes q is nonzero (nonzero) or equals colour
 __device__ inline uint32_t byte_mask16(uint4 q, int colour, bool nonzero)
 {
 	const uint32_t w[4] = {q.x, q.y, q.z, q.w};
@@ -1043,6 +1045,31 @@ __device__ inline uint32_t byte_mask16(uint4 q, int colour, bool nonzero)
 	return m;
 }
 
+// selection masks of the 32 tiles from lo (lo < ntiles, a multiple of 32)
+__device__ inline void compact_piece(const uint8_t *flags, const uint8_t *colours, int colour, const uint8_t *flags2,
+                                     int lo, int ntiles, uint32_t &sel, uint32_t &sel2)
+{
+	const uint4 *f = reinterpret_cast<const uint4 *>(flags + lo);
+	const uint4 f0 = f[0], f1 = f[1];
+	sel = byte_mask16(f0, 0, true) | (byte_mask16(f1, 0, true) << 16);
+	if (colour >= 0) {
+		const uint4 *c = reinterpret_cast<const uint4 *>(colours + lo);
+		const uint4 c0 = c[0], c1 = c[1];
+		sel &= byte_mask16(c0, colour, false) | (byte_mask16(c1, colour, false) << 16);
+	}
+	sel2 = 0;
+	if (flags2) {
+		const uint4 *f2 = reinterpret_cast<const uint4 *>(flags2 + lo);
+		const uint4 g0 = f2[0], g1 = f2[1];
+		sel2 = byte_mask16(g0, 0, true) | (byte_mask16(g1, 0, true) << 16);
+	}
+	const int n = ntiles - lo;
+	if (n < 32) {
+		sel &= (1u << n) - 1u;
+		sel2 &= (1u << n) - 1u;
+	}
+}
+
 __global__ __launch_bounds__(kCompactThreads) void k_compact_block(uint8_t *__restrict__ flags, int ntiles,
                                                                    const uint8_t *__restrict__ colours, int colour,
                                                                    int clear, int32_t *__restrict__ list,
@@ -1051,52 +1078,43 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact_block(uint8_t *__re
                                                                    int32_t *__restrict__ count2)
 {
 	__shared__ int sums[2][kCompactThreads / 64];
-	const int lo = (int)threadIdx.x * 32;
-	uint32_t sel = 0, sel2 = 0;
-	if (lo < ntiles) {
-		const uint4 *f = reinterpret_cast<const uint4 *>(flags + lo);
-		const uint4 *c = reinterpret_cast<const uint4 *>(colours + lo);
-		const uint4 f0 = f[0], f1 = f[1];
-		uint4 c0 = make_uint4(0, 0, 0, 0), c1 = c0, g0 = c0, g1 = c0;
-		if (colour >= 0) {
-			c0 = c[0];
-			c1 = c[1];
-		}
-		if (flags2) {
-			const uint4 *f2 = reinterpret_cast<const uint4 *>(flags2 + lo);
-			g0 = f2[0];
-			g1 = f2[1];
-		}
-		sel = byte_mask16(f0, 0, true) | (byte_mask16(f1, 0, true) << 16);
-		if (colour >= 0) {
-			sel &= byte_mask16(c0, colour, false) | (byte_mask16(c1, colour, false) << 16);
-		}
-		sel2 = byte_mask16(g0, 0, true) | (byte_mask16(g1, 0, true) << 16);
-		const int n = ntiles - lo;
-		if (n < 32) {
-			sel &= (1u << n) - 1u;
-			sel2 &= (1u << n) - 1u;
+	const int pieces = (ntiles + 32 * kCompactThreads - 1) / (32 * kCompactThreads);  // <= kCompactMaxPieces
+	const int base = (int)threadIdx.x * 32 * pieces;
+	uint32_t sel[kCompactMaxPieces];
+	int n = 0, n2 = 0;
+#pragma unroll
+	for (int q = 0; q < kCompactMaxPieces; ++q) {
+		sel[q] = 0;
+		const int lo = base + 32 * q;
+		if (q < pieces && lo < ntiles) {
+			uint32_t s2;
+			compact_piece(flags, colours, colour, flags2, lo, ntiles, sel[q], s2);
+			n += __popc(sel[q]);
+			n2 += __popc(s2);
 		}
 	}
 	int total = 0;
-	int pos = block_exclusive_scan_1024(__popc(sel), sums[0], total);
+	int pos = block_exclusive_scan_1024(n, sums[0], total);
 	if (threadIdx.x == 0) {
 		*count = total;
 	}
 	if (flags2) {
 		int total2 = 0;
-		block_exclusive_scan_1024(__popc(sel2), sums[1], total2);
+		block_exclusive_scan_1024(n2, sums[1], total2);
 		if (threadIdx.x == 0) {
 			*count2 = total2;
 		}
 	}
-	for (uint32_t m = sel; m; m &= m - 1) {
-		const int i = lo + __ffs(m) - 1;
-		if (list) {
-			list[pos++] = i;
-		}
-		if (clear) {
-			flags[i] = 0;
+#pragma unroll
+	for (int q = 0; q < kCompactMaxPieces; ++q) {
+		for (uint32_t m = sel[q]; m; m &= m - 1) {
+			const int i = base + 32 * q + __ffs(m) - 1;
+			if (list) {
+				list[pos++] = i;
+			}
+			if (clear) {
+				flags[i] = 0;
+			}
 		}
 	}
 }
