@@ -35,131 +35,96 @@ constexpr uint32_t kStateExcess = 1u << 30;
 // ---------------------------------------------------------------------------------------------
 // init: residual planes, cancellation, excess, residual sink link. Sets status bit 0 for a negative
 // capacity, bit 1 when a voxel's inflow or an edge's capacity pair does not fit int32, and bit 2
-// when some edge's capacity pair exceeds 32767 (then residuals cannot be staged as int16). One
-// workgroup per tile (grid-stride over the tiles of every instance), thread (lx, ly) owning a
-// z-column, so x stays the coalesced axis.
-template <int kConn, int kTX, int kTY, int kTZ>
-__global__ __launch_bounds__(kTX *kTY) void k_grid_init(GridView gv, int ntiles)
+// when some edge's capacity pair exceeds 32767 (then residuals cannot be staged as int16).
+// Voxel-parallel over z-columns rather than tiles: the xy plane of a grid is cut into strips of
+// kInitThreads consecutive xy indices (x fastest, running on across rows, so no lane idles on a
+// partial tile), z into chunks of kInitDepth; workgroup b takes one strip × one chunk (strip-major
+// within its chunk; block_inst / inst_first_block give the grid of b in a batch), thread t the
+// column of its xy index, whose +z neighbours are its own next voxels. All loads of a voxel are
+// issued before its arithmetic and stores.
+constexpr int kInitThreads = 256, kInitDepth = 8;
+
+__host__ __device__ inline int64_t init_blocks(int64_t nx, int64_t ny, int64_t nz)
 {
-	using T = Tile<kConn, kTX, kTY, kTZ>;
+	return ((nx * ny + kInitThreads - 1) / kInitThreads) * ((nz + kInitDepth - 1) / kInitDepth);
+}
+
+template <int kConn>
+__global__ __launch_bounds__(kInitThreads) void k_grid_init(GridView gv, const int32_t *__restrict__ block_inst,
+                                                           const int32_t *__restrict__ inst_first_block)
+{
 	using D = Dirs<kConn>;
 	constexpr int K = D::K, H = D::H;
-	__shared__ long long red[T::NT / 64];
-	const int tid = threadIdx.x;
-	const int lx = tid % T::TX, ly = tid / T::TX;
-	for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-		int local;
-		const GridView g = instance_view(gv, tile, local);
-		int tx, ty, tz;
-		tile_coords(g, local, tx, ty, tz);
-		const int x = tx * T::TX + lx, y = ty * T::TY + ly, oz = tz * T::TZ;
-		const uint32_t xy = slice_xy(g, x, y), sxy = (uint32_t)(g.nx * g.ny);
-		uint32_t valid = 0;
-		int32_t ex[T::TZ], rt[T::TZ];
-		long long incoming[T::TZ];
-		uint32_t state[T::TZ];
-		long long cancelled = 0;
-		bool negative = false, overflow = false, wide = false;
-		// loads of a group first, then its arithmetic and stores: t-links, then the n-links in
-		// groups of kGroup directions
-#pragma unroll
-		for (int k = 0; k < T::TZ; ++k) {
-			valid |= (x < g.nx && y < g.ny && oz + k < g.nz) ? (1u << k) : 0u;
-			ex[k] = rt[k] = 0;
-			if (valid & (1u << k)) {
-				ex[k] = tile_at(g.cs, g, oz, k, xy, sxy);
-				rt[k] = tile_at(g.ct, g, oz, k, xy, sxy);
+	__shared__ long long red[kInitThreads / 64];
+	int lb = blockIdx.x;
+	GridView g = gv;
+	if (gv.inst) {
+		const int i = block_inst[blockIdx.x];
+		lb = blockIdx.x - inst_first_block[i];
+		g = instance_at(gv, i);
+	}
+	const uint32_t sxy = (uint32_t)(g.nx * g.ny);
+	const int strips = (int)((sxy + kInitThreads - 1) / kInitThreads);
+	const int chunk = lb / strips, strip = lb - chunk * strips;
+	const uint32_t xy = (uint32_t)strip * kInitThreads + threadIdx.x;
+	const int y = (int)(xy / (uint32_t)g.nx), x = (int)(xy - (uint32_t)y * (uint32_t)g.nx);
+	const int oz = chunk * kInitDepth;
+	long long cancelled = 0;
+	bool negative = false, overflow = false, wide = false;
+	if (xy < sxy) {
+#pragma unroll 2
+		for (int k = 0; k < kInitDepth; ++k) {
+			const int z = oz + k;
+			if (z >= g.nz) {
+				continue;
 			}
-		}
+			int32_t out[K], in[K];
+			const int32_t cs = tile_at(g.cs, g, oz, k, xy, sxy), ct = tile_at(g.ct, g, oz, k, xy, sxy);
 #pragma unroll
-		for (int k = 0; k < T::TZ; ++k) {
-			const int32_t cs = ex[k], ct = rt[k];
+			for (int d = 0; d < K; ++d) {
+				out[d] = in[d] = 0;
+				if (!inside(g, x + D::dx(d), y + D::dy(d), z + D::dz(d))) {
+					continue;
+				}
+				if (d < H) {
+					out[d] = tile_at(g.cn + (int64_t)d * g.nvox, g, oz, k, xy, sxy);
+					in[d] = g.cnb ? tile_at(g.cnb + (int64_t)d * g.nvox, g, oz, k, xy, sxy) : out[d];
+				} else {
+					const uint32_t uxy = slice_xy(g, x + D::dx(d), y + D::dy(d));
+					const int uk = k + D::dz(d);
+					in[d] = tile_at(g.cn + (int64_t)(d - H) * g.nvox, g, oz, uk, uxy, sxy);
+					out[d] = g.cnb ? tile_at(g.cnb + (int64_t)(d - H) * g.nvox, g, oz, uk, uxy, sxy) : in[d];
+				}
+			}
 			negative |= (cs < 0) | (ct < 0);
 			const int32_t m = cs < ct ? cs : ct;
 			cancelled += m;
-			ex[k] = cs - m;
-			rt[k] = ct - m;
-			incoming[k] = ex[k];
-			state[k] = (ex[k] > 0 ? kStateExcess : 0u) | (rt[k] > 0 ? kStateSink : 0u);
-		}
-		constexpr int kGroup = 3;
+			long long incoming = cs - m;
+			uint32_t state = (cs - m > 0 ? kStateExcess : 0u) | (ct - m > 0 ? kStateSink : 0u);
 #pragma unroll
-		for (int d0 = 0; d0 < K; d0 += kGroup) {
-			int32_t out[kGroup][T::TZ], in[kGroup][T::TZ];
-#pragma unroll
-			for (int j = 0; j < kGroup; ++j) {
-				const int d = d0 + j;
-#pragma unroll
-				for (int k = 0; k < T::TZ; ++k) {
-					out[j][k] = in[j][k] = 0;
-					if (d >= K || !(valid & (1u << k)) || !inside(g, x + D::dx(d), y + D::dy(d), oz + k + D::dz(d))) {
-						continue;
-					}
-					if (d < H) {
-						out[j][k] = tile_at(g.cn + (int64_t)d * g.nvox, g, oz, k, xy, sxy);
-						if (g.cnb) {
-							in[j][k] = tile_at(g.cnb + (int64_t)d * g.nvox, g, oz, k, xy, sxy);
-						}
-					} else {
-						const int uk = k + D::dz(d);
-						const uint32_t uxy = slice_xy(g, x + D::dx(d), y + D::dy(d));
-						in[j][k] = tile_at(g.cn + (int64_t)(d - H) * g.nvox, g, oz, uk, uxy, sxy);
-						if (g.cnb) {
-							out[j][k] = tile_at(g.cnb + (int64_t)(d - H) * g.nvox, g, oz, uk, uxy, sxy);
-						}
-					}
-				}
+			for (int d = 0; d < K; ++d) {
+				negative |= (out[d] < 0) | (in[d] < 0);
+				overflow |= ((long long)out[d] + (long long)in[d]) > 0x7fffffffLL;
+				wide |= ((long long)out[d] + (long long)in[d]) > 0x7fffLL;
+				incoming += in[d];
+				tile_at(g.r + (int64_t)d * g.nvox, g, oz, k, xy, sxy) = out[d];
+				state |= out[d] > 0 ? (1u << d) : 0u;
 			}
-#pragma unroll
-			for (int j = 0; j < kGroup; ++j) {
-				const int d = d0 + j;
-				if (d >= K) {
-					continue;
-				}
-#pragma unroll
-				for (int k = 0; k < T::TZ; ++k) {
-					if (!(valid & (1u << k))) {
-						continue;
-					}
-					int32_t o = out[j][k], i = in[j][k];
-					if (!g.cnb) {
-						// symmetric capacities: one plane holds both directions
-						if (d < H) {
-							i = o;
-						} else {
-							o = i;
-						}
-					}
-					negative |= (o < 0) | (i < 0);
-					overflow |= ((long long)o + (long long)i) > 0x7fffffffLL;
-					wide |= ((long long)o + (long long)i) > 0x7fffLL;
-					incoming[k] += i;
-					tile_at(g.r + (int64_t)d * g.nvox, g, oz, k, xy, sxy) = o;
-					state[k] |= o > 0 ? (1u << d) : 0u;
-				}
-			}
+			overflow |= incoming > 0x7fffffffLL;
+			tile_at(g.omask, g, oz, k, xy, sxy) = state;
+			tile_at(g.e, g, oz, k, xy, sxy) = cs - m;
+			tile_at(g.rt, g, oz, k, xy, sxy) = ct - m;
 		}
-#pragma unroll
-		for (int k = 0; k < T::TZ; ++k) {
-			if (!(valid & (1u << k))) {
-				continue;
-			}
-			overflow |= incoming[k] > 0x7fffffffLL;
-			tile_at(g.omask, g, oz, k, xy, sxy) = state[k];
-			tile_at(g.e, g, oz, k, xy, sxy) = ex[k];
-			tile_at(g.rt, g, oz, k, xy, sxy) = rt[k];
-		}
-		const int bits = (negative ? 1 : 0) | (overflow ? 2 : 0) | (wide ? 4 : 0);
-		// (a plain read first: when every tile sets a bit, one atomic per tile on one word would
-		// serialise in the L2)
-		if (bits && (*(volatile int32_t *)g.status & bits) != bits) {
-			atomicOr(g.status, bits);
-		}
-		const long long total = block_sum_i64<T::NT>(cancelled, red);
-		if (tid == 0 && total) {
-			atomicAdd(flow_slot(g, tile), (unsigned long long)total);
-		}
-		__syncthreads();  // red is reused by the next tile
+	}
+	const int bits = (negative ? 1 : 0) | (overflow ? 2 : 0) | (wide ? 4 : 0);
+	// (a plain read first: when every workgroup sets a bit, one atomic each on one word would
+	// serialise in the L2)
+	if (bits && (*(volatile int32_t *)g.status & bits) != bits) {
+		atomicOr(g.status, bits);
+	}
+	const long long total = block_sum_i64<kInitThreads>(cancelled, red);
+	if (threadIdx.x == 0 && total) {
+		atomicAdd(flow_slot(g, blockIdx.x), (unsigned long long)total);
 	}
 }
 

@@ -223,6 +223,7 @@ void GridSolver::allocate_grids(int conn, const std::vector<InstanceDesc> &inst,
 	mSolved = false;
 	mStarted = false;
 	mVolumeKept = false;
+	mTilesReady = false;
 	mInst = inst;
 	mFlows.assign(count, 0);
 	mNx = inst[0].nx;
@@ -402,6 +403,11 @@ const InstanceDesc &GridSolver::instance(int index) const
 
 void GridSolver::configure_tiles(const Shape &shape)
 {
+	if (mTilesReady && mReadyWave == mWave && mReadyShape.tx == shape.tx && mReadyShape.ty == shape.ty &&
+	    mReadyShape.tz == shape.tz) {
+		return;
+	}
+	mTilesReady = false;
 	mShape = shape;
 	const int ncolours = mConn == 6 ? 2 : 8;
 	std::vector<int> per_colour(ncolours, 0);
@@ -473,6 +479,32 @@ void GridSolver::configure_tiles(const Shape &shape)
 		                       hipMemcpyHostToDevice, mStream));
 		CGP_HIP(hipStreamSynchronize(mStream));
 	}
+	// workgroups of the voxel-parallel init (init_blocks per grid)
+	{
+		std::vector<int32_t> first(mInst.size());
+		int64_t blocks = 0;
+		for (size_t i = 0; i < mInst.size(); ++i) {
+			first[i] = (int32_t)blocks;
+			blocks += init_blocks(mInst[i].nx, mInst[i].ny, mInst[i].nz);
+		}
+		mInitBlocks = (int)blocks;
+		if (mInst.size() > 1) {
+			std::vector<int32_t> owner_block((size_t)blocks);
+			for (size_t i = 0; i < mInst.size(); ++i) {
+				const int64_t end = i + 1 < mInst.size() ? first[i + 1] : blocks;
+				for (int64_t b = first[i]; b < end; ++b) {
+					owner_block[(size_t)b] = (int32_t)i;
+				}
+			}
+			mInitBlockInst.allocate((size_t)blocks);
+			mInitFirstBlock.allocate(mInst.size());
+			CGP_HIP(hipMemcpyAsync(mInitBlockInst.get(), owner_block.data(), owner_block.size() * sizeof(int32_t),
+			                       hipMemcpyHostToDevice, mStream));
+			CGP_HIP(hipMemcpyAsync(mInitFirstBlock.get(), first.data(), first.size() * sizeof(int32_t),
+			                       hipMemcpyHostToDevice, mStream));
+			CGP_HIP(hipStreamSynchronize(mStream));
+		}
+	}
 	if (mInst.size() > 1) {
 		mInstDev.allocate(mInst.size());
 		mTileInst.allocate(mTiles);
@@ -488,6 +520,9 @@ void GridSolver::configure_tiles(const Shape &shape)
 		mFaces.allocate((size_t)mTiles * wave::kFaces);
 	}
 	CGP_HIP(hipStreamSynchronize(mStream));
+	mTilesReady = true;
+	mReadyWave = mWave;
+	mReadyShape = shape;
 }
 
 static GridView make_view(int nx, int ny, int nz, int ntx, int nty, int ntz, int64_t vox, DeviceBuffer<int32_t> &e,
@@ -553,12 +588,15 @@ void GridSolver::launch_init()
 		return;
 	}
 	launch(kInit, mVox, bytes, [&] {
-		dispatch_shape(mConn, mShape.tx, mShape.ty, mShape.tz, [&](auto tag) {
-			using S = decltype(tag);
-			// one workgroup per tile: a grid-stride loop would chain the per-tile latencies
-			hipLaunchKernelGGL((k_grid_init<S::conn, S::tx, S::ty, S::tz>), dim3(mTiles),
-			                   dim3(S::tx * S::ty), 0, mStream, g, mTiles);
-		});
+		const bool batch = mInst.size() > 1;
+		const int32_t *block_inst = batch ? mInitBlockInst.get() : nullptr;
+		const int32_t *first = batch ? mInitFirstBlock.get() : nullptr;
+		if (mConn == 6) {
+			hipLaunchKernelGGL(k_grid_init<6>, dim3(mInitBlocks), dim3(kInitThreads), 0, mStream, g, block_inst, first);
+		} else {
+			hipLaunchKernelGGL(k_grid_init<26>, dim3(mInitBlocks), dim3(kInitThreads), 0, mStream, g, block_inst,
+			                   first);
+		}
 	});
 }
 

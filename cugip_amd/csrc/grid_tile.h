@@ -115,15 +115,9 @@ __device__ inline unsigned long long *counter_slot(const GridView &g, int kind, 
 	return g.counters + kind * kSlotWords + (tile & (kSlots - 1)) * kSlotStride;
 }
 
-// The view of the instance that owns global tile `tile`, and the tile's id inside it. For a single
-// grid this is the view itself.
-__device__ inline GridView instance_view(const GridView &g, int tile, int &local)
+// The view of instance i of a batch.
+__device__ inline GridView instance_at(const GridView &g, int i)
 {
-	if (!g.inst) {
-		local = tile;
-		return g;
-	}
-	const int i = g.tile_inst[tile];
 	const InstanceDesc d = g.inst[i];
 	GridView v = g;
 	v.nx = d.nx;
@@ -145,7 +139,19 @@ __device__ inline GridView instance_view(const GridView &g, int tile, int &local
 	v.cn = g.cn + d.vbase * (d.conn / 2);
 	v.cnb = g.cnb ? g.cnb + d.vbase * (d.conn / 2) : nullptr;
 	v.flow = g.flow + (int64_t)i * kSlotWords;
-	local = tile - d.tbase;
+	return v;
+}
+
+// The view of the instance that owns global tile `tile`, and the tile's id inside it. For a single
+// grid this is the view itself.
+__device__ inline GridView instance_view(const GridView &g, int tile, int &local)
+{
+	if (!g.inst) {
+		local = tile;
+		return g;
+	}
+	const GridView v = instance_at(g, g.tile_inst[tile]);
+	local = tile - v.tbase;
 	return v;
 }
 

@@ -115,6 +115,12 @@ private:
 	std::vector<int64_t> mFlows;               // per-instance flow of the last solve
 	DeviceBuffer<InstanceDesc> mInstDev;       // batches only
 	DeviceBuffer<int32_t> mTileInst;           // batches only: instance of every tile
+	DeviceBuffer<int32_t> mInitBlockInst;      // batches only: instance of every init workgroup
+	DeviceBuffer<int32_t> mInitFirstBlock;     // batches only: first init workgroup of every instance
+	int mInitBlocks = 0;
+	// configure_tiles' tables are kept while the grids, the tile shape and the engine stay the same
+	bool mTilesReady = false, mReadyWave = false;
+	Shape mReadyShape{0, 0, 0};
 	DeviceBuffer<uint8_t> mTileColour;         // colour of every tile inside its instance
 	DeviceBuffer<int32_t> mColourTiles;        // tile ids grouped by colour, then one count per colour
 	std::vector<int> mColourStart, mColourSize;
