@@ -72,7 +72,7 @@ __global__ __launch_bounds__(kInitThreads) void k_grid_init(GridView gv, const i
 	long long cancelled = 0;
 	bool negative = false, overflow = false, wide = false;
 	if (xy < sxy) {
-#pragma unroll 2
+#pragma unroll
 		for (int k = 0; k < kInitDepth; ++k) {
 			const int z = oz + k;
 			if (z >= g.nz) {
