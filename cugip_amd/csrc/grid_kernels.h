@@ -71,8 +71,10 @@ __global__ __launch_bounds__(kInitThreads) void k_grid_init(GridView gv, const i
 	const int oz = chunk * kInitDepth;
 	long long cancelled = 0;
 	bool negative = false, overflow = false, wide = false;
+	// the whole column unrolled for 6C; 26C keeps 2 voxels in flight (52 capacities each)
+	constexpr int kUnroll = kConn == 6 ? kInitDepth : 2;
 	if (xy < sxy) {
-#pragma unroll
+#pragma unroll kUnroll
 		for (int k = 0; k < kInitDepth; ++k) {
 			const int z = oz + k;
 			if (z >= g.nz) {
@@ -333,7 +335,8 @@ template <int kConn, int kTX, int kTY, int kTZ>
 __global__ __launch_bounds__(kTX *kTY) void k_grid_bfs(GridView gv, const int32_t *__restrict__ list,
                                                        const int32_t *__restrict__ count, int32_t threshold,
                                                        uint8_t *__restrict__ dirty_next,
-                                                       uint8_t *__restrict__ tile_active)
+                                                       uint8_t *__restrict__ tile_active,
+                                                       uint8_t *__restrict__ tile_work)
 {
 	using T = Tile<kConn, kTX, kTY, kTZ>;
 	const int tile = list[blockIdx.x];  // in bounds: the grid never exceeds the list's capacity
@@ -348,11 +351,16 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_bfs(GridView gv, const int32_
 
 	__shared__ int32_t hl[T::NP];
 	__shared__ uint8_t marks[27];
+	__shared__ int span[2];  // highest label holding excess, lowest finite label
 	const int tid = threadIdx.x;
 	const int lx = tid % T::TX, ly = tid / T::TX;
 	const int gx = ox + lx, gy = oy + ly;
 	if (tid < 27) {
 		marks[tid] = 0;
+	}
+	if (tid == 0) {
+		span[0] = 0;
+		span[1] = kLabelInf;
 	}
 	// every global load of the visit first (halo labels, state words), then the LDS stores
 	HaloLabels<T> halo;
@@ -403,6 +411,7 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_bfs(GridView gv, const int32_
 	relax_tile<T>(hl, open, valid, lx, ly);
 
 	int active = 0;
+	int top = 0, bottom = kLabelInf;
 #pragma unroll
 	for (int k = 0; k < T::TZ; ++k) {
 		if (!(valid & (1u << k))) {
@@ -411,6 +420,10 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_bfs(GridView gv, const int32_
 		const int p = T::pidx(lx, ly, k);
 		const int32_t nl = hl[p];
 		active |= has_excess[k] && nl < kLabelInf;
+		if (has_excess[k] && nl < kLabelInf && nl > top) {
+			top = nl;
+		}
+		bottom = nl < bottom ? nl : bottom;
 		if (nl != before[k]) {
 			tile_at(g.h, g, oz, k, xy, sxy) = nl;
 #pragma unroll
@@ -423,12 +436,22 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_bfs(GridView gv, const int32_
 			}
 		}
 	}
+	top = wave_max_i32(top);
+	bottom = -wave_max_i32(-bottom);
+	if ((tid & 63) == 0) {
+		atomicMax(&span[0], top);
+		atomicMin(&span[1], bottom);
+	}
 	active = __syncthreads_or(active);
 	if (tid < 27 && marks[tid]) {
 		mark_neighbour_tile(g, dirty_next, tx, ty, tz, tid);
 	}
 	if (tid == 0) {
 		tile_active[tile] = active ? 1 : 0;
+		// discharge work estimate: the levels between the tile's lowest label and its highest
+		// label holding excess (the discharge runs the heaviest tiles first)
+		const int levels = active ? span[0] - span[1] + 1 : 0;
+		tile_work[tile] = (uint8_t)(levels < 255 ? levels : 255);
 		atomicAdd(counter_slot(g, 1, tile), 1ull);
 	}
 }
@@ -1035,14 +1058,21 @@ __device__ inline void compact_piece(const uint8_t *flags, const uint8_t *colour
 	}
 }
 
+// With work set (a per-tile estimate), the list is ordered by decreasing work class (kWorkClasses
+// classes of work / 4, the last one open-ended), in tile order within a class: the heaviest tiles
+// start first and do not finish after everyone else.
+constexpr int kWorkClasses = 8;
+
 __global__ __launch_bounds__(kCompactThreads) void k_compact_block(uint8_t *__restrict__ flags, int ntiles,
                                                                    const uint8_t *__restrict__ colours, int colour,
                                                                    int clear, int32_t *__restrict__ list,
                                                                    int32_t *__restrict__ count,
                                                                    const uint8_t *__restrict__ flags2,
-                                                                   int32_t *__restrict__ count2)
+                                                                   int32_t *__restrict__ count2,
+                                                                   const uint8_t *__restrict__ work)
 {
 	__shared__ int sums[2][kCompactThreads / 64];
+	__shared__ int class_sums[kWorkClasses][kCompactThreads / 64];
 	const int pieces = (ntiles + 32 * kCompactThreads - 1) / (32 * kCompactThreads);  // <= kCompactMaxPieces
 	const int base = (int)threadIdx.x * 32 * pieces;
 	uint32_t sel[kCompactMaxPieces];
@@ -1074,11 +1104,55 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact_block(uint8_t *__re
 	for (int q = 0; q < kCompactMaxPieces; ++q) {
 		for (uint32_t m = sel[q]; m; m &= m - 1) {
 			const int i = base + 32 * q + __ffs(m) - 1;
-			if (list) {
+			if (list && !work) {
 				list[pos++] = i;
 			}
 			if (clear) {
 				flags[i] = 0;
+			}
+		}
+	}
+	if (!list || !work) {
+		return;
+	}
+	// work classes: class c holds work in [4 (kWorkClasses - 1 - c), ...), heaviest first
+	int per_class[kWorkClasses];
+#pragma unroll
+	for (int c = 0; c < kWorkClasses; ++c) {
+		per_class[c] = 0;
+	}
+	auto class_of = [](int w) {
+		const int b = w >> 2;
+		return kWorkClasses - 1 - (b < kWorkClasses - 1 ? b : kWorkClasses - 1);
+	};
+#pragma unroll
+	for (int q = 0; q < kCompactMaxPieces; ++q) {
+		for (uint32_t m = sel[q]; m; m &= m - 1) {
+			const int c = class_of(work[base + 32 * q + __ffs(m) - 1]);
+#pragma unroll
+			for (int cc = 0; cc < kWorkClasses; ++cc) {
+				per_class[cc] += cc == c ? 1 : 0;
+			}
+		}
+	}
+	int offset[kWorkClasses];
+	int start = 0;
+#pragma unroll
+	for (int c = 0; c < kWorkClasses; ++c) {
+		int class_total = 0;
+		offset[c] = start + block_exclusive_scan_1024(per_class[c], class_sums[c], class_total);
+		start += class_total;
+	}
+#pragma unroll
+	for (int q = 0; q < kCompactMaxPieces; ++q) {
+		for (uint32_t m = sel[q]; m; m &= m - 1) {
+			const int i = base + 32 * q + __ffs(m) - 1;
+			const int c = class_of(work[i]);
+#pragma unroll
+			for (int cc = 0; cc < kWorkClasses; ++cc) {
+				if (cc == c) {
+					list[offset[cc]++] = i;
+				}
 			}
 		}
 	}

@@ -446,6 +446,7 @@ void GridSolver::configure_tiles(const Shape &shape)
 	// flag arrays padded to whole 32-byte chunks: the one-workgroup compaction reads them 16 bytes at a time
 	const int64_t padded = ((int64_t)mTiles + 31) / 32 * 32;
 	mActive.allocate(padded);
+	mTileWork.allocate(padded);
 	mDirtyA.allocate(padded);
 	mDirtyB.allocate(padded);
 	mList.allocate(mTiles);
@@ -600,13 +601,13 @@ void GridSolver::launch_init()
 	});
 }
 
-void GridSolver::launch_compact(uint8_t *flags, int colour, bool clear)
+void GridSolver::launch_compact(uint8_t *flags, int colour, bool clear, const uint8_t *work)
 {
 	if (mTiles <= kCompactBlockTiles) {
 		launch(kCompact, 0, 0.0, [&] {
 			hipLaunchKernelGGL(k_compact_block, dim3(1), dim3(kCompactThreads), 0, mStream, flags, mTiles,
 			                   mTileColour.get(), colour, clear ? 1 : 0, mList.get(), mCount.get(),
-			                   (const uint8_t *)nullptr, (int32_t *)nullptr);
+			                   (const uint8_t *)nullptr, (int32_t *)nullptr, work);
 		});
 		return;
 	}
@@ -627,7 +628,7 @@ void GridSolver::count_two(uint8_t *a, uint8_t *b, int &na, int &nb)
 		launch(kCompact, 0, 0.0, [&] {
 			hipLaunchKernelGGL(k_compact_block, dim3(1), dim3(kCompactThreads), 0, mStream, a, mTiles,
 			                   mTileColour.get(), -1, 0, (int32_t *)nullptr, mCount.get() + 1, (const uint8_t *)b,
-			                   mCount.get() + 2);
+			                   mCount.get() + 2, (const uint8_t *)nullptr);
 		});
 		CGP_HIP(hipMemcpyAsync(pinned, mCount.get() + 1, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, mStream));
 		CGP_HIP(hipStreamSynchronize(mStream));
@@ -721,7 +722,7 @@ int GridSolver::run_global_relabel(int32_t threshold, cgp_stats &stats)
 					                                  hipLaunchKernelGGL((k_grid_bfs<S::conn, S::tx, S::ty, S::tz>),
 					                                                     dim3(grid), dim3(S::tx * S::ty), 0, mStream, g,
 					                                                     list, count, threshold, mDirtyA.get(),
-					                                                     mActive.get());
+					                                                     mActive.get(), mTileWork.get());
 				                                  });
 			                                  });
 		                                  });
@@ -761,7 +762,8 @@ int GridSolver::run_global_relabel(int32_t threshold, cgp_stats &stats)
 void GridSolver::run_discharge_phase(int colour, int max_iters)
 {
 	GridView g = CGP_VIEW();
-	launch_compact(mActive.get(), colour);
+	// heaviest tiles first (work estimates of the last relabel; the wave engine has none)
+	launch_compact(mActive.get(), colour, false, mWave ? nullptr : mTileWork.get());
 	// every tile of one colour; blocks beyond the compacted count exit at once
 	const int blocks = mMaxColourTiles;
 	if (mWave) {

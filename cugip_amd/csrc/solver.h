@@ -96,7 +96,7 @@ private:
 	void launch_init();
 	int run_global_relabel(int32_t threshold, cgp_stats &stats);
 	int count_flags(const uint8_t *flags, int colour);
-	void launch_compact(uint8_t *flags, int colour, bool clear = false);
+	void launch_compact(uint8_t *flags, int colour, bool clear = false, const uint8_t *work = nullptr);
 	void count_two(uint8_t *a, uint8_t *b, int &na, int &nb);
 	template <typename Pass>
 	int colour_sweeps(uint8_t *dirty, uint8_t *also_count, int &also, Pass &&pass);
@@ -147,6 +147,7 @@ private:
 	DeviceBuffer<unsigned long long> mCutSlots;  // cut capacity: kSlots slots and their sum
 	DeviceBuffer<int32_t> mStatus, mCount, mList, mAllTiles;
 	DeviceBuffer<uint8_t> mActive, mDirtyA, mDirtyB;
+	DeviceBuffer<uint8_t> mTileWork;  // per tile: discharge work estimate from the last relabel
 	DeviceBuffer<uint32_t> mBits, mSBits, mReach;  // wave engine: bit planes, S_min and reach planes
 	DeviceBuffer<int32_t> mFaces;                  // wave engine: face labels
 	bool mWave = false;                    // wave engine (6-connected, 32x8x4 tiles)
