@@ -825,11 +825,13 @@ __global__ __launch_bounds__(kTX *kTY, 6) void k_grid_discharge(GridView gv, con
 // S_min closure pass: s(u) = 1 if u has excess or some v with s(v) = 1 has residual v -> u > 0.
 // s starts at 0 (a fill before the first pass) and bits only rise, so concurrently running tiles
 // may set bits in each other's voxels. first_pass seeds from the excess; the open directions come
-// from the state word.
+// from the state word. With reached set, a pass that puts a voxel with a residual sink link into the
+// closure stores 1 there: some excess can still reach t, the preflow is not maximum yet.
 template <int kConn, int kTX, int kTY, int kTZ>
 __global__ __launch_bounds__(kTX *kTY) void k_grid_source_closure(GridView gv, const int32_t *__restrict__ list,
                                                                   const int32_t *__restrict__ count, int first_pass,
-                                                                  uint8_t *__restrict__ dirty_next)
+                                                                  uint8_t *__restrict__ dirty_next,
+                                                                  int32_t *__restrict__ reached)
 {
 	using T = Tile<kConn, kTX, kTY, kTZ>;
 	using D = typename T::D;
@@ -855,6 +857,7 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_source_closure(GridView gv, c
 	bool valid[T::TZ];
 	uint32_t open[T::TZ];
 	uint8_t before[T::TZ];
+	uint32_t sink = 0;  // bit k: own voxel k has a residual sink link
 	// all loads first, then the LDS stores
 #pragma unroll
 	for (int k = 0; k < T::TZ; ++k) {
@@ -871,6 +874,7 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_source_closure(GridView gv, c
 		const uint32_t st = open[k];
 		sl[T::vidx(lx, ly, k)] = ((first_pass && (st & kStateExcess)) || before[k]) ? 1 : 0;
 		open[k] = st & ((1u << T::K) - 1u);
+		sink |= (st & kStateSink) ? (1u << k) : 0u;
 	}
 	__syncthreads();
 	for (;;) {
@@ -901,12 +905,14 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_source_closure(GridView gv, c
 		}
 	}
 	// write back own voxels; push the closure across the tile faces
+	bool hit = false;
 #pragma unroll
 	for (int k = 0; k < T::TZ; ++k) {
 		const int i = T::vidx(lx, ly, k);
 		if (!valid[k]) {
 			continue;
 		}
+		hit |= sl[i] && ((sink >> k) & 1u);
 		if (sl[i] != before[k]) {
 			tile_at(g.s, g, oz, k, xy, sxy) = sl[i];
 		}
@@ -928,6 +934,9 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_source_closure(GridView gv, c
 				marks[slot] = 1;
 			}
 		}
+	}
+	if (reached && __ballot(hit) && (tid & 63) == 0) {
+		*reached = 1;  // every writer stores the same value
 	}
 	__syncthreads();
 	if (tid < 27 && marks[tid]) {

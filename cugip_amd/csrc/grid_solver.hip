@@ -251,7 +251,7 @@ void GridSolver::allocate_grids(int conn, const std::vector<InstanceDesc> &inst,
 	mCounters.allocate(3 * kSlotWords);
 	mCounterTotal.allocate(1);
 	mStatus.allocate(1);
-	mCount.allocate(3);
+	mCount.allocate(4);  // [0] list count, [1] [2] count_two, [3] closure reached t
 	mHostScratch.allocate(count > 8 ? count : 8);
 }
 
@@ -630,7 +630,7 @@ void GridSolver::count_two(uint8_t *a, uint8_t *b, int &na, int &nb)
 			                   mTileColour.get(), -1, 0, (int32_t *)nullptr, mCount.get() + 1, (const uint8_t *)b,
 			                   mCount.get() + 2, (const uint8_t *)nullptr);
 		});
-		CGP_HIP(hipMemcpyAsync(pinned, mCount.get() + 1, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, mStream));
+		CGP_HIP(hipMemcpyAsync(pinned, mCount.get() + 1, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, mStream));
 		CGP_HIP(hipStreamSynchronize(mStream));
 		na = pinned[0];
 		nb = b ? pinned[1] : 0;
@@ -648,7 +648,7 @@ void GridSolver::count_two(uint8_t *a, uint8_t *b, int &na, int &nb)
 			                   0, (int32_t *)nullptr, mCount.get() + 2);
 		}
 	});
-	CGP_HIP(hipMemcpyAsync(pinned, mCount.get() + 1, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, mStream));
+	CGP_HIP(hipMemcpyAsync(pinned, mCount.get() + 1, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, mStream));
 	CGP_HIP(hipStreamSynchronize(mStream));
 	na = pinned[0];
 	nb = b ? pinned[1] : 0;
@@ -662,7 +662,7 @@ void GridSolver::count_two(uint8_t *a, uint8_t *b, int &na, int &nb)
 // Returns the number of pass launches; `also` receives the flag count of also_count (if given)
 // at the final check.
 template <typename Pass>
-int GridSolver::colour_sweeps(uint8_t *dirty, uint8_t *also_count, int &also, Pass &&pass)
+int GridSolver::colour_sweeps(uint8_t *dirty, uint8_t *also_count, int &also, Pass &&pass, bool *stop)
 {
 	constexpr int kCyclesPerCheck = 2;
 	int launches = 0;
@@ -672,12 +672,30 @@ int GridSolver::colour_sweeps(uint8_t *dirty, uint8_t *also_count, int &also, Pa
 		}
 		pass(mColourTiles.get() + mColourStart[c], mColourTiles.get() + mTiles + c, mColourSize[c], true);
 		++launches;
+		if (stop && c + 1 < mColours) {
+			// a test that fails usually fails in its first pass: read the flag after each colour
+			int32_t *pinned = reinterpret_cast<int32_t *>(mHostScratch.get());
+			CGP_HIP(hipMemcpyAsync(pinned, mCount.get() + 3, sizeof(int32_t), hipMemcpyDeviceToHost, mStream));
+			CGP_HIP(hipStreamSynchronize(mStream));
+			if (pinned[0]) {
+				*stop = true;
+				return launches;
+			}
+		}
 	}
 	for (;;) {
 		int pending = 0;
 		count_two(dirty, also_count, pending, also);
 		if (pending == 0) {
 			return launches;
+		}
+		if (stop) {
+			// the sweeps' own flag (the closure reaching t), read with the counts
+			const int32_t *pinned = reinterpret_cast<const int32_t *>(mHostScratch.get());
+			if (pinned[2]) {
+				*stop = true;
+				return launches;
+			}
 		}
 		for (int cycle = 0; cycle < kCyclesPerCheck; ++cycle) {
 			for (int c = 0; c < mColours; ++c) {
@@ -792,11 +810,15 @@ void GridSolver::run_discharge_phase(int colour, int max_iters)
 	});
 }
 
-void GridSolver::run_source_closure(cgp_stats &stats)
+bool GridSolver::run_source_closure(cgp_stats &stats, bool check)
 {
 	GridView g = CGP_VIEW();
 	const int64_t tile_voxels = (int64_t)mShape.tx * mShape.ty * mShape.tz;
 	if (!mWave) {
+		int32_t *reached = check ? mCount.get() + 3 : nullptr;
+		launch(kFill, 0, 0.0, [&] {
+			hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, mStream, mCount.get() + 3, (int64_t)1, 0);
+		});
 		launch(kFill, mVox, 1.0, [&] {
 			hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(mVox, 256, 8192)), dim3(256), 0, mStream, mS.get(), mVox,
 			                   (uint8_t)0);
@@ -806,20 +828,29 @@ void GridSolver::run_source_closure(cgp_stats &stats)
 			                   (int64_t)mTiles, (uint8_t)0);
 		});
 		int unused = 0;
+		bool hit = false;
 		// reads the state word and s, writes s
-		stats.cut_passes += colour_sweeps(mDirtyB.get(), nullptr, unused,
-		                                  [&](const int32_t *list, const int32_t *count, int grid, bool first) {
-			                                  launch(kClosure, first ? grid * tile_voxels : -1, 6.0, [&] {
-				                                  dispatch_shape(mConn, mShape.tx, mShape.ty, mShape.tz, [&](auto tag) {
-					                                  using S = decltype(tag);
-					                                  hipLaunchKernelGGL(
-					                                      (k_grid_source_closure<S::conn, S::tx, S::ty, S::tz>),
-					                                      dim3(grid), dim3(S::tx * S::ty), 0, mStream, g, list, count,
-					                                      first ? 1 : 0, mDirtyB.get());
-				                                  });
-			                                  });
-		                                  });
-		return;
+		stats.cut_passes += colour_sweeps(
+		    mDirtyB.get(), nullptr, unused,
+		    [&](const int32_t *list, const int32_t *count, int grid, bool first) {
+			    launch(kClosure, first ? grid * tile_voxels : -1, 6.0, [&] {
+				    dispatch_shape(mConn, mShape.tx, mShape.ty, mShape.tz, [&](auto tag) {
+					    using S = decltype(tag);
+					    hipLaunchKernelGGL((k_grid_source_closure<S::conn, S::tx, S::ty, S::tz>), dim3(grid),
+					                       dim3(S::tx * S::ty), 0, mStream, g, list, count, first ? 1 : 0,
+					                       mDirtyB.get(), reached);
+				    });
+			    });
+		    },
+		    check ? &hit : nullptr);
+		if (check && !hit) {
+			// the sweeps ended without a stop: read the flag of the last passes
+			int32_t *pinned = reinterpret_cast<int32_t *>(mHostScratch.get());
+			CGP_HIP(hipMemcpyAsync(pinned, mCount.get() + 3, sizeof(int32_t), hipMemcpyDeviceToHost, mStream));
+			CGP_HIP(hipStreamSynchronize(mStream));
+			hit = pinned[0] != 0;
+		}
+		return hit;
 	}
 	launch(kFill, mVox, 0.125, [&] {
 		hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for((int64_t)mTiles * wave::R * 4, 256, 8192)), dim3(256), 0,
@@ -850,6 +881,7 @@ void GridSolver::run_source_closure(cgp_stats &stats)
 		count = mCount.get();
 		blocks = dirty;
 	}
+	return false;  // the wave engine is never asked to check
 }
 
 // Sum of the slots of visit counter `kind` (0 discharge, 1 relabel, 2 closure).
@@ -899,6 +931,7 @@ int64_t GridSolver::solve(const cgp_policy &policy, cgp_stats *stats_out, const 
 	mSolved = false;
 	mStoppedEarly = false;
 	mLabelsExact = false;
+	mClosedByCheck = false;
 	if (mCertificate > 0 && mInst.size() != 1) {
 		fail(CGP_ERR_INVALID, "a certificate applies to a single grid, not a batch");
 	}
@@ -979,11 +1012,26 @@ int64_t GridSolver::solve(const cgp_policy &policy, cgp_stats *stats_out, const 
 		if (mStoppedEarly) {
 			break;
 		}
+		if (!mWave && threshold == 0) {
+			// termination test by the closure instead of one more global relabel: when the closure of
+			// the excess reaches no voxel with a residual sink link, no excess can reach t, so the
+			// preflow is maximum and the closure is S_min. Otherwise it stops at the first check that
+			// sees t reached and the loop relabels as before.
+			t0 = std::chrono::steady_clock::now();
+			const bool reached = run_source_closure(stats, true);
+			stats.cut_ms += ms_since(t0);
+			if (!reached) {
+				mClosedByCheck = true;
+				break;
+			}
+		}
 	}
 	auto t0 = std::chrono::steady_clock::now();
-	run_source_closure(stats);
+	if (!mClosedByCheck) {
+		run_source_closure(stats);
+	}
 	const int64_t flow = read_flow();
-	stats.cut_ms = ms_since(t0);
+	stats.cut_ms += ms_since(t0);
 	stats.discharge_tile_visits = (int64_t)read_counter(0);
 	stats.bfs_tile_visits = (int64_t)read_counter(1);
 	stats.cut_tile_visits = (int64_t)read_counter(2);
@@ -995,7 +1043,9 @@ int64_t GridSolver::solve(const cgp_policy &policy, cgp_stats *stats_out, const 
 	stats.tile_z = mShape.tz;
 	stats.conn = mConn;
 	mSolved = true;
-	mLabelsExact = !mStoppedEarly;
+	// labels are those of the last relabel when the solve stopped at a certificate or at the
+	// closure test; T_min and label readbacks relabel once on request
+	mLabelsExact = !mStoppedEarly && !mClosedByCheck;
 	trace.emit(CGP_TRACE_AFTER_PUSH, round, 1, flow);
 	trace.emit(CGP_TRACE_FINISHED, round, 1, flow);
 	if (stats_out) {
@@ -1099,6 +1149,9 @@ void GridSolver::state(int index, int32_t *excess, int32_t *labels, uint32_t *sa
 		CGP_HIP(hipStreamSynchronize(mStream));
 	}
 	if (labels) {
+		if (mSolved) {
+			ensure_exact_labels();
+		}
 		if (mWave) {
 			GridView g = CGP_VIEW();
 			hipLaunchKernelGGL(wave::k_wave_bfs, dim3(mTiles), dim3(64), 0, mStream, g, mBits.get(), mTiles,

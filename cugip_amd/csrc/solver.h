@@ -99,9 +99,11 @@ private:
 	void launch_compact(uint8_t *flags, int colour, bool clear = false, const uint8_t *work = nullptr);
 	void count_two(uint8_t *a, uint8_t *b, int &na, int &nb);
 	template <typename Pass>
-	int colour_sweeps(uint8_t *dirty, uint8_t *also_count, int &also, Pass &&pass);
+	int colour_sweeps(uint8_t *dirty, uint8_t *also_count, int &also, Pass &&pass, bool *stop = nullptr);
 	void run_discharge_phase(int colour, int max_iters);
-	void run_source_closure(cgp_stats &stats);
+	// check: also report whether the closure reached a voxel with a residual sink link (the
+	// preflow is then not maximum and the closure is not S_min); it stops early when it does
+	bool run_source_closure(cgp_stats &stats, bool check = false);
 	int64_t read_flow();
 	unsigned long long read_counter(int kind);
 	const InstanceDesc &instance(int index) const;
@@ -134,6 +136,7 @@ private:
 	bool mVolumeKept = false;
 	bool mStoppedEarly = false;
 	bool mLabelsExact = false;  // labels (T_min) come from an exact relabel of the final state
+	bool mClosedByCheck = false;  // the last solve ended at the closure test (its S is S_min)
 	void ensure_exact_labels();
 	bool mHasBack = false;
 	bool mWideResiduals = true;  // some edge's capacity pair exceeds 32767: stage residuals as int32
