@@ -1067,10 +1067,53 @@ __device__ inline void compact_piece(const uint8_t *flags, const uint8_t *colour
 	}
 }
 
-// With work set (a per-tile estimate), the list is ordered by decreasing work class (kWorkClasses
-// classes of work / 4, the last one open-ended), in tile order within a class: the heaviest tiles
-// start first and do not finish after everyone else.
+// With work set (a per-tile estimate) and more than kOrderMin tiles listed (fewer all run at once),
+// the list is ordered by decreasing work class (kWorkClasses classes of work / 4, the last one
+// open-ended), in tile order within a class: the heaviest tiles start first and do not finish
+// after everyone else. The per-class counts travel packed, 16 bits a class in two 64-bit words, so
+// one workgroup scan orders all classes.
 constexpr int kWorkClasses = 8;
+constexpr int kOrderMin = 2048;
+
+__device__ inline int work_class(int w)
+{
+	const int b = w >> 2;
+	return kWorkClasses - 1 - (b < kWorkClasses - 1 ? b : kWorkClasses - 1);
+}
+
+// Exclusive scan of (lo, hi) pairs of packed counts over the workgroup; totals in tlo, thi.
+__device__ inline void block_exclusive_scan_pair_1024(unsigned long long &lo, unsigned long long &hi,
+                                                      unsigned long long (*wave_sums)[2], unsigned long long &tlo,
+                                                      unsigned long long &thi)
+{
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	unsigned long long a = lo, b = hi;
+#pragma unroll
+	for (int o = 1; o < 64; o <<= 1) {
+		const unsigned long long ta = __shfl_up(a, (unsigned)o, 64), tb = __shfl_up(b, (unsigned)o, 64);
+		if (lane >= o) {
+			a += ta;
+			b += tb;
+		}
+	}
+	if (lane == 63) {
+		wave_sums[wave][0] = a;
+		wave_sums[wave][1] = b;
+	}
+	__syncthreads();
+	unsigned long long before_a = 0, before_b = 0;
+	tlo = thi = 0;
+#pragma unroll
+	for (int w = 0; w < kCompactThreads / 64; ++w) {
+		const unsigned long long sa = wave_sums[w][0], sb = wave_sums[w][1];
+		before_a += w < wave ? sa : 0;
+		before_b += w < wave ? sb : 0;
+		tlo += sa;
+		thi += sb;
+	}
+	lo = before_a + a - lo;
+	hi = before_b + b - hi;
+}
 
 __global__ __launch_bounds__(kCompactThreads) void k_compact_block(uint8_t *__restrict__ flags, int ntiles,
                                                                    const uint8_t *__restrict__ colours, int colour,
@@ -1081,7 +1124,7 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact_block(uint8_t *__re
                                                                    const uint8_t *__restrict__ work)
 {
 	__shared__ int sums[2][kCompactThreads / 64];
-	__shared__ int class_sums[kWorkClasses][kCompactThreads / 64];
+	__shared__ unsigned long long class_sums[kCompactThreads / 64][2];
 	const int pieces = (ntiles + 32 * kCompactThreads - 1) / (32 * kCompactThreads);  // <= kCompactMaxPieces
 	const int base = (int)threadIdx.x * 32 * pieces;
 	uint32_t sel[kCompactMaxPieces];
@@ -1109,11 +1152,12 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact_block(uint8_t *__re
 			*count2 = total2;
 		}
 	}
+	const bool ordered = list && work && total > kOrderMin;  // uniform
 #pragma unroll
 	for (int q = 0; q < kCompactMaxPieces; ++q) {
 		for (uint32_t m = sel[q]; m; m &= m - 1) {
 			const int i = base + 32 * q + __ffs(m) - 1;
-			if (list && !work) {
+			if (list && !ordered) {
 				list[pos++] = i;
 			}
 			if (clear) {
@@ -1121,42 +1165,36 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact_block(uint8_t *__re
 			}
 		}
 	}
-	if (!list || !work) {
+	if (!ordered) {
 		return;
 	}
-	// work classes: class c holds work in [4 (kWorkClasses - 1 - c), ...), heaviest first
-	int per_class[kWorkClasses];
-#pragma unroll
-	for (int c = 0; c < kWorkClasses; ++c) {
-		per_class[c] = 0;
-	}
-	auto class_of = [](int w) {
-		const int b = w >> 2;
-		return kWorkClasses - 1 - (b < kWorkClasses - 1 ? b : kWorkClasses - 1);
-	};
+	// per-class counts, 16 bits each: classes 0-3 in lo, 4-7 in hi
+	unsigned long long lo = 0, hi = 0;
 #pragma unroll
 	for (int q = 0; q < kCompactMaxPieces; ++q) {
 		for (uint32_t m = sel[q]; m; m &= m - 1) {
-			const int c = class_of(work[base + 32 * q + __ffs(m) - 1]);
-#pragma unroll
-			for (int cc = 0; cc < kWorkClasses; ++cc) {
-				per_class[cc] += cc == c ? 1 : 0;
-			}
+			const int c = work_class(work[base + 32 * q + __ffs(m) - 1]);
+			const unsigned long long one = 1ull << (16 * (c & 3));
+			lo += c < 4 ? one : 0ull;
+			hi += c < 4 ? 0ull : one;
 		}
 	}
+	unsigned long long tlo = 0, thi = 0;
+	block_exclusive_scan_pair_1024(lo, hi, class_sums, tlo, thi);
+	// class c starts after the totals of the classes before it
 	int offset[kWorkClasses];
 	int start = 0;
 #pragma unroll
 	for (int c = 0; c < kWorkClasses; ++c) {
-		int class_total = 0;
-		offset[c] = start + block_exclusive_scan_1024(per_class[c], class_sums[c], class_total);
-		start += class_total;
+		const unsigned long long t = c < 4 ? tlo : thi, e = c < 4 ? lo : hi;
+		offset[c] = start + (int)((e >> (16 * (c & 3))) & 0xffffull);
+		start += (int)((t >> (16 * (c & 3))) & 0xffffull);
 	}
 #pragma unroll
 	for (int q = 0; q < kCompactMaxPieces; ++q) {
 		for (uint32_t m = sel[q]; m; m &= m - 1) {
 			const int i = base + 32 * q + __ffs(m) - 1;
-			const int c = class_of(work[i]);
+			const int c = work_class(work[i]);
 #pragma unroll
 			for (int cc = 0; cc < kWorkClasses; ++cc) {
 				if (cc == c) {
