@@ -137,6 +137,8 @@ private:
 	bool mStoppedEarly = false;
 	bool mLabelsExact = false;  // labels (T_min) come from an exact relabel of the final state
 	bool mClosedByCheck = false;  // the last solve ended at the closure test (its S is S_min)
+	bool mStatusPending = false;  // the init status is still to be read (with the next count_two)
+	void apply_status(int32_t status);
 	void ensure_exact_labels();
 	bool mHasBack = false;
 	bool mWideResiduals = true;  // some edge's capacity pair exceeds 32767: stage residuals as int32

@@ -81,7 +81,10 @@ typedef struct cgp_stats {
 	int64_t bfs_tile_visits;   /* grid: tiles processed by the global relabel */
 	int64_t discharge_tile_visits;
 	int64_t cut_tile_visits;
-	double init_ms, relabel_ms, discharge_ms, cut_ms, total_ms; /* host wall time per phase */
+	/* host wall time per phase; kernels run asynchronously, so a phase's time includes waiting for
+	   the device work queued before it (e.g. the init kernel's time shows in relabel_ms, the
+	   discharge phases' in cut_ms) */
+	double init_ms, relabel_ms, discharge_ms, cut_ms, total_ms;
 	int64_t voxels;            /* vertices (grid: voxels) */
 	int32_t tile_x, tile_y, tile_z, conn;
 	int32_t reserved[8];
