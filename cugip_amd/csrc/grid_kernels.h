@@ -1000,8 +1000,10 @@ __global__ __launch_bounds__(256) void k_compact_tiles(uint8_t *__restrict__ fla
 // written rather than accumulated (no fill beforehand). With flags2 set, a second flag array (no
 // colour filter, no list) is counted into count2 in the same pass.
 constexpr int kCompactThreads = 1024;
-constexpr int kCompactMaxPieces = 1;  // more pieces per thread measured slower than the multi-workgroup path
-constexpr int kCompactBlockTiles = 32 * kCompactThreads * kCompactMaxPieces;
+// one piece per thread for up to 32768 tiles; up to eight (262144 tiles) only where the order by
+// work pays for the slower pass (more pieces measured slower than the multi-workgroup path)
+constexpr int kCompactBlockTiles = 32 * kCompactThreads;
+constexpr int kCompactOrderedTiles = 8 * kCompactBlockTiles;
 
 __device__ inline int block_exclusive_scan_1024(int v, int *wave_sums, int &total)
 {
@@ -1115,6 +1117,7 @@ __device__ inline void block_exclusive_scan_pair_1024(unsigned long long &lo, un
 	hi = before_b + b - hi;
 }
 
+template <int kCompactMaxPieces>
 __global__ __launch_bounds__(kCompactThreads) void k_compact_block(uint8_t *__restrict__ flags, int ntiles,
                                                                    const uint8_t *__restrict__ colours, int colour,
                                                                    int clear, int32_t *__restrict__ list,
@@ -1152,7 +1155,8 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact_block(uint8_t *__re
 			*count2 = total2;
 		}
 	}
-	const bool ordered = list && work && total > kOrderMin;  // uniform
+	// uniform; the packed class counts hold 16 bits, so longer lists keep tile order
+	const bool ordered = list && work && total > kOrderMin && total <= 0xffff;
 #pragma unroll
 	for (int q = 0; q < kCompactMaxPieces; ++q) {
 		for (uint32_t m = sel[q]; m; m &= m - 1) {

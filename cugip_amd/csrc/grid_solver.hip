@@ -603,11 +603,17 @@ void GridSolver::launch_init()
 
 void GridSolver::launch_compact(uint8_t *flags, int colour, bool clear, const uint8_t *work)
 {
-	if (mTiles <= kCompactBlockTiles) {
+	if (mTiles <= kCompactBlockTiles || (work && mTiles <= kCompactOrderedTiles)) {
 		launch(kCompact, 0, 0.0, [&] {
-			hipLaunchKernelGGL(k_compact_block, dim3(1), dim3(kCompactThreads), 0, mStream, flags, mTiles,
-			                   mTileColour.get(), colour, clear ? 1 : 0, mList.get(), mCount.get(),
-			                   (const uint8_t *)nullptr, (int32_t *)nullptr, work);
+			if (mTiles <= kCompactBlockTiles) {
+				hipLaunchKernelGGL(k_compact_block<1>, dim3(1), dim3(kCompactThreads), 0, mStream, flags, mTiles,
+				                   mTileColour.get(), colour, clear ? 1 : 0, mList.get(), mCount.get(),
+				                   (const uint8_t *)nullptr, (int32_t *)nullptr, work);
+			} else {
+				hipLaunchKernelGGL(k_compact_block<8>, dim3(1), dim3(kCompactThreads), 0, mStream, flags, mTiles,
+				                   mTileColour.get(), colour, clear ? 1 : 0, mList.get(), mCount.get(),
+				                   (const uint8_t *)nullptr, (int32_t *)nullptr, work);
+			}
 		});
 		return;
 	}
@@ -643,7 +649,7 @@ void GridSolver::count_two(uint8_t *a, uint8_t *b, int &na, int &nb)
 	}
 	if (mTiles <= kCompactBlockTiles) {
 		launch(kCompact, 0, 0.0, [&] {
-			hipLaunchKernelGGL(k_compact_block, dim3(1), dim3(kCompactThreads), 0, mStream, a, mTiles,
+			hipLaunchKernelGGL(k_compact_block<1>, dim3(1), dim3(kCompactThreads), 0, mStream, a, mTiles,
 			                   mTileColour.get(), -1, 0, (int32_t *)nullptr, mCount.get() + 1, (const uint8_t *)b,
 			                   mCount.get() + 2, (const uint8_t *)nullptr);
 		});
