@@ -401,6 +401,45 @@ const InstanceDesc &GridSolver::instance(int index) const
 	return mInst[(size_t)index];
 }
 
+// Colours of every tile inside its instance: face parity (tx + ty + tz) & 1, or with corners the
+// parity per axis, so that no two tiles of a colour share a face, edge or corner.
+void GridSolver::build_colours(ColourSet &set, int ncolours, bool corners)
+{
+	std::vector<uint8_t> colour((size_t)mTiles);
+	std::vector<int> per_colour(ncolours, 0);
+	for (const InstanceDesc &d : mInst) {
+		const int n = d.ntx * d.nty * d.ntz;
+		for (int t = 0; t < n; ++t) {
+			const int tx = t % d.ntx, ty = (t / d.ntx) % d.nty, tz = t / (d.ntx * d.nty);
+			const int c = corners ? ((tx & 1) | ((ty & 1) << 1) | ((tz & 1) << 2)) : ((tx + ty + tz) & 1);
+			colour[(size_t)(d.tbase + t)] = (uint8_t)c;
+			++per_colour[c];
+		}
+	}
+	// largest colour class: the grid of a pass must cover every tile of one colour
+	set.max_size = *std::max_element(per_colour.begin(), per_colour.end());
+	set.colours = ncolours;
+	set.colour.allocate(((size_t)mTiles + 31) / 32 * 32);
+	CGP_HIP(hipMemcpyAsync(set.colour.get(), colour.data(), colour.size(), hipMemcpyHostToDevice, mStream));
+	set.start.assign(ncolours, 0);
+	set.size.assign(per_colour.begin(), per_colour.end());
+	for (int c = 1; c < ncolours; ++c) {
+		set.start[c] = set.start[c - 1] + set.size[c - 1];
+	}
+	std::vector<int32_t> grouped((size_t)mTiles + ncolours);
+	std::vector<int> fill(set.start);
+	for (int t = 0; t < mTiles; ++t) {
+		grouped[(size_t)fill[colour[(size_t)t]]++] = t;
+	}
+	for (int c = 0; c < ncolours; ++c) {
+		grouped[(size_t)mTiles + c] = set.size[c];
+	}
+	set.tiles.allocate(grouped.size());
+	CGP_HIP(hipMemcpyAsync(set.tiles.get(), grouped.data(), grouped.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+	                       mStream));
+	CGP_HIP(hipStreamSynchronize(mStream));  // the host vectors go out of scope
+}
+
 void GridSolver::configure_tiles(const Shape &shape)
 {
 	if (mTilesReady && mReadyWave == mWave && mReadyShape.tx == shape.tx && mReadyShape.ty == shape.ty &&
@@ -409,8 +448,6 @@ void GridSolver::configure_tiles(const Shape &shape)
 	}
 	mTilesReady = false;
 	mShape = shape;
-	const int ncolours = mConn == 6 ? 2 : 8;
-	std::vector<int> per_colour(ncolours, 0);
 	int64_t tiles = 0;
 	for (InstanceDesc &d : mInst) {
 		d.ntx = (d.nx + shape.tx - 1) / shape.tx;
@@ -426,23 +463,14 @@ void GridSolver::configure_tiles(const Shape &shape)
 	mNty = mInst[0].nty;
 	mNtz = mInst[0].ntz;
 	mTiles = (int)tiles;
-	std::vector<uint8_t> colour((size_t)mTiles);
 	std::vector<int32_t> owner(mInst.size() > 1 ? (size_t)mTiles : 0);
 	for (size_t i = 0; i < mInst.size(); ++i) {
 		const InstanceDesc &d = mInst[i];
 		const int n = d.ntx * d.nty * d.ntz;
-		for (int t = 0; t < n; ++t) {
-			const int tx = t % d.ntx, ty = (t / d.ntx) % d.nty, tz = t / (d.ntx * d.nty);
-			const int c = mConn == 6 ? ((tx + ty + tz) & 1) : ((tx & 1) | ((ty & 1) << 1) | ((tz & 1) << 2));
-			colour[(size_t)(d.tbase + t)] = (uint8_t)c;
-			++per_colour[c];
-			if (!owner.empty()) {
-				owner[(size_t)(d.tbase + t)] = (int32_t)i;
-			}
+		for (int t = 0; t < n && !owner.empty(); ++t) {
+			owner[(size_t)(d.tbase + t)] = (int32_t)i;
 		}
 	}
-	// largest colour class: the grid of a discharge phase must cover every tile of one colour
-	mMaxColourTiles = *std::max_element(per_colour.begin(), per_colour.end());
 	// flag arrays padded to whole 32-byte chunks: the one-workgroup compaction reads them 16 bytes at a time
 	const int64_t padded = ((int64_t)mTiles + 31) / 32 * 32;
 	mActive.allocate(padded);
@@ -451,35 +479,14 @@ void GridSolver::configure_tiles(const Shape &shape)
 	mDirtyB.allocate(padded);
 	mList.allocate(mTiles);
 	mAllTiles.allocate(mTiles + 1);
-	mTileColour.allocate(padded);
 	std::vector<int32_t> all(mTiles + 1);
 	for (int t = 0; t < mTiles; ++t) {
 		all[t] = t;
 	}
 	all[mTiles] = mTiles;  // count slot for the all-tiles list
 	CGP_HIP(hipMemcpyAsync(mAllTiles.get(), all.data(), all.size() * sizeof(int32_t), hipMemcpyHostToDevice, mStream));
-	CGP_HIP(hipMemcpyAsync(mTileColour.get(), colour.data(), colour.size(), hipMemcpyHostToDevice, mStream));
-	// tiles grouped by colour (the first sweep of a relabel or closure visits them colour by colour)
-	mColours = ncolours;
-	mColourStart.assign(ncolours, 0);
-	mColourSize.assign(per_colour.begin(), per_colour.end());
-	for (int c = 1; c < ncolours; ++c) {
-		mColourStart[c] = mColourStart[c - 1] + mColourSize[c - 1];
-	}
-	{
-		std::vector<int32_t> grouped((size_t)mTiles + ncolours);
-		std::vector<int> fill(mColourStart);
-		for (int t = 0; t < mTiles; ++t) {
-			grouped[(size_t)fill[colour[(size_t)t]]++] = t;
-		}
-		for (int c = 0; c < ncolours; ++c) {
-			grouped[(size_t)mTiles + c] = mColourSize[c];
-		}
-		mColourTiles.allocate(grouped.size());
-		CGP_HIP(hipMemcpyAsync(mColourTiles.get(), grouped.data(), grouped.size() * sizeof(int32_t),
-		                       hipMemcpyHostToDevice, mStream));
-		CGP_HIP(hipStreamSynchronize(mStream));
-	}
+	build_colours(mPhase, mConn == 6 ? 2 : 8, mConn != 6);
+	build_colours(mSweep, 2, false);
 	// workgroups of the voxel-parallel init (init_blocks per grid)
 	{
 		std::vector<int32_t> first(mInst.size());
@@ -601,17 +608,17 @@ void GridSolver::launch_init()
 	});
 }
 
-void GridSolver::launch_compact(uint8_t *flags, int colour, bool clear, const uint8_t *work)
+void GridSolver::launch_compact(const ColourSet &set, uint8_t *flags, int colour, bool clear, const uint8_t *work)
 {
 	if (mTiles <= kCompactBlockTiles || (work && mTiles <= kCompactOrderedTiles)) {
 		launch(kCompact, 0, 0.0, [&] {
 			if (mTiles <= kCompactBlockTiles) {
 				hipLaunchKernelGGL(k_compact_block<1>, dim3(1), dim3(kCompactThreads), 0, mStream, flags, mTiles,
-				                   mTileColour.get(), colour, clear ? 1 : 0, mList.get(), mCount.get(),
+				                   set.colour.get(), colour, clear ? 1 : 0, mList.get(), mCount.get(),
 				                   (const uint8_t *)nullptr, (int32_t *)nullptr, work);
 			} else {
 				hipLaunchKernelGGL(k_compact_block<8>, dim3(1), dim3(kCompactThreads), 0, mStream, flags, mTiles,
-				                   mTileColour.get(), colour, clear ? 1 : 0, mList.get(), mCount.get(),
+				                   set.colour.get(), colour, clear ? 1 : 0, mList.get(), mCount.get(),
 				                   (const uint8_t *)nullptr, (int32_t *)nullptr, work);
 			}
 		});
@@ -622,7 +629,7 @@ void GridSolver::launch_compact(uint8_t *flags, int colour, bool clear, const ui
 	});
 	launch(kCompact, 0, 0.0, [&] {
 		hipLaunchKernelGGL(k_compact_tiles, dim3(blocks_for(mTiles, 256, 1 << 20)), dim3(256), 0, mStream, flags,
-		                   mTiles, mTileColour.get(), colour, clear ? 1 : 0, mList.get(), mCount.get());
+		                   mTiles, set.colour.get(), colour, clear ? 1 : 0, mList.get(), mCount.get());
 	});
 }
 
@@ -650,7 +657,7 @@ void GridSolver::count_two(uint8_t *a, uint8_t *b, int &na, int &nb)
 	if (mTiles <= kCompactBlockTiles) {
 		launch(kCompact, 0, 0.0, [&] {
 			hipLaunchKernelGGL(k_compact_block<1>, dim3(1), dim3(kCompactThreads), 0, mStream, a, mTiles,
-			                   mTileColour.get(), -1, 0, (int32_t *)nullptr, mCount.get() + 1, (const uint8_t *)b,
+			                   mSweep.colour.get(), -1, 0, (int32_t *)nullptr, mCount.get() + 1, (const uint8_t *)b,
 			                   mCount.get() + 2, (const uint8_t *)nullptr);
 		});
 		CGP_HIP(hipMemcpyAsync(pinned, mCount.get() + 1, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, mStream));
@@ -667,10 +674,10 @@ void GridSolver::count_two(uint8_t *a, uint8_t *b, int &na, int &nb)
 	});
 	launch(kCompact, 0, 0.0, [&] {
 		const int blocks = blocks_for(mTiles, 256, 1 << 20);
-		hipLaunchKernelGGL(k_compact_tiles, dim3(blocks), dim3(256), 0, mStream, a, mTiles, mTileColour.get(), -1, 0,
+		hipLaunchKernelGGL(k_compact_tiles, dim3(blocks), dim3(256), 0, mStream, a, mTiles, mSweep.colour.get(), -1, 0,
 		                   (int32_t *)nullptr, mCount.get() + 1);
 		if (b) {
-			hipLaunchKernelGGL(k_compact_tiles, dim3(blocks), dim3(256), 0, mStream, b, mTiles, mTileColour.get(), -1,
+			hipLaunchKernelGGL(k_compact_tiles, dim3(blocks), dim3(256), 0, mStream, b, mTiles, mSweep.colour.get(), -1,
 			                   0, (int32_t *)nullptr, mCount.get() + 2);
 		}
 	});
@@ -695,15 +702,16 @@ int GridSolver::colour_sweeps(uint8_t *dirty, uint8_t *also_count, int &also, Pa
 {
 	constexpr int kCyclesPerCheck = 2;
 	int launches = 0;
-	for (int c = 0; c < mColours; ++c) {
-		if (mColourSize[c] == 0) {
+	const ColourSet &cs = mSweep;
+	for (int c = 0; c < cs.colours; ++c) {
+		if (cs.size[c] == 0) {
 			continue;  // e.g. a grid one tile thick has no tile of the odd colours
 		}
-		pass(mColourTiles.get() + mColourStart[c], mColourTiles.get() + mTiles + c, mColourSize[c], true);
+		pass(cs.tiles.get() + cs.start[c], cs.tiles.get() + mTiles + c, cs.size[c], true);
 		++launches;
-		if (stop && mColours > 2 && c + 1 < mColours) {
-			// a test that fails usually fails in its first pass: with many colours (26C), read the
-			// flag after each one
+		if (stop && mConn != 6 && c + 1 < cs.colours) {
+			// a test that fails usually fails in its first pass; 26C solves take several rounds, so
+			// their tests fail often: read the flag after each colour
 			int32_t *pinned = reinterpret_cast<int32_t *>(mHostScratch.get());
 			CGP_HIP(hipMemcpyAsync(pinned, mCount.get() + 3, sizeof(int32_t), hipMemcpyDeviceToHost, mStream));
 			CGP_HIP(hipStreamSynchronize(mStream));
@@ -728,9 +736,9 @@ int GridSolver::colour_sweeps(uint8_t *dirty, uint8_t *also_count, int &also, Pa
 			}
 		}
 		for (int cycle = 0; cycle < kCyclesPerCheck; ++cycle) {
-			for (int c = 0; c < mColours; ++c) {
-				launch_compact(dirty, c, true);
-				pass(mList.get(), mCount.get(), mMaxColourTiles, false);
+			for (int c = 0; c < cs.colours; ++c) {
+				launch_compact(cs, dirty, c, true);
+				pass(mList.get(), mCount.get(), cs.max_size, false);
 				++launches;
 			}
 		}
@@ -739,7 +747,7 @@ int GridSolver::colour_sweeps(uint8_t *dirty, uint8_t *also_count, int &also, Pa
 
 int GridSolver::count_flags(const uint8_t *flags, int colour)
 {
-	launch_compact(const_cast<uint8_t *>(flags), colour);
+	launch_compact(mSweep, const_cast<uint8_t *>(flags), colour);
 	int32_t *pinned = reinterpret_cast<int32_t *>(mHostScratch.get());
 	CGP_HIP(hipMemcpyAsync(pinned, mCount.get(), sizeof(int32_t), hipMemcpyDeviceToHost, mStream));
 	CGP_HIP(hipStreamSynchronize(mStream));
@@ -811,9 +819,9 @@ void GridSolver::run_discharge_phase(int colour, int max_iters)
 {
 	GridView g = CGP_VIEW();
 	// heaviest tiles first (work estimates of the last relabel; the wave engine has none)
-	launch_compact(mActive.get(), colour, false, mWave ? nullptr : mTileWork.get());
+	launch_compact(mPhase, mActive.get(), colour, false, mWave ? nullptr : mTileWork.get());
 	// every tile of one colour; blocks beyond the compacted count exit at once
-	const int blocks = mMaxColourTiles;
+	const int blocks = mPhase.max_size;
 	if (mWave) {
 		// reads and writes e, rt, r[6] and the bit planes; writes the face labels
 		launch(kDischarge, -1, 8.0 * (mConn + 2) + 2.0 + 4.0 * wave::kFaces / wave::NV, [&] {

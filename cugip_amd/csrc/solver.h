@@ -96,7 +96,16 @@ private:
 	void launch_init();
 	int run_global_relabel(int32_t threshold, cgp_stats &stats);
 	int count_flags(const uint8_t *flags, int colour);
-	void launch_compact(uint8_t *flags, int colour, bool clear = false, const uint8_t *work = nullptr);
+	// A colouring of the tiles: per-tile colour, the tile ids grouped by colour (then one count per
+	// colour), the groups' starts and sizes, and the largest group.
+	struct ColourSet {
+		DeviceBuffer<uint8_t> colour;  // padded like the flag arrays
+		DeviceBuffer<int32_t> tiles;
+		std::vector<int> start, size;
+		int colours = 0, max_size = 0;
+	};
+	void launch_compact(const ColourSet &set, uint8_t *flags, int colour, bool clear = false,
+	                    const uint8_t *work = nullptr);
 	void count_two(uint8_t *a, uint8_t *b, int &na, int &nb);
 	template <typename Pass>
 	int colour_sweeps(uint8_t *dirty, uint8_t *also_count, int &also, Pass &&pass, bool *stop = nullptr);
@@ -123,12 +132,13 @@ private:
 	// configure_tiles' tables are kept while the grids, the tile shape and the engine stay the same
 	bool mTilesReady = false, mReadyWave = false;
 	Shape mReadyShape{0, 0, 0};
-	DeviceBuffer<uint8_t> mTileColour;         // colour of every tile inside its instance
-	DeviceBuffer<int32_t> mColourTiles;        // tile ids grouped by colour, then one count per colour
-	std::vector<int> mColourStart, mColourSize;
-	int mColours = 0;
+
 	Shape mShape{0, 0, 0};
-	int mNtx = 0, mNty = 0, mNtz = 0, mTiles = 0, mMaxColourTiles = 0;
+	// Discharge phases need that no two tiles of a colour touch (2 colours for 6C, 8 for 26C); the
+	// relabel and closure sweeps only that a pass sees its face neighbours' results (face parity).
+	ColourSet mPhase, mSweep;
+	void build_colours(ColourSet &set, int ncolours, bool corners);
+	int mNtx = 0, mNty = 0, mNtz = 0, mTiles = 0;
 	bool mSolved = false;
 	bool mStarted = false;  // a solve has initialised the residual state
 	int64_t mCertificate = 0;
