@@ -27,6 +27,9 @@ struct ShapeTag {
 	F(6, 64, 4, 4)         \
 	F(6, 32, 4, 4)         \
 	F(6, 64, 4, 8)         \
+	F(6, 16, 8, 4)         \
+	F(6, 16, 4, 4)         \
+	F(6, 32, 2, 4)         \
 	F(26, 32, 4, 4)        \
 	F(26, 16, 8, 4)        \
 	F(26, 16, 4, 4)

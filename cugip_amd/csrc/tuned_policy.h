@@ -18,11 +18,11 @@ struct TunedPolicy {
 
 static const TunedPolicy kTunedPolicies[] = {
 	// conn batch max_voxels  tile        phases iters
-	{6, 0, 2097152LL, 64, 4, 4, 4, 16},  // grid6_small: 2.95 ms
-	{6, 0, INT64_MAX, 32, 4, 4, 8, 16},  // grid6_large: 5.68 ms
-	{6, 1, INT64_MAX, 32, 4, 4, 8, 16},  // batch6: 3.56 ms
-	{26, 0, INT64_MAX, 16, 8, 4, 16, 2},  // grid26: 6.87 ms
-	{26, 1, INT64_MAX, 32, 4, 4, 24, 2},  // batch26: 6.65 ms
+	{6, 0, 2097152LL, 32, 2, 4, 12, 1},  // grid6_small: 2.69 ms
+	{6, 0, INT64_MAX, 16, 8, 4, 8, 16},  // grid6_large: 5.48 ms
+	{6, 1, INT64_MAX, 16, 8, 4, 8, 16},  // batch6: 3.32 ms
+	{26, 0, INT64_MAX, 16, 8, 4, 16, 2},  // grid26: 5.37 ms
+	{26, 1, INT64_MAX, 16, 8, 4, 24, 1},  // batch26: 5.08 ms
 };
 
 inline TunedPolicy tuned_policy(int conn, bool batch, int64_t voxels_per_grid)
