@@ -694,32 +694,65 @@ __global__ __launch_bounds__(kTX *kTY, 6) void k_grid_discharge(GridView gv, con
 					Dirty::touch(dm, i);
 				}
 				const uint32_t a = adm.get(k);
+				// sends f along direction d: the own residual, and the neighbour's excess and reverse
+				// residual inside the tile, or the face accumulator (6C) beyond it
+				auto send = [&](int d, int32_t rv, int32_t f) {
+					rl[d][i] = (RT)(rv - f);
+					Dirty::set(dm, i, d);
+					const int nx = lx + D::dx(d), ny = ly + D::dy(d), nz = k + D::dz(d);
+					if (nx >= 0 && nx < T::TX && ny >= 0 && ny < T::TY && nz >= 0 && nz < T::TZ) {
+						const int j = i + T::voff(d);
+						rl[opposite<kConn>(d)][j] = (RT)(rl[opposite<kConn>(d)][j] + f);
+						Dirty::set(dm, j, opposite<kConn>(d));
+						atomicAdd(&el[j], f);
+						next = level - 1 > next ? level - 1 : next;
+					} else if (kConn == 6) {
+						if (d == 0 || d == 3) {
+							out_x[k] += f;
+						} else if (d == 1 || d == 4) {
+							out_y[k] += f;
+						} else if (d == 2) {
+							out_zlo += f;
+						} else {
+							out_zhi += f;
+						}
+					}
+				};
+				if (kConn == 6) {
 #pragma unroll
-				for (int d = 0; d < K; ++d) {
-					if (ex > 0 && (a & (1u << d))) {
-						const int32_t rv = rl[d][i];
-						if (rv > 0) {
-							const int32_t f = ex < rv ? ex : rv;
-							rl[d][i] = (RT)(rv - f);
-							ex -= f;
-							Dirty::set(dm, i, d);
-							const int nx = lx + D::dx(d), ny = ly + D::dy(d), nz = k + D::dz(d);
-							if (nx >= 0 && nx < T::TX && ny >= 0 && ny < T::TY && nz >= 0 && nz < T::TZ) {
-								const int j = i + T::voff(d);
-								rl[opposite<kConn>(d)][j] = (RT)(rl[opposite<kConn>(d)][j] + f);
-								Dirty::set(dm, j, opposite<kConn>(d));
-								atomicAdd(&el[j], f);
-								next = level - 1 > next ? level - 1 : next;
-							} else if (kConn == 6) {
-								if (d == 0 || d == 3) {
-									out_x[k] += f;
-								} else if (d == 1 || d == 4) {
-									out_y[k] += f;
-								} else if (d == 2) {
-									out_zlo += f;
-								} else {
-									out_zhi += f;
-								}
+					for (int d = 0; d < K; ++d) {
+						if (ex > 0 && (a & (1u << d))) {
+							const int32_t rv = rl[d][i];
+							if (rv > 0) {
+								const int32_t f = ex < rv ? ex : rv;
+								ex -= f;
+								send(d, rv, f);
+							}
+						}
+					}
+				} else {
+					// 26 directions: the admissible residuals of a group are read together and split
+					// greedily in registers (the same split, direction by direction), so a voxel waits
+					// for one LDS latency per group instead of one per direction
+					constexpr int kGroup = 7;
+#pragma unroll
+					for (int d0 = 0; d0 < K; d0 += kGroup) {
+						if (ex <= 0 || !(a & (((1u << kGroup) - 1u) << d0))) {
+							continue;
+						}
+						int32_t rv[kGroup];
+#pragma unroll
+						for (int j = 0; j < kGroup; ++j) {
+							const int d = d0 + j;
+							rv[j] = (d < K && (a & (1u << d))) ? (int32_t)rl[d < K ? d : 0][i] : 0;
+						}
+#pragma unroll
+						for (int j = 0; j < kGroup; ++j) {
+							const int d = d0 + j;
+							const int32_t f = ex < rv[j] ? ex : rv[j];
+							if (d < K && f > 0) {
+								ex -= f;
+								send(d, rv[j], f);
 							}
 						}
 					}
