@@ -31,7 +31,9 @@ WORKLOADS = {
     "grid6_large": (6, False, None, lambda: [cugip_amd.make_instance(256, 256, 256, s)[1:] for s in (1, 2, 3)]),
     "batch6": (6, True, None, lambda: [cugip_amd.make_instance(nx, ny, nz, 1, i, 6)[1:]
                                        for i, (nz, ny, nx) in enumerate(cugip_amd.batch_shapes(128, seed=1))]),
-    "grid26": (26, False, None, lambda: [cugip_amd.make_instance(160, 160, 128, 3, 0, 26)[1:]]),
+    "grid26_small": (26, False, 1 << 22, lambda: [cugip_amd.make_instance(160, 160, 128, s, 0, 26)[1:]
+                                                  for s in (3, 4)]),
+    "grid26": (26, False, None, lambda: [cugip_amd.make_instance(256, 256, 192, s, 0, 26)[1:] for s in (3, 4)]),
     "batch26": (26, True, None, lambda: [cugip_amd.make_instance(nx, ny, nz, 2, i, 26)[1:]
                                          for i, (nz, ny, nx) in enumerate(cugip_amd.batch_shapes(24, 2, 24, 64))]),
 }
