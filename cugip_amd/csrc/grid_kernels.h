@@ -997,35 +997,6 @@ __global__ __launch_bounds__(256) void k_sum_slots(const unsigned long long *__r
 	total[i] = sum;
 }
 
-// ---------------------------------------------------------------------------------------------
-// Compact the indices of flagged tiles (optionally of one colour) with a wave64 ballot and one
-// atomic per wave. colours[t] is the colour of tile t inside its instance: 6-connected
-// (tx+ty+tz) & 1, 26-connected (tx&1) | (ty&1)<<1 | (tz&1)<<2.
-// With clear set, the selected flags are reset (the pass that takes the list consumes them); with
-// list null, only the count is taken.
-__global__ __launch_bounds__(256) void k_compact_tiles(uint8_t *__restrict__ flags, int ntiles,
-                                                       const uint8_t *__restrict__ colours, int colour, int clear,
-                                                       int32_t *__restrict__ list, int32_t *__restrict__ count)
-{
-	const int t = blockIdx.x * blockDim.x + threadIdx.x;
-	const bool pred = t < ntiles && flags[t] && (colour < 0 || colours[t] == colour);
-	const unsigned long long ballot = __ballot(pred);
-	const int lane = threadIdx.x & 63;
-	int base = 0;
-	if (lane == 0 && ballot) {
-		base = atomicAdd(count, (int)__popcll(ballot));
-	}
-	base = __shfl(base, 0, 64);
-	if (pred) {
-		if (list) {
-			list[base + (int)__popcll(ballot & ((1ull << lane) - 1ull))] = t;
-		}
-		if (clear) {
-			flags[t] = 0;
-		}
-	}
-}
-
 // The same compaction by one workgroup, for up to kCompactBlockTiles tiles: thread t takes the
 // chunk [t·32m, (t+1)·32m) (m = ceil(ntiles / 32768)) in pieces of 32 tiles, each read with two
 // 16-byte loads per array (the flag arrays are padded to whole 32-byte pieces); a workgroup scan of
@@ -1062,6 +1033,54 @@ __device__ inline int block_exclusive_scan_1024(int v, int *wave_sums, int &tota
 		total += s;
 	}
 	return before + inc - v;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Compact the indices of flagged tiles (optionally of one colour) into a list, for any number of
+// tiles: workgroups of 1024 threads take 4096 tiles (4 per thread, one 4-byte load per array; the
+// flag arrays are padded to whole 32-byte pieces), scan their selections and reserve their part
+// of the list with one atomic each (per-wave atomics on the one count serialise in the L2). The
+// count must be zero beforehand. colours[t] is the colour of tile t inside its instance. With
+// clear set, the selected flags are reset (the pass that takes the list consumes them); with list
+// null, only the count is taken.
+constexpr int kCompactTilesPerThread = 4;
+constexpr int kCompactTilesPerGroup = kCompactTilesPerThread * kCompactThreads;
+
+__global__ __launch_bounds__(kCompactThreads) void k_compact_tiles(uint8_t *__restrict__ flags, int ntiles,
+                                                                   const uint8_t *__restrict__ colours, int colour,
+                                                                   int clear, int32_t *__restrict__ list,
+                                                                   int32_t *__restrict__ count)
+{
+	__shared__ int sums[kCompactThreads / 64];
+	__shared__ int base;
+	const int t0 = ((int)blockIdx.x * kCompactThreads + (int)threadIdx.x) * kCompactTilesPerThread;
+	uint32_t sel = 0;
+	if (t0 < ntiles) {
+		const uint32_t f = *reinterpret_cast<const uint32_t *>(flags + t0);
+		const uint32_t c = colour >= 0 ? *reinterpret_cast<const uint32_t *>(colours + t0) : 0u;
+#pragma unroll
+		for (int j = 0; j < kCompactTilesPerThread; ++j) {
+			const bool on = ((f >> (8 * j)) & 0xffu) != 0u &&
+			                (colour < 0 || ((c >> (8 * j)) & 0xffu) == (uint32_t)colour) && t0 + j < ntiles;
+			sel |= (on ? 1u : 0u) << j;
+		}
+	}
+	int total = 0;
+	int pos = block_exclusive_scan_1024(__popc(sel), sums, total);
+	if (threadIdx.x == 0) {
+		base = total ? atomicAdd(count, total) : 0;
+	}
+	__syncthreads();
+	pos += base;
+	for (uint32_t m = sel; m; m &= m - 1) {
+		const int i = t0 + __ffs(m) - 1;
+		if (list) {
+			list[pos++] = i;
+		}
+		if (clear) {
+			flags[i] = 0;
+		}
+	}
 }
 
 // bit j set where byte j of the 16 bytes q is nonzero (nonzero) or equals colour

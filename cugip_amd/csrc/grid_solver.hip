@@ -631,7 +631,8 @@ void GridSolver::launch_compact(const ColourSet &set, uint8_t *flags, int colour
 		hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, mStream, mCount.get(), (int64_t)1, 0);
 	});
 	launch(kCompact, 0, 0.0, [&] {
-		hipLaunchKernelGGL(k_compact_tiles, dim3(blocks_for(mTiles, 256, 1 << 20)), dim3(256), 0, mStream, flags,
+		hipLaunchKernelGGL(k_compact_tiles, dim3((mTiles + kCompactTilesPerGroup - 1) / kCompactTilesPerGroup),
+		                   dim3(kCompactThreads), 0, mStream, flags,
 		                   mTiles, set.colour.get(), colour, clear ? 1 : 0, mList.get(), mCount.get());
 	});
 }
@@ -676,11 +677,11 @@ void GridSolver::count_two(uint8_t *a, uint8_t *b, int &na, int &nb)
 		hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, mStream, mCount.get() + 1, (int64_t)2, 0);
 	});
 	launch(kCompact, 0, 0.0, [&] {
-		const int blocks = blocks_for(mTiles, 256, 1 << 20);
-		hipLaunchKernelGGL(k_compact_tiles, dim3(blocks), dim3(256), 0, mStream, a, mTiles, mSweep.colour.get(), -1, 0,
+		const int blocks = (mTiles + kCompactTilesPerGroup - 1) / kCompactTilesPerGroup;
+		hipLaunchKernelGGL(k_compact_tiles, dim3(blocks), dim3(kCompactThreads), 0, mStream, a, mTiles, mSweep.colour.get(), -1, 0,
 		                   (int32_t *)nullptr, mCount.get() + 1);
 		if (b) {
-			hipLaunchKernelGGL(k_compact_tiles, dim3(blocks), dim3(256), 0, mStream, b, mTiles, mSweep.colour.get(), -1,
+			hipLaunchKernelGGL(k_compact_tiles, dim3(blocks), dim3(kCompactThreads), 0, mStream, b, mTiles, mSweep.colour.get(), -1,
 			                   0, (int32_t *)nullptr, mCount.get() + 2);
 		}
 	});
