@@ -767,6 +767,17 @@ __global__ __launch_bounds__(kTX *kTY, 6) void k_grid_discharge(GridView gv, con
 			__syncthreads();
 			level = (int)(key & 0xffffffffull);
 		}
+		// when the sweep left no excess in the tile, another region relabel would only find that
+		// out; the labels in hl stay valid (pushes along admissible edges keep them valid) and are
+		// what the write-back stores
+		int left = 0;
+#pragma unroll
+		for (int k = 0; k < T::TZ; ++k) {
+			left |= ((valid >> k) & 1u) && el[T::vidx(lx, ly, k)] > 0;
+		}
+		if (!__syncthreads_or(left)) {
+			break;
+		}
 	}
 
 	// ---- write back what changed; pushes that left the tile go to the neighbouring voxels
