@@ -200,6 +200,34 @@ def test_errors():
         cugip_amd.GridGraph().set_grid(ones, ones, cn, conn=8)
 
 
+@pytest.mark.parametrize("conn", [6, 26])
+def test_capacity_errors_found_by_the_solve(conn):
+    """The init kernel's checks (negative capacity, int32 headroom) reach the caller as errors, also
+    though the solver reads them only with the relabel's first count; the handle then solves a
+    valid grid as usual."""
+    say(f"grid capacity errors conn {conn}")
+    vol, cs, ct, cn = cugip_amd.make_instance(40, 24, 16, 2, 0, conn)
+    flow = oracle.grid_solve("bk", 40, 24, 16, conn, cs, ct, cn)[0]
+    with cugip_amd.GridGraph() as g:
+        bad = cs.copy()
+        bad.reshape(-1)[123] = -1
+        g.set_grid(bad, ct, cn, conn=conn)
+        with pytest.raises(cugip_amd.CgpError) as e:
+            g.max_flow()
+        assert e.value.code == cugip_amd.CGP_ERR_INVALID
+        with pytest.raises(cugip_amd.CgpError):
+            g.source_set()  # no result after a failed solve
+        big, zero = cs.copy(), ct.copy()
+        big.reshape(-1)[77] = 2 ** 31 - 1  # plus the incoming n-links: beyond int32
+        zero.reshape(-1)[77] = 0
+        g.set_grid(big, zero, cn, conn=conn)
+        with pytest.raises(cugip_amd.CgpError) as e:
+            g.max_flow()
+        assert e.value.code == cugip_amd.CGP_ERR_OVERFLOW
+        g.set_grid(cs, ct, cn, conn=conn)
+        assert g.max_flow() == flow
+
+
 def test_round_cap_reports_not_converged():
     say("grid round cap")
     vol, cs, ct, cn = cugip_amd.make_instance(64, 64, 32, seed=1)
