@@ -617,7 +617,8 @@ __global__ __launch_bounds__(kTX *kTY, 6) void k_grid_discharge(GridView gv, con
 			}
 			open[k] = o;
 		}
-		__syncthreads();  // all reads of the previous sweep's labels are done
+		// (no barrier before these writes: the last reads of hl, the admissible-direction pass of
+		// the previous iteration, are separated from them by the sweep's barriers)
 #pragma unroll
 		for (int k = 0; k < T::TZ; ++k) {
 			if (valid & (1u << k)) {
