@@ -430,6 +430,28 @@ def test_cut_capacity_entry():
             assert g.cut_capacity(1 - g.fill_segments(1, 0)) == flow
 
 
+def test_6c_above_the_one_workgroup_compaction():
+    """A 6-connected grid with more tiles than one workgroup compacts (61440 tiles of the 16x8x4
+    shape): the relabel and closure lists take the multi-workgroup compaction, the discharge lists
+    the ordered pieces path. Capacities on the device from the volume; S_min and V minus T_min are
+    both minimum cuts, nested, and the flow matches a solve with 32x4x4 tiles (122880 tiles)."""
+    nx, ny, nz = 384, 320, 256
+    say("grid 6C 384x320x256")
+    vol = cugip_amd.generate_volume(nx, ny, nz, seed=3)
+    with cugip_amd.GridGraph() as g:
+        g.set_grid_volume(vol, conn=6)
+        g.set_policy(tile_x=16, tile_y=8, tile_z=4)
+        flow = g.max_flow()
+        smin = g.source_set()
+        tmin = g.fill_segments(1, 0)
+        assert g.cut_capacity(smin) == flow
+        assert g.cut_capacity(1 - tmin) == flow
+        assert not (smin.astype(bool) & tmin.astype(bool)).any()
+        g.set_policy(tile_x=32, tile_y=4, tile_z=4)
+        assert g.max_flow() == flow
+        assert digest(g.source_set()) == digest(smin)
+
+
 def test_full_size_26c_512x512x384():
     """The 26-connected 512x512x384 workload (about 100 M voxels, capacities computed on the device
     from the volume): the S_min and the V minus T_min cuts both have the flow's capacity (so both are
