@@ -166,8 +166,10 @@ int cgp_max_flow(cgp_graph graph, int64_t *flow, cgp_stats *stats);
 int cgp_fill_segments(cgp_graph graph, uint8_t *out, uint8_t mask, uint8_t background);
 /* canonical minimum cut S_min as 0/1 bytes (host buffer of n bytes) */
 int cgp_source_set(cgp_graph graph, uint8_t *out);
-/* final vertex labels (BFS distance to t, or CGP_LABEL_UNREACHABLE); debug view of
-   Graph<TFlow>::mLabels, graph_cut.hpp:135 */
+/* final vertex labels (exact BFS distance to t in the final residual graph, or
+   CGP_LABEL_UNREACHABLE); debug view of Graph<TFlow>::mLabels, graph_cut.hpp:135. A grid solve
+   ends by testing the S_min closure, not with a global relabel, so the first call after a solve
+   (like cgp_fill_segments) runs one exact relabel. */
 #define CGP_LABEL_UNREACHABLE 0x3fffffff
 int cgp_labels(cgp_graph graph, int32_t *out);
 
@@ -176,8 +178,8 @@ int cgp_labels(cgp_graph graph, int32_t *out);
    (benchmarks/graph_cut/graph_cut.cu:82-177: saturated edges, excess, labels), and the GraphML
    dump of Graph::save_to_graphml (advanced_operations/graph_to_graphml.hpp:6-61). */
 /* State of grid `index` (0 for a single grid) once a solve has started, e.g. from a trace hook:
-   excess (int32), labels (int32, CGP_LABEL_UNREACHABLE when t is unreachable; the exact labels of
-   the last global relabel) and saturated directions (uint32 bit d set when the edge to the
+   excess (int32), labels (int32, CGP_LABEL_UNREACHABLE when t is unreachable; during a solve the
+   labels as they stand, after it the exact labels of the final residual graph) and saturated directions (uint32 bit d set when the edge to the
    neighbour in residual direction d has capacity > 0 and residual 0; d < conn/2 are the offsets of
    cap_n, d >= conn/2 their mirrors). Any output may be NULL. */
 int cgp_grid_state(cgp_graph graph, int32_t index, int32_t *excess, int32_t *labels, uint32_t *saturated);
