@@ -252,10 +252,10 @@ void GridSolver::allocate_grids(int conn, const std::vector<InstanceDesc> &inst,
 	mFlow.allocate(count * kSlotWords);
 	mFlowTotal.allocate(count);
 	mCounters.allocate(3 * kSlotWords);
-	mCounterTotal.allocate(1);
+	mCounterTotal.allocate(3);
 	mStatus.allocate(1);
 	mCount.allocate(4);  // [0] list count, [1] [2] count_two, [3] closure reached t
-	mHostScratch.allocate(count > 8 ? count : 8);
+	mHostScratch.allocate(count + 3 > 8 ? count + 3 : 8);  // flows + visit counters
 }
 
 void GridSolver::set_batch(int conn, const GridSpec *specs, int count, bool device_pointers)
@@ -728,9 +728,6 @@ int GridSolver::colour_sweeps(uint8_t *dirty, uint8_t *also_count, int &also, Pa
 	for (;;) {
 		int pending = 0;
 		count_two(dirty, also_count, pending, also);
-		if (pending == 0) {
-			return launches;
-		}
 		if (stop) {
 			// the sweeps' own flag (the closure reaching t), read with the counts
 			const int32_t *pinned = reinterpret_cast<const int32_t *>(mHostScratch.get());
@@ -738,6 +735,9 @@ int GridSolver::colour_sweeps(uint8_t *dirty, uint8_t *also_count, int &also, Pa
 				*stop = true;
 				return launches;
 			}
+		}
+		if (pending == 0) {
+			return launches;
 		}
 		for (int cycle = 0; cycle < kCyclesPerCheck; ++cycle) {
 			for (int c = 0; c < cs.colours; ++c) {
@@ -885,13 +885,7 @@ bool GridSolver::run_source_closure(cgp_stats &stats, bool check)
 			    });
 		    },
 		    check ? &hit : nullptr);
-		if (check && !hit) {
-			// the sweeps ended without a stop: read the flag of the last passes
-			int32_t *pinned = reinterpret_cast<int32_t *>(mHostScratch.get());
-			CGP_HIP(hipMemcpyAsync(pinned, mCount.get() + 3, sizeof(int32_t), hipMemcpyDeviceToHost, mStream));
-			CGP_HIP(hipStreamSynchronize(mStream));
-			hit = pinned[0] != 0;
-		}
+		// (the last count of the sweeps read the flag after their last pass)
 		return hit;
 	}
 	launch(kFill, mVox, 0.125, [&] {
@@ -938,7 +932,9 @@ unsigned long long GridSolver::read_counter(int kind)
 	return pinned[0];
 }
 
-int64_t GridSolver::read_flow()
+// Flow of every grid (summed into mFlows) and, with visits set, the three visit counters, with one
+// host wait.
+int64_t GridSolver::read_flow(unsigned long long *visits)
 {
 	const size_t n = mInst.size();
 	unsigned long long *pinned = reinterpret_cast<unsigned long long *>(mHostScratch.get());
@@ -946,7 +942,18 @@ int64_t GridSolver::read_flow()
 	                   mFlowTotal.get());
 	CGP_LAUNCH_CHECK();
 	CGP_HIP(hipMemcpyAsync(pinned, mFlowTotal.get(), n * sizeof(unsigned long long), hipMemcpyDeviceToHost, mStream));
+	if (visits) {
+		hipLaunchKernelGGL(k_sum_slots, dim3(1), dim3(64), 0, mStream, mCounters.get(), (int64_t)3, mCounterTotal.get());
+		CGP_LAUNCH_CHECK();
+		CGP_HIP(hipMemcpyAsync(pinned + n, mCounterTotal.get(), 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+		                       mStream));
+	}
 	CGP_HIP(hipStreamSynchronize(mStream));
+	if (visits) {
+		visits[0] = pinned[n];
+		visits[1] = pinned[n + 1];
+		visits[2] = pinned[n + 2];
+	}
 	int64_t total = 0;
 	for (size_t i = 0; i < n; ++i) {
 		mFlows[i] = (int64_t)pinned[i];
@@ -1070,11 +1077,12 @@ int64_t GridSolver::solve(const cgp_policy &policy, cgp_stats *stats_out, const 
 	if (!mClosedByCheck) {
 		run_source_closure(stats);
 	}
-	const int64_t flow = read_flow();
+	unsigned long long visits[3];
+	const int64_t flow = read_flow(visits);
 	stats.cut_ms += ms_since(t0);
-	stats.discharge_tile_visits = (int64_t)read_counter(0);
-	stats.bfs_tile_visits = (int64_t)read_counter(1);
-	stats.cut_tile_visits = (int64_t)read_counter(2);
+	stats.discharge_tile_visits = (int64_t)visits[0];
+	stats.bfs_tile_visits = (int64_t)visits[1];
+	stats.cut_tile_visits = (int64_t)visits[2];
 	stats.flow = flow;
 	stats.total_ms = ms_since(t_start);
 	stats.voxels = mVox;

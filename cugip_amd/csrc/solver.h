@@ -113,7 +113,7 @@ private:
 	// check: also report whether the closure reached a voxel with a residual sink link (the
 	// preflow is then not maximum and the closure is not S_min); it stops early when it does
 	bool run_source_closure(cgp_stats &stats, bool check = false);
-	int64_t read_flow();
+	int64_t read_flow(unsigned long long *visits = nullptr);
 	unsigned long long read_counter(int kind);
 	const InstanceDesc &instance(int index) const;
 	std::vector<InstanceDesc> describe(int conn, const GridSpec *specs, int count);
