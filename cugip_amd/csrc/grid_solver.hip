@@ -725,19 +725,23 @@ int GridSolver::colour_sweeps(uint8_t *dirty, uint8_t *also_count, int &also, Pa
 			}
 		}
 	}
-	for (;;) {
-		int pending = 0;
-		count_two(dirty, also_count, pending, also);
-		if (stop) {
-			// the sweeps' own flag (the closure reaching t), read with the counts
-			const int32_t *pinned = reinterpret_cast<const int32_t *>(mHostScratch.get());
-			if (pinned[2]) {
-				*stop = true;
+	// the first sweep always leaves dirty tiles (the first colour ran before its neighbours), so the
+	// first cycles follow it without a check (one host wait fewer)
+	for (bool check = false;; check = true) {
+		if (check) {
+			int pending = 0;
+			count_two(dirty, also_count, pending, also);
+			if (stop) {
+				// the sweeps' own flag (the closure reaching t), read with the counts
+				const int32_t *pinned = reinterpret_cast<const int32_t *>(mHostScratch.get());
+				if (pinned[2]) {
+					*stop = true;
+					return launches;
+				}
+			}
+			if (pending == 0) {
 				return launches;
 			}
-		}
-		if (pending == 0) {
-			return launches;
 		}
 		for (int cycle = 0; cycle < kCyclesPerCheck; ++cycle) {
 			for (int c = 0; c < cs.colours; ++c) {
