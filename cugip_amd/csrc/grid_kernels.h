@@ -1112,20 +1112,25 @@ __device__ inline uint32_t byte_mask16(uint4 q, int colour, bool nonzero)
 __device__ inline void compact_piece(const uint8_t *flags, const uint8_t *colours, int colour, const uint8_t *flags2,
                                      int lo, int ntiles, uint32_t &sel, uint32_t &sel2)
 {
+	// all loads first: one memory latency for the three arrays
 	const uint4 *f = reinterpret_cast<const uint4 *>(flags + lo);
 	const uint4 f0 = f[0], f1 = f[1];
-	sel = byte_mask16(f0, 0, true) | (byte_mask16(f1, 0, true) << 16);
+	uint4 c0 = {}, c1 = {}, g0 = {}, g1 = {};
 	if (colour >= 0) {
 		const uint4 *c = reinterpret_cast<const uint4 *>(colours + lo);
-		const uint4 c0 = c[0], c1 = c[1];
-		sel &= byte_mask16(c0, colour, false) | (byte_mask16(c1, colour, false) << 16);
+		c0 = c[0];
+		c1 = c[1];
 	}
-	sel2 = 0;
 	if (flags2) {
 		const uint4 *f2 = reinterpret_cast<const uint4 *>(flags2 + lo);
-		const uint4 g0 = f2[0], g1 = f2[1];
-		sel2 = byte_mask16(g0, 0, true) | (byte_mask16(g1, 0, true) << 16);
+		g0 = f2[0];
+		g1 = f2[1];
 	}
+	sel = byte_mask16(f0, 0, true) | (byte_mask16(f1, 0, true) << 16);
+	if (colour >= 0) {
+		sel &= byte_mask16(c0, colour, false) | (byte_mask16(c1, colour, false) << 16);
+	}
+	sel2 = flags2 ? byte_mask16(g0, 0, true) | (byte_mask16(g1, 0, true) << 16) : 0u;
 	const int n = ntiles - lo;
 	if (n < 32) {
 		sel &= (1u << n) - 1u;
@@ -1146,6 +1151,27 @@ __device__ inline int work_class(int w)
 	const int b = w >> 2;
 	return kWorkClasses - 1 - (b < kWorkClasses - 1 ? b : kWorkClasses - 1);
 }
+
+// The work bytes of the 32 tiles of a piece, read with two 16-byte loads (one dependent load per
+// listed tile would put a memory latency on every step of the loops over the selection bits).
+struct PieceWork {
+	uint4 lo, hi;
+	__device__ PieceWork(const uint8_t *work, int first)
+	{
+		const uint4 *p = reinterpret_cast<const uint4 *>(work + first);
+		lo = p[0];
+		hi = p[1];
+	}
+	// work class of tile j (0 <= j < 32) of the piece
+	__device__ int cls(int j) const
+	{
+		const int d = j >> 2;
+		const uint32_t a = (d & 1) ? lo.y : lo.x, b = (d & 1) ? lo.w : lo.z;
+		const uint32_t c = (d & 1) ? hi.y : hi.x, e = (d & 1) ? hi.w : hi.z;
+		const uint32_t w = d < 4 ? ((d & 2) ? b : a) : ((d & 2) ? e : c);
+		return work_class((int)((w >> (8 * (j & 3))) & 0xffu));
+	}
+};
 
 // Exclusive scan of (lo, hi) pairs of packed counts over the workgroup; totals in tlo, thi.
 __device__ inline void block_exclusive_scan_pair_1024(unsigned long long &lo, unsigned long long &hi,
@@ -1240,8 +1266,12 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact_block(uint8_t *__re
 	unsigned long long lo = 0, hi = 0;
 #pragma unroll
 	for (int q = 0; q < kCompactMaxPieces; ++q) {
+		if (!sel[q]) {
+			continue;
+		}
+		const PieceWork pw(work, base + 32 * q);
 		for (uint32_t m = sel[q]; m; m &= m - 1) {
-			const int c = work_class(work[base + 32 * q + __ffs(m) - 1]);
+			const int c = pw.cls(__ffs(m) - 1);
 			const unsigned long long one = 1ull << (16 * (c & 3));
 			lo += c < 4 ? one : 0ull;
 			hi += c < 4 ? 0ull : one;
@@ -1260,9 +1290,13 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact_block(uint8_t *__re
 	}
 #pragma unroll
 	for (int q = 0; q < kCompactMaxPieces; ++q) {
+		if (!sel[q]) {
+			continue;
+		}
+		const PieceWork pw(work, base + 32 * q);
 		for (uint32_t m = sel[q]; m; m &= m - 1) {
 			const int i = base + 32 * q + __ffs(m) - 1;
-			const int c = work_class(work[i]);
+			const int c = pw.cls(__ffs(m) - 1);
 #pragma unroll
 			for (int cc = 0; cc < kWorkClasses; ++cc) {
 				if (cc == c) {
