@@ -1110,11 +1110,12 @@ __device__ inline uint32_t byte_mask16(uint4 q, int colour, bool nonzero)
 
 // selection masks of the 32 tiles from lo (lo < ntiles, a multiple of 32)
 __device__ inline void compact_piece(const uint8_t *flags, const uint8_t *colours, int colour, const uint8_t *flags2,
-                                     int lo, int ntiles, uint32_t &sel, uint32_t &sel2)
+                                     int lo, int ntiles, uint32_t &sel, uint32_t &sel2, uint4 &f0, uint4 &f1)
 {
 	// all loads first: one memory latency for the three arrays
 	const uint4 *f = reinterpret_cast<const uint4 *>(flags + lo);
-	const uint4 f0 = f[0], f1 = f[1];
+	f0 = f[0];
+	f1 = f[1];
 	uint4 c0 = {}, c1 = {}, g0 = {}, g1 = {};
 	if (colour >= 0) {
 		const uint4 *c = reinterpret_cast<const uint4 *>(colours + lo);
@@ -1152,26 +1153,28 @@ __device__ inline int work_class(int w)
 	return kWorkClasses - 1 - (b < kWorkClasses - 1 ? b : kWorkClasses - 1);
 }
 
-// The work bytes of the 32 tiles of a piece, read with two 16-byte loads (one dependent load per
-// listed tile would put a memory latency on every step of the loops over the selection bits).
-struct PieceWork {
-	uint4 lo, hi;
-	__device__ PieceWork(const uint8_t *work, int first)
-	{
-		const uint4 *p = reinterpret_cast<const uint4 *>(work + first);
-		lo = p[0];
-		hi = p[1];
+// Masks of the 32 tiles of a piece by work class (bit j of cm[c]: tile j is of class c), from
+// two 16-byte loads of their work bytes (one dependent load per listed tile would put a memory
+// latency on every step of the loops over the selection bits; the masks need no register array
+// indexed at run time).
+__device__ inline void class_masks(const uint8_t *work, int first, uint32_t (&cm)[kWorkClasses])
+{
+	const uint4 *p = reinterpret_cast<const uint4 *>(work + first);
+	const uint4 lo = p[0], hi = p[1];
+	const uint32_t w[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+#pragma unroll
+	for (int c = 0; c < kWorkClasses; ++c) {
+		cm[c] = 0;
 	}
-	// work class of tile j (0 <= j < 32) of the piece
-	__device__ int cls(int j) const
-	{
-		const int d = j >> 2;
-		const uint32_t a = (d & 1) ? lo.y : lo.x, b = (d & 1) ? lo.w : lo.z;
-		const uint32_t c = (d & 1) ? hi.y : hi.x, e = (d & 1) ? hi.w : hi.z;
-		const uint32_t w = d < 4 ? ((d & 2) ? b : a) : ((d & 2) ? e : c);
-		return work_class((int)((w >> (8 * (j & 3))) & 0xffu));
+#pragma unroll
+	for (int j = 0; j < 32; ++j) {
+		const int cls = work_class((int)((w[j >> 2] >> (8 * (j & 3))) & 0xffu));
+#pragma unroll
+		for (int c = 0; c < kWorkClasses; ++c) {
+			cm[c] |= (cls == c ? 1u : 0u) << j;
+		}
 	}
-};
+}
 
 // Exclusive scan of (lo, hi) pairs of packed counts over the workgroup; totals in tlo, thi.
 __device__ inline void block_exclusive_scan_pair_1024(unsigned long long &lo, unsigned long long &hi,
@@ -1207,6 +1210,35 @@ __device__ inline void block_exclusive_scan_pair_1024(unsigned long long &lo, un
 	hi = before_b + b - hi;
 }
 
+// q with byte j zeroed where bit j of bits is set
+__device__ inline uint4 zero_bytes16(uint4 q, uint32_t bits)
+{
+	uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+	for (int i = 0; i < 4; ++i) {
+		uint32_t keep = 0;
+#pragma unroll
+		for (int j = 0; j < 4; ++j) {
+			keep |= ((bits >> (4 * i + j)) & 1u) ? 0u : 0xffu << (8 * j);
+		}
+		w[i] &= keep;
+	}
+	return make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+// LDS written by lanes of this wave is read by other lanes of it after this point (a wave's LDS
+// operations complete in order; the fences keep the compiler from moving them across)
+__device__ inline void wave_lds_sync()
+{
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+	__builtin_amdgcn_wave_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// list entries staged per wave and pass, for waves with more than kDirect entries
+constexpr int kStage = 512;
+constexpr int kDirect = 256;
+
 template <int kCompactMaxPieces>
 __global__ __launch_bounds__(kCompactThreads) void k_compact_block(uint8_t *__restrict__ flags, int ntiles,
                                                                    const uint8_t *__restrict__ colours, int colour,
@@ -1221,6 +1253,9 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact_block(uint8_t *__re
 	const int pieces = (ntiles + 32 * kCompactThreads - 1) / (32 * kCompactThreads);  // <= kCompactMaxPieces
 	const int base = (int)threadIdx.x * 32 * pieces;
 	uint32_t sel[kCompactMaxPieces];
+	// one piece keeps its flag bytes for the clear
+	constexpr bool kHold = kCompactMaxPieces == 1;
+	uint4 held0 = {}, held1 = {};
 	int n = 0, n2 = 0;
 #pragma unroll
 	for (int q = 0; q < kCompactMaxPieces; ++q) {
@@ -1228,7 +1263,12 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact_block(uint8_t *__re
 		const int lo = base + 32 * q;
 		if (q < pieces && lo < ntiles) {
 			uint32_t s2;
-			compact_piece(flags, colours, colour, flags2, lo, ntiles, sel[q], s2);
+			uint4 f0, f1;
+			compact_piece(flags, colours, colour, flags2, lo, ntiles, sel[q], s2, f0, f1);
+			if constexpr (kHold) {
+				held0 = f0;
+				held1 = f1;
+			}
 			n += __popc(sel[q]);
 			n2 += __popc(s2);
 		}
@@ -1247,19 +1287,62 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact_block(uint8_t *__re
 	}
 	// uniform; the packed class counts hold 16 bits, so longer lists keep tile order
 	const bool ordered = list && work && total > kOrderMin && total <= 0xffff;
-#pragma unroll
-	for (int q = 0; q < kCompactMaxPieces; ++q) {
-		for (uint32_t m = sel[q]; m; m &= m - 1) {
-			const int i = base + 32 * q + __ffs(m) - 1;
-			if (list && !ordered) {
-				list[pos++] = i;
+	const int lane = (int)threadIdx.x & 63, wave = (int)threadIdx.x >> 6;
+	if (clear) {
+		if constexpr (kHold) {
+			// the piece written back whole, the selected bytes zeroed
+			if (sel[0]) {
+				uint4 *f = reinterpret_cast<uint4 *>(flags + base);
+				f[0] = zero_bytes16(held0, sel[0] & 0xffffu);
+				f[1] = zero_bytes16(held1, sel[0] >> 16);
 			}
-			if (clear) {
-				flags[i] = 0;
+		} else {
+#pragma unroll
+			for (int q = 0; q < kCompactMaxPieces; ++q) {
+				for (uint32_t m = sel[q]; m; m &= m - 1) {
+					flags[base + 32 * q + __ffs(m) - 1] = 0;
+				}
 			}
 		}
 	}
+	if (!list) {
+		return;
+	}
 	if (!ordered) {
+		const int first = __shfl(pos, 0, 64), mine = pos - first, wn = sums[0][wave];
+		if (wn <= kDirect) {
+#pragma unroll
+			for (int q = 0; q < kCompactMaxPieces; ++q) {
+				for (uint32_t m = sel[q]; m; m &= m - 1) {
+					list[pos++] = base + 32 * q + __ffs(m) - 1;
+				}
+			}
+			return;
+		}
+		// A wave with many entries places them in its stage in LDS, kStage at a time, and copies
+		// them out by consecutive lanes (a thread writing its own entries straight to the list
+		// puts every lane of a store on another cache line).
+		__shared__ int stage[kCompactThreads / 64][kStage];
+		int *st = stage[wave];
+		for (int c0 = 0; c0 < wn; c0 += kStage) {
+			if (n && mine < c0 + kStage && mine + n > c0) {
+				int k = mine;
+#pragma unroll
+				for (int q = 0; q < kCompactMaxPieces; ++q) {
+					for (uint32_t m = sel[q]; m; m &= m - 1, ++k) {
+						if (k >= c0 && k < c0 + kStage) {
+							st[k - c0] = base + 32 * q + __ffs(m) - 1;
+						}
+					}
+				}
+			}
+			wave_lds_sync();
+			const int cnt = wn - c0 < kStage ? wn - c0 : kStage;
+			for (int e = lane; e < cnt; e += 64) {
+				list[first + c0 + e] = st[e];
+			}
+			wave_lds_sync();
+		}
 		return;
 	}
 	// per-class counts, 16 bits each: classes 0-3 in lo, 4-7 in hi
@@ -1269,12 +1352,13 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact_block(uint8_t *__re
 		if (!sel[q]) {
 			continue;
 		}
-		const PieceWork pw(work, base + 32 * q);
-		for (uint32_t m = sel[q]; m; m &= m - 1) {
-			const int c = pw.cls(__ffs(m) - 1);
-			const unsigned long long one = 1ull << (16 * (c & 3));
-			lo += c < 4 ? one : 0ull;
-			hi += c < 4 ? 0ull : one;
+		uint32_t cm[kWorkClasses];
+		class_masks(work, base + 32 * q, cm);
+#pragma unroll
+		for (int c = 0; c < kWorkClasses; ++c) {
+			const unsigned long long k = (unsigned long long)__popc(cm[c] & sel[q]) << (16 * (c & 3));
+			lo += c < 4 ? k : 0ull;
+			hi += c < 4 ? 0ull : k;
 		}
 	}
 	unsigned long long tlo = 0, thi = 0;
@@ -1293,15 +1377,12 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact_block(uint8_t *__re
 		if (!sel[q]) {
 			continue;
 		}
-		const PieceWork pw(work, base + 32 * q);
-		for (uint32_t m = sel[q]; m; m &= m - 1) {
-			const int i = base + 32 * q + __ffs(m) - 1;
-			const int c = pw.cls(__ffs(m) - 1);
+		uint32_t cm[kWorkClasses];
+		class_masks(work, base + 32 * q, cm);
 #pragma unroll
-			for (int cc = 0; cc < kWorkClasses; ++cc) {
-				if (cc == c) {
-					list[offset[cc]++] = i;
-				}
+		for (int c = 0; c < kWorkClasses; ++c) {
+			for (uint32_t m = cm[c] & sel[q]; m; m &= m - 1) {
+				list[offset[c]++] = base + 32 * q + __ffs(m) - 1;
 			}
 		}
 	}
