@@ -1109,13 +1109,18 @@ __device__ inline uint32_t byte_mask16(uint4 q, int colour, bool nonzero)
 }
 
 // selection masks of the 32 tiles from lo (lo < ntiles, a multiple of 32)
+// (and with kKeep the flag bytes in keep0, keep1)
+template <bool kKeep>
 __device__ inline void compact_piece(const uint8_t *flags, const uint8_t *colours, int colour, const uint8_t *flags2,
-                                     int lo, int ntiles, uint32_t &sel, uint32_t &sel2, uint4 &f0, uint4 &f1)
+                                     int lo, int ntiles, uint32_t &sel, uint32_t &sel2, uint4 &keep0, uint4 &keep1)
 {
 	// all loads first: one memory latency for the three arrays
 	const uint4 *f = reinterpret_cast<const uint4 *>(flags + lo);
-	f0 = f[0];
-	f1 = f[1];
+	const uint4 f0 = f[0], f1 = f[1];
+	if constexpr (kKeep) {
+		keep0 = f0;
+		keep1 = f1;
+	}
 	uint4 c0 = {}, c1 = {}, g0 = {}, g1 = {};
 	if (colour >= 0) {
 		const uint4 *c = reinterpret_cast<const uint4 *>(colours + lo);
@@ -1175,6 +1180,27 @@ __device__ inline void class_masks(const uint8_t *work, int first, uint32_t (&cm
 		}
 	}
 }
+
+// The work bytes of a piece for the variant with several pieces a thread (per-class masks of every
+// piece would not fit its registers): the class of one tile at a time.
+struct PieceWork {
+	uint4 lo, hi;
+	__device__ PieceWork(const uint8_t *work, int first)
+	{
+		const uint4 *p = reinterpret_cast<const uint4 *>(work + first);
+		lo = p[0];
+		hi = p[1];
+	}
+	// work class of tile j (0 <= j < 32) of the piece
+	__device__ int cls(int j) const
+	{
+		const int d = j >> 2;
+		const uint32_t a = (d & 1) ? lo.y : lo.x, b = (d & 1) ? lo.w : lo.z;
+		const uint32_t c = (d & 1) ? hi.y : hi.x, e = (d & 1) ? hi.w : hi.z;
+		const uint32_t w = d < 4 ? ((d & 2) ? b : a) : ((d & 2) ? e : c);
+		return work_class((int)((w >> (8 * (j & 3))) & 0xffu));
+	}
+};
 
 // Exclusive scan of (lo, hi) pairs of packed counts over the workgroup; totals in tlo, thi.
 __device__ inline void block_exclusive_scan_pair_1024(unsigned long long &lo, unsigned long long &hi,
@@ -1263,15 +1289,11 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact_block(uint8_t *__re
 		const int lo = base + 32 * q;
 		if (q < pieces && lo < ntiles) {
 			uint32_t s2;
-			uint4 f0, f1;
-			compact_piece(flags, colours, colour, flags2, lo, ntiles, sel[q], s2, f0, f1);
-			if constexpr (kHold) {
-				held0 = f0;
-				held1 = f1;
-			}
+			compact_piece<kHold>(flags, colours, colour, flags2, lo, ntiles, sel[q], s2, held0, held1);
 			n += __popc(sel[q]);
 			n2 += __popc(s2);
 		}
+
 	}
 	int total = 0;
 	int pos = block_exclusive_scan_1024(n, sums[0], total);
@@ -1310,7 +1332,9 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact_block(uint8_t *__re
 	}
 	if (!ordered) {
 		const int first = __shfl(pos, 0, 64), mine = pos - first, wn = sums[0][wave];
-		if (wn <= kDirect) {
+		// (staged only with one piece a thread: with more, a thread's entries span many stage
+		// passes and it would walk its selection bits in every pass)
+		if (!kHold || wn <= kDirect) {
 #pragma unroll
 			for (int q = 0; q < kCompactMaxPieces; ++q) {
 				for (uint32_t m = sel[q]; m; m &= m - 1) {
@@ -1352,13 +1376,23 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact_block(uint8_t *__re
 		if (!sel[q]) {
 			continue;
 		}
-		uint32_t cm[kWorkClasses];
-		class_masks(work, base + 32 * q, cm);
+		if constexpr (kHold) {
+			uint32_t cm[kWorkClasses];
+			class_masks(work, base + 32 * q, cm);
 #pragma unroll
-		for (int c = 0; c < kWorkClasses; ++c) {
-			const unsigned long long k = (unsigned long long)__popc(cm[c] & sel[q]) << (16 * (c & 3));
-			lo += c < 4 ? k : 0ull;
-			hi += c < 4 ? 0ull : k;
+			for (int c = 0; c < kWorkClasses; ++c) {
+				const unsigned long long k = (unsigned long long)__popc(cm[c] & sel[q]) << (16 * (c & 3));
+				lo += c < 4 ? k : 0ull;
+				hi += c < 4 ? 0ull : k;
+			}
+		} else {
+			const PieceWork pw(work, base + 32 * q);
+			for (uint32_t m = sel[q]; m; m &= m - 1) {
+				const int c = pw.cls(__ffs(m) - 1);
+				const unsigned long long one = 1ull << (16 * (c & 3));
+				lo += c < 4 ? one : 0ull;
+				hi += c < 4 ? 0ull : one;
+			}
 		}
 	}
 	unsigned long long tlo = 0, thi = 0;
@@ -1377,12 +1411,25 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact_block(uint8_t *__re
 		if (!sel[q]) {
 			continue;
 		}
-		uint32_t cm[kWorkClasses];
-		class_masks(work, base + 32 * q, cm);
+		if constexpr (kHold) {
+			uint32_t cm[kWorkClasses];
+			class_masks(work, base + 32 * q, cm);
 #pragma unroll
-		for (int c = 0; c < kWorkClasses; ++c) {
-			for (uint32_t m = cm[c] & sel[q]; m; m &= m - 1) {
-				list[offset[c]++] = base + 32 * q + __ffs(m) - 1;
+			for (int c = 0; c < kWorkClasses; ++c) {
+				for (uint32_t m = cm[c] & sel[q]; m; m &= m - 1) {
+					list[offset[c]++] = base + 32 * q + __ffs(m) - 1;
+				}
+			}
+		} else {
+			const PieceWork pw(work, base + 32 * q);
+			for (uint32_t m = sel[q]; m; m &= m - 1) {
+				const int c = pw.cls(__ffs(m) - 1);
+#pragma unroll
+				for (int cc = 0; cc < kWorkClasses; ++cc) {
+					if (cc == c) {
+						list[offset[cc]++] = base + 32 * q + __ffs(m) - 1;
+					}
+				}
 			}
 		}
 	}
