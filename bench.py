@@ -216,6 +216,11 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--shape", type=int, nargs=3, metavar=("NX", "NY", "NZ"),
+                    help="grid dimensions (default: --size in every axis)")
+    ap.add_argument("--volume", action="store_true",
+                    help="capacities built on the device from a generated uint8 volume (the image front end), "
+                         "as for the 512x512x384 26-connected configuration; no CPU baseline or batch")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--conn", type=int, default=6)
     ap.add_argument("--cpu-runs", type=int, default=3)
@@ -250,16 +255,23 @@ def main():
         sys.exit(1)
 
     n = args.size
-    nx = ny = nz = n
+    nx, ny, nz = args.shape if args.shape else (n, n, n)
     vox = nx * ny * nz
     log(f"generating {nx}x{ny}x{nz} {args.conn}-connected instance seed {args.seed}")
-    vol, cs, ct, cn = cugip_amd.make_instance(nx, ny, nz, seed=args.seed, conn=args.conn)
-    if args.calibration_copies:
+    if args.volume:
+        vol = cugip_amd.generate_volume(nx, ny, nz, seed=args.seed)
+        cs = ct = cn = None
+    else:
+        vol, cs, ct, cn = cugip_amd.make_instance(nx, ny, nz, seed=args.seed, conn=args.conn)
+    if args.calibration_copies and cs is not None:
         log(f"calibration copies: {calibration_copies(cugip_amd, cs, ct, cn)} bytes")
 
     g = cugip_amd.GridGraph()
     t0 = time.perf_counter()
-    g.set_grid(cs, ct, cn, conn=args.conn)
+    if args.volume:
+        g.set_grid_volume(vol, conn=args.conn)
+    else:
+        g.set_grid(cs, ct, cn, conn=args.conn)
     h2d_ms = (time.perf_counter() - t0) * 1e3
 
     for i in range(args.warmup):
@@ -302,7 +314,10 @@ def main():
     achieved = dom["algorithmic_bytes"] / (dom["device_ms"] / 1e3) / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "hbm_traffic.json")
-    if os.path.exists(tpath):
+    # the counter runs behind that file measured the default workload only
+    default_workload = (not args.shape and not args.volume and n == ap.get_default("size")
+                        and args.conn == ap.get_default("conn") and args.seed == ap.get_default("seed"))
+    if default_workload and os.path.exists(tpath):
         try:
             tdata = json.load(open(tpath))
             traffic = tdata.get(dom["name"], {}).get("bytes_per_launch")
@@ -318,7 +333,7 @@ def main():
     batch = None
     learned = None
     learned_batched = None
-    if args.batch > 0:
+    if args.batch > 0 and not args.volume and nx == ny == nz == n:
         batch = batched_vs_plain(cugip_amd, n, args.conn, args.seed, args.batch)
         log(f"batched vs plain: {batch}")
         weights = os.path.join(ROOT, "cugip_amd", "cnn_weights.npy")
@@ -329,7 +344,7 @@ def main():
             learned_batched = round(batch["plain_ms"] / (batch["batched_ms"] + learned["inference_ms"]), 3)
 
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and not args.volume:
         cpu = cpu_baseline(nx, ny, nz, args.conn, cs, ct, cn, args.cpu_runs)
         if cpu["flow"] != flow:
             log(f"CPU baseline flow {cpu['flow']} differs from GPU flow {flow}")
