@@ -1014,7 +1014,9 @@ __global__ __launch_bounds__(256) void k_sum_slots(const unsigned long long *__r
 // 16-byte loads per array (the flag arrays are padded to whole 32-byte pieces); a workgroup scan of
 // the chunk counts places the chunks in order, so the list comes out sorted and the count is
 // written rather than accumulated (no fill beforehand). With flags2 set, a second flag array (no
-// colour filter, no list) is counted into count2 in the same pass.
+// colour filter, no list) is counted into count2 in the same pass. With one piece a thread, a wave
+// with more than kDirect list entries stores them through an LDS stage by consecutive lanes, and
+// consumed flags go back as whole 16-byte words.
 constexpr int kCompactThreads = 1024;
 // one piece per thread for up to 32768 tiles; up to eight (262144 tiles) only where the order by
 // work pays for the slower pass (more pieces measured slower than the multi-workgroup path)
