@@ -104,6 +104,91 @@ def batched_vs_plain(cugip_amd, n, conn, seed, count, repeats=3):
                 plain_ms=round(plain_s * 1e3, 3), bit_exact=bool(same))
 
 
+def host_cores():
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_batch_baseline(grids, threads):
+    """Boykov-Kolmogorov on every grid of the batch workload, one grid per thread at a time on
+    `threads` host threads (the native solver releases the GIL). value = voxels / the longest
+    per-thread sum of maxflow() times (graph construction excluded, as for the one-core leg);
+    wall_s is the pool's wall time including construction."""
+    import threading
+    from concurrent.futures import ThreadPoolExecutor
+
+    import oracle
+    busy = {}
+    flows = [None] * len(grids)
+
+    def one(i):
+        cs, ct, cn = grids[i]
+        nz, ny, nx = cs.shape
+        ea, eb, fwd, bwd = oracle.grid_edges(nx, ny, nz, 6, cn)
+        f, _, _, t = oracle.bk(nx * ny * nz, ea, eb, fwd, bwd, cs.reshape(-1), ct.reshape(-1))
+        flows[i] = f
+        tid = threading.get_ident()
+        busy[tid] = busy.get(tid, 0.0) + t
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as pool:
+        list(pool.map(one, range(len(grids))))
+    wall = time.perf_counter() - t0
+    vox = sum(g[0].size for g in grids)
+    longest = max(busy.values())
+    return dict(value=round(vox / longest / 1e6, 2), unit="Mvoxels/s", cores=threads, kind="port",
+                sample=f"all {len(grids)} grids of the batch workload, Boykov-Kolmogorov maxflow() only, one grid "
+                       f"per thread on {threads} threads",
+                seconds=round(longest, 3), wall_s=round(wall, 3), flows=flows)
+
+
+def batch_workload(cugip_amd, count, seed, repeats=3, cpu=True):
+    """The batch workload: `count` grids with edge lengths log-uniform in [32, 128] per axis and grid,
+    solved as one batch (device-driven engine, capacities resident in HBM) against the plain
+    single-grid solver run on each grid in turn; every grid's flow and S_min compared bit for bit.
+    With cpu, the all-cores Boykov-Kolmogorov baseline over the same grids."""
+    import numpy as np
+    grids = cugip_amd.make_batch(count, seed)
+    vox = sum(g[0].size for g in grids)
+    with cugip_amd.GridBatch() as b:
+        b.set_batch(grids, 6)
+        b.solve()  # first solve captures the round graph
+        ts = []
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            flows = b.solve()
+            ts.append(time.perf_counter() - t0)
+        stats = b.stats.as_dict()
+        batch_s = statistics.median(ts)
+        smin = [np.packbits(b.batch_source_set(i)) for i in range(count)]
+    plain_s = 0.0
+    same = True
+    with cugip_amd.GridGraph() as g:
+        for i, (cs, ct, cn) in enumerate(grids):
+            g.set_grid(cs, ct, cn)
+            g.max_flow()
+            tg = []
+            for _ in range(repeats):
+                t0 = time.perf_counter()
+                f = g.max_flow()
+                tg.append(time.perf_counter() - t0)
+            plain_s += statistics.median(tg)
+            same &= f == flows[i] and np.array_equal(np.packbits(g.source_set()), smin[i])
+    out = dict(workload=f"{count} grids, edge lengths log-uniform in [32, 128] per axis (seed {seed}), "
+                        f"6-connected, {vox} voxels",
+               batched_ms=round(batch_s * 1e3, 3), batched_mvox_s=round(vox / batch_s / 1e6, 1),
+               plain_ms=round(plain_s * 1e3, 3), plain_mvox_s=round(vox / plain_s / 1e6, 1),
+               speedup=round(plain_s / batch_s, 3), bit_exact=bool(same), replays=stats["graph_replays"],
+               rounds=stats["rounds"], tile=[stats["tile_x"], stats["tile_y"], stats["tile_z"]])
+    if cpu:
+        c = cpu_batch_baseline(grids, host_cores())
+        c["flows_match_gpu"] = c.pop("flows") == flows
+        out["cpu_all_cores"] = c
+    return out
+
+
 def learned_vs_plain(cugip_amd, n, seed, count, weights, repeats=3):
     """The learned cut proposal on the batched-comparison grids: per grid, inference + certificate
     (capacity of the predicted cut) + the certified solve, against the plain solve; flows and S_min
@@ -223,9 +308,14 @@ def main():
                          "as for the 512x512x384 26-connected configuration; no CPU baseline or batch")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--conn", type=int, default=6)
+    ap.add_argument("--engine", choices=["auto", "tiles", "graph"], default="auto",
+                    help="grid engine: host-driven loop (tiles), device-driven graph replay (graph), or the "
+                         "library's choice (auto)")
     ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch", type=int, default=8, help="grids of the batched-vs-plain comparison (0: skip)")
+    ap.add_argument("--batch-workload", type=int, default=512,
+                    help="grids of the batch workload leg, edge lengths log-uniform in [32, 128] (0: skip)")
     ap.add_argument("--calibration-copies", action="store_true",
                     help="start with device-to-device copies of known size (for rocprofv3 --pmc)")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
@@ -266,6 +356,8 @@ def main():
     if args.calibration_copies and cs is not None:
         log(f"calibration copies: {calibration_copies(cugip_amd, cs, ct, cn)} bytes")
 
+    engine = {"auto": cugip_amd.ENGINE_AUTO, "tiles": cugip_amd.ENGINE_TILES,
+              "graph": cugip_amd.ENGINE_GRAPH}[args.engine]
     g = cugip_amd.GridGraph()
     t0 = time.perf_counter()
     if args.volume:
@@ -273,6 +365,7 @@ def main():
     else:
         g.set_grid(cs, ct, cn, conn=args.conn)
     h2d_ms = (time.perf_counter() - t0) * 1e3
+    g.set_policy(engine=engine)
 
     for i in range(args.warmup):
         g.max_flow()
@@ -302,7 +395,9 @@ def main():
         pass
     log(f"verified against golden: {verified}")
 
-    # profiled solve: per-kernel device time with HIP events on the solver's stream
+    # profiled solve: per-kernel device time with HIP events on the solver's stream (per-launch
+    # events need the host-driven loop, which runs the same kernels)
+    g.set_policy()
     g.set_profiling(True)
     g.max_flow()
     prof = g.profile()
@@ -343,6 +438,12 @@ def main():
             # inference for every grid plus one batched solve, against the plain solves
             learned_batched = round(batch["plain_ms"] / (batch["batched_ms"] + learned["inference_ms"]), 3)
 
+    batch512 = None
+    if args.batch_workload > 0 and not args.volume and args.conn == 6:
+        log(f"batch workload: {args.batch_workload} grids")
+        batch512 = batch_workload(cugip_amd, args.batch_workload, args.seed, cpu=not args.no_cpu)
+        log(f"batch workload: {batch512}")
+
     cpu = None
     if not args.no_cpu and not args.volume:
         cpu = cpu_baseline(nx, ny, nz, args.conn, cs, ct, cn, args.cpu_runs)
@@ -379,6 +480,7 @@ def main():
             "batched_vs_plain": batch,
             "learned_vs_plain": learned,
             "learned_batched_speedup": learned_batched,
+            "batch512": batch512,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -32,7 +32,7 @@ CGP_ERR_STATE = 6
 LABEL_UNREACHABLE = 0x3FFFFFFF
 
 TRACE_STARTED, TRACE_BEGIN_ITERATION, TRACE_AFTER_RELABEL, TRACE_AFTER_PUSH, TRACE_FINISHED = range(5)
-ENGINE_AUTO, ENGINE_TILES, ENGINE_WAVE = 0, 1, 2
+ENGINE_AUTO, ENGINE_TILES, ENGINE_GRAPH = 0, 1, 2
 
 
 class Edge(ctypes.Structure):
@@ -64,7 +64,8 @@ class Stats(ctypes.Structure):
         ("cut_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
         ("voxels", ctypes.c_int64),
         ("tile_x", ctypes.c_int32), ("tile_y", ctypes.c_int32), ("tile_z", ctypes.c_int32), ("conn", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 8),
+        ("graph_replays", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 7),
     ]
 
     def as_dict(self):
@@ -147,6 +148,7 @@ _lib.cgp_set_grid_volume.argtypes = [_vp] + [ctypes.c_int32] * 4 + [_vp, _vp, ct
                                                                     ctypes.c_int32]
 _lib.cgp_grid_capacities.argtypes = [_vp, ctypes.c_int32, _i32p, _i32p, _i32p, _i32p]
 _lib.cgp_grid_state.argtypes = [_vp, ctypes.c_int32, _i32p, _i32p, ctypes.POINTER(ctypes.c_uint32)]
+_lib.cgp_grid_residuals.argtypes = [_vp, ctypes.c_int32, _i32p, _i32p]
 _lib.cgp_save_graphml.argtypes = [_vp, ctypes.c_char_p]
 _lib.cgp_cut_capacity.argtypes = [_vp, ctypes.c_int32, _u8p, ctypes.POINTER(ctypes.c_int64)]
 _lib.cgp_cnn_create.argtypes = [ctypes.POINTER(_vp)]
@@ -336,6 +338,16 @@ class _Handle:
         return e, lab, sat
 
 
+    def residuals(self, index=0):
+        """(residuals, sink_residual) of grid `index` once a solve has started: the conn planes of
+        the residual v -> v + o_d, shaped (conn, nz, ny, nx), and the residual v -> t, (nz, ny, nx)."""
+        shape = tuple(self.shapes[index]) if hasattr(self, "shapes") else tuple(self.shape)
+        r = np.empty((self.conn,) + shape, np.int32)
+        rt = np.empty(shape, np.int32)
+        self._check(_lib.cgp_grid_residuals(self._h, index, _p32(r), _p32(rt)))
+        return r, rt
+
+
 class Graph(_Handle):
     """Generic graph: cugip::Graph<TFlow> (set_vertex_count / set_nweights / set_tweights / max_flow /
     fill_segments, graph_cut.hpp:54-107)."""
@@ -356,8 +368,10 @@ class Graph(_Handle):
             arr = np.frombuffer(rec, dtype=np.uint32, count=2 * m).reshape(m, 2)
             arr[:, 0] = hi
             arr[:, 1] = lo
-        fwd = _i32(fwd)
-        bwd = _i32(bwd)
+        fwd = _i32(fwd).reshape(-1)
+        bwd = _i32(bwd).reshape(-1)
+        if fwd.size != m or bwd.size != m:
+            raise CgpError(CGP_ERR_INVALID, "fwd and bwd must hold one capacity per edge")
         self._check(_lib.cgp_set_nweights(self._h, m, rec, _p32(fwd), _p32(bwd)))
 
     def set_tweights(self, source, sink):
@@ -387,6 +401,7 @@ class GridGraph(_Handle):
                                       _p32(cb) if cb is not None else None))
         self._n = nx * ny * nz
         self.shape = (nz, ny, nx)
+        self.conn = conn
 
     def set_grid_volume(self, volume, markers=None, weights=None, conn=6):
         """Capacities computed on the device from a uint8 (nz, ny, nx) volume (and optional marker
@@ -703,3 +718,21 @@ def make_instance(nx, ny, nz, seed, instance=0, conn=6):
     vol = generate_volume(nx, ny, nz, seed, instance)
     cs, ct, cn = caps_from_volume(vol, conn)
     return vol, cs, ct, cn
+
+
+def make_batch(count, seed=0, conn=6, threads=None, lo=32, hi=128):
+    """The batch workload: `count` grids with edge lengths log-uniform in [lo, hi] per axis and grid
+    (batch_shapes), grid i being the synthetic instance (seed, i) of its shape. Returns a list of
+    (cap_source, cap_sink, cap_n). The generator runs in native code without the GIL, so the grids
+    are built by a pool of host threads."""
+    from concurrent.futures import ThreadPoolExecutor
+    shapes = batch_shapes(count, seed, lo, hi)
+
+    def one(i):
+        nz, ny, nx = shapes[i]
+        _, cs, ct, cn = make_instance(nx, ny, nz, seed, i, conn)
+        return cs, ct, cn
+
+    workers = threads or min(16, os.cpu_count() or 1)
+    with ThreadPoolExecutor(max_workers=workers) as pool:
+        return list(pool.map(one, range(count)))

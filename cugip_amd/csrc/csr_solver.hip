@@ -322,6 +322,10 @@ void CsrSolver::set_nweights(int m, const cgp_edge *edges, const int32_t *fwd, c
 	if (m < 0 || (m > 0 && (!edges || !fwd || !bwd))) {
 		fail(CGP_ERR_INVALID, "invalid edge arrays");
 	}
+	// every edge makes two arcs, and arc indices are int32
+	if (m >= (1 << 30)) {
+		fail(CGP_ERR_INVALID, "at most 2^30 - 1 edges are supported");
+	}
 	std::vector<int32_t> degree(mN + 1, 0);
 	for (int i = 0; i < m; ++i) {
 		const uint32_t a = edges[i].first, b = edges[i].second;

@@ -38,7 +38,7 @@ namespace {
 std::mutex g_create_error_mutex;
 std::string g_create_error;
 
-constexpr int32_t kAbiVersion = 1;
+constexpr int32_t kAbiVersion = 2;
 
 template <typename Fn>
 int guarded(cgp_graph g, Fn &&fn)
@@ -368,6 +368,11 @@ int cgp_grid_capacities(cgp_graph g, int32_t index, int32_t *cap_source, int32_t
 int cgp_grid_state(cgp_graph g, int32_t index, int32_t *excess, int32_t *labels, uint32_t *saturated)
 {
 	return guarded(g, [&] { grid_of(g).state(index, excess, labels, saturated); });
+}
+
+int cgp_grid_residuals(cgp_graph g, int32_t index, int32_t *residuals, int32_t *sink_residual)
+{
+	return guarded(g, [&] { grid_of(g).residuals(index, residuals, sink_residual); });
 }
 
 int cgp_cut_capacity(cgp_graph g, int32_t index, const uint8_t *source_side, int64_t *capacity)

@@ -21,6 +21,7 @@
 #pragma once
 
 #include "common.h"
+#include "grid_control.h"
 #include "grid_tile.h"
 
 namespace cgp {
@@ -51,11 +52,15 @@ __host__ __device__ inline int64_t init_blocks(int64_t nx, int64_t ny, int64_t n
 
 template <int kConn>
 __global__ __launch_bounds__(kInitThreads) void k_grid_init(GridView gv, const int32_t *__restrict__ block_inst,
-                                                           const int32_t *__restrict__ inst_first_block)
+                                                           const int32_t *__restrict__ inst_first_block,
+                                                           const SolveControl *ctl)
 {
 	using D = Dirs<kConn>;
 	constexpr int K = D::K, H = D::H;
 	__shared__ long long red[kInitThreads / 64];
+	if (!gate_open(ctl, kGateInit)) {
+		return;
+	}
 	int lb = blockIdx.x;
 	GridView g = gv;
 	if (gv.inst) {
@@ -527,7 +532,11 @@ __global__ __launch_bounds__(kTX *kTY, 6) void k_grid_discharge(GridView gv, con
 	__shared__ int32_t el[T::NV];
 	__shared__ RT rl[K][T::NV];
 	__shared__ uint32_t dm[Dirty::kWords(T::NV)];   // residual directions written during this visit
-	__shared__ unsigned long long key;
+	// (step << 32 | level) of the sweep's steps, double-buffered by step parity: a wave that has
+	// read step s's value and raced ahead writes step s + 1's into the other slot, so a slower wave
+	// still reading step s's slot cannot see it; step s + 2 reuses the slot only after the barrier
+	// of step s + 1, which every wave passes after its read of step s
+	__shared__ unsigned long long key[2];
 	__shared__ long long red[T::NT / 64];
 	__shared__ uint8_t marks[27];
 
@@ -537,8 +546,8 @@ __global__ __launch_bounds__(kTX *kTY, 6) void k_grid_discharge(GridView gv, con
 	if (tid < 27) {
 		marks[tid] = 0;
 	}
-	if (tid == 0) {
-		key = 0;
+	if (tid < 2) {
+		key[tid] = 0;
 	}
 	for (int w = tid; w < Dirty::kWords(T::NV); w += T::NT) {
 		dm[w] = 0;
@@ -660,10 +669,10 @@ __global__ __launch_bounds__(kTX *kTY, 6) void k_grid_discharge(GridView gv, con
 		++step;
 		top = wave_max_i32(top);
 		if ((tid & 63) == 0) {
-			atomicMax(&key, ((unsigned long long)step << 32) | (unsigned)top);
+			atomicMax(&key[step & 1], ((unsigned long long)step << 32) | (unsigned)top);
 		}
 		__syncthreads();
-		int level = (int)(key & 0xffffffffull);
+		int level = (int)(key[step & 1] & 0xffffffffull);
 		if (level == 0) {
 			break;
 		}
@@ -763,10 +772,10 @@ __global__ __launch_bounds__(kTX *kTY, 6) void k_grid_discharge(GridView gv, con
 			++step;
 			next = wave_max_i32(next);
 			if ((tid & 63) == 0) {
-				atomicMax(&key, ((unsigned long long)step << 32) | (unsigned)next);
+				atomicMax(&key[step & 1], ((unsigned long long)step << 32) | (unsigned)next);
 			}
 			__syncthreads();
-			level = (int)(key & 0xffffffffull);
+			level = (int)(key[step & 1] & 0xffffffffull);
 		}
 		// when the sweep left no excess in the tile, another region relabel would only find that
 		// out; the labels in hl stay valid (pushes along admissible edges keep them valid) and are
@@ -876,12 +885,13 @@ template <int kConn, int kTX, int kTY, int kTZ>
 __global__ __launch_bounds__(kTX *kTY) void k_grid_source_closure(GridView gv, const int32_t *__restrict__ list,
                                                                   const int32_t *__restrict__ count, int first_pass,
                                                                   uint8_t *__restrict__ dirty_next,
-                                                                  int32_t *__restrict__ reached)
+                                                                  int32_t *__restrict__ reached,
+                                                                  const SolveControl *ctl)
 {
 	using T = Tile<kConn, kTX, kTY, kTZ>;
 	using D = typename T::D;
 	const int tile = list[blockIdx.x];  // in bounds: the grid never exceeds the list's capacity
-	if ((int)blockIdx.x >= *count) {
+	if ((int)blockIdx.x >= *count || !gate_open(ctl, kGateClosure | kGateUnlessReached)) {
 		return;
 	}
 	int local;
@@ -1063,10 +1073,14 @@ constexpr int kCompactTilesPerGroup = kCompactTilesPerThread * kCompactThreads;
 __global__ __launch_bounds__(kCompactThreads) void k_compact_tiles(uint8_t *__restrict__ flags, int ntiles,
                                                                    const uint8_t *__restrict__ colours, int colour,
                                                                    int clear, int32_t *__restrict__ list,
-                                                                   int32_t *__restrict__ count)
+                                                                   int32_t *__restrict__ count,
+                                                                   const SolveControl *ctl, unsigned gate)
 {
 	__shared__ int sums[kCompactThreads / 64];
 	__shared__ int base;
+	if (!gate_open(ctl, gate)) {
+		return;  // the count stays at the zero it was filled with
+	}
 	const int t0 = ((int)blockIdx.x * kCompactThreads + (int)threadIdx.x) * kCompactTilesPerThread;
 	uint32_t sel = 0;
 	if (t0 < ntiles) {
@@ -1274,10 +1288,21 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact_block(uint8_t *__re
                                                                    int32_t *__restrict__ count,
                                                                    const uint8_t *__restrict__ flags2,
                                                                    int32_t *__restrict__ count2,
-                                                                   const uint8_t *__restrict__ work)
+                                                                   const uint8_t *__restrict__ work,
+                                                                   const SolveControl *ctl, unsigned gate)
 {
 	__shared__ int sums[2][kCompactThreads / 64];
 	__shared__ unsigned long long class_sums[kCompactThreads / 64][2];
+	if (!gate_open(ctl, gate)) {
+		// a closed gate lists nothing
+		if (threadIdx.x == 0) {
+			*count = 0;
+			if (flags2) {
+				*count2 = 0;
+			}
+		}
+		return;
+	}
 	const int pieces = (ntiles + 32 * kCompactThreads - 1) / (32 * kCompactThreads);  // <= kCompactMaxPieces
 	const int base = (int)threadIdx.x * 32 * pieces;
 	uint32_t sel[kCompactMaxPieces];
@@ -1437,6 +1462,166 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact_block(uint8_t *__re
 	}
 }
 
+// ---------------------------------------------------------------------------------------------
+// Controllers of the device-driven solve (single workgroups; SolveControl in grid_control.h).
+
+// Start of a device-driven solve (one thread).
+__global__ void k_ctl_reset(SolveControl *ctl, int32_t max_rounds)
+{
+	if (threadIdx.x == 0 && blockIdx.x == 0) {
+		SolveControl c = {};
+		c.mode = kModeInit;
+		c.max_rounds = max_rounds;
+		*ctl = c;
+	}
+}
+
+// Sets the stage (one thread): the host continues a solve that stopped for kModeNeedWide.
+__global__ void k_ctl_set_mode(SolveControl *ctl, int32_t mode)
+{
+	if (threadIdx.x == 0 && blockIdx.x == 0) {
+		ctl->mode = mode;
+	}
+}
+
+// After the init kernel: input errors fail the solve; residuals wider than the graph's discharge
+// variant stop it for the host to switch variants.
+__global__ void k_ctl_after_init(SolveControl *ctl, const int32_t *status, int wide_graph)
+{
+	if (threadIdx.x != 0 || blockIdx.x != 0 || ctl->mode != kModeInit) {
+		return;
+	}
+	const int32_t s = *status;
+	ctl->status = s;
+	if (s & 3) {
+		ctl->mode = kModeFailed;
+	} else if ((s & 4) && !wide_graph) {
+		ctl->mode = kModeNeedWide;
+	} else {
+		ctl->mode = kModeRelabelNew;
+	}
+}
+
+// Start of a colour-sweep stage: the first sweep visits every tile of both colours when the stage
+// starts from scratch (a mode in fresh_mask), none when it continues. sizes: the two colours' tile
+// counts.
+__global__ void k_ctl_stage_begin(SolveControl *ctl, unsigned fresh_mask, const int32_t *sizes)
+{
+	if (threadIdx.x != 0 || blockIdx.x != 0) {
+		return;
+	}
+	const bool fresh = (fresh_mask >> ctl->mode) & 1u;
+	ctl->first[0] = fresh ? sizes[0] : 0;
+	ctl->first[1] = fresh ? sizes[1] : 0;
+	if (fresh) {
+		ctl->reached = 0;
+	}
+}
+
+constexpr int kCtlThreads = 1024;
+
+// Number of nonzero bytes of flags[0, n) (padded to whole 16-byte words) over one workgroup, in
+// thread 0.
+__device__ inline int ctl_count_flags(const uint8_t *flags, int n, int *wave_sums)
+{
+	int c = 0;
+	const int words = (n + 15) / 16;
+	for (int w = (int)threadIdx.x; w < words; w += kCtlThreads) {
+		const uint4 q = reinterpret_cast<const uint4 *>(flags)[w];
+		const uint32_t v[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+		for (int j = 0; j < 16; ++j) {
+			const int i = w * 16 + j;
+			c += (i < n && ((v[j >> 2] >> (8 * (j & 3))) & 0xffu)) ? 1 : 0;
+		}
+	}
+#pragma unroll
+	for (int off = 32; off > 0; off >>= 1) {
+		c += __shfl_xor(c, off, 64);
+	}
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	if (lane == 0) {
+		wave_sums[wave] = c;
+	}
+	__syncthreads();
+	int total = 0;
+	if (threadIdx.x == 0) {
+		for (int w = 0; w < kCtlThreads / 64; ++w) {
+			total += wave_sums[w];
+		}
+	}
+	__syncthreads();
+	return total;
+}
+
+// End of the relabel stage: dirty tiles left continue it; otherwise a relabel that found no active
+// tile ends the discharge rounds (the S_min closure follows), and one that did starts the
+// discharge phases unless the round cap is reached.
+__global__ __launch_bounds__(kCtlThreads) void k_ctl_relabel_end(SolveControl *ctl, const uint8_t *dirty,
+                                                                 const uint8_t *active, int ntiles)
+{
+	__shared__ int sums[kCtlThreads / 64];
+	const int m = *(volatile const int32_t *)&ctl->mode;
+	if (m != kModeRelabelNew && m != kModeRelabelCont) {
+		return;
+	}
+	const int nd = ctl_count_flags(dirty, ntiles, sums);
+	const int na = ctl_count_flags(active, ntiles, sums);
+	if (threadIdx.x == 0) {
+		ctl->dirty = nd;
+		if (nd > 0) {
+			ctl->mode = kModeRelabelCont;
+			return;
+		}
+		ctl->active = na;
+		const int rounds = ctl->rounds + 1;
+		ctl->rounds = rounds;
+		if (na == 0) {
+			ctl->mode = kModeFinalNew;
+		} else if (rounds - 1 >= ctl->max_rounds) {
+			ctl->mode = kModeNotConverged;
+		} else {
+			ctl->mode = kModeDischarge;
+		}
+	}
+}
+
+// End of the discharge phases: the termination test follows.
+__global__ void k_ctl_discharge_end(SolveControl *ctl, int phases)
+{
+	if (threadIdx.x == 0 && blockIdx.x == 0 && ctl->mode == kModeDischarge) {
+		ctl->phases += phases;
+		ctl->mode = kModeClosureNew;
+	}
+}
+
+// End of a closure stage. The termination test that reached t goes back to a global relabel; one
+// that did not, with no dirty tile left, has found S_min (so has the final closure).
+__global__ __launch_bounds__(kCtlThreads) void k_ctl_closure_end(SolveControl *ctl, const uint8_t *dirty, int ntiles)
+{
+	__shared__ int sums[kCtlThreads / 64];
+	const int m = *(volatile const int32_t *)&ctl->mode;
+	if (m < kModeClosureNew || m > kModeFinalCont) {
+		return;
+	}
+	const bool check = closure_check_mode(m);
+	if (check && *(volatile const int32_t *)&ctl->reached) {
+		if (threadIdx.x == 0) {
+			ctl->mode = kModeRelabelNew;
+		}
+		return;
+	}
+	const int nd = ctl_count_flags(dirty, ntiles, sums);
+	if (threadIdx.x == 0) {
+		ctl->dirty = nd;
+		if (nd > 0) {
+			ctl->mode = check ? kModeClosureCont : kModeFinalCont;
+		} else {
+			ctl->mode = check ? kModeDoneClosed : kModeDoneExact;
+		}
+	}
+}
+
 // Image -> graph front end: t-links by intensity, then marker and border forcing; n-links by the
 // absolute intensity difference per neighbour class (the caller-side construction of
 // benchmarks/graph_cut/graph_cut.cpp:32-98, as table lookups).
@@ -1564,16 +1749,24 @@ __global__ __launch_bounds__(256) void k_cut_capacity(int nx, int ny, int nz, co
 	}
 }
 
-__global__ __launch_bounds__(256) void k_fill_u8(uint8_t *__restrict__ p, int64_t n, uint8_t value)
+__global__ __launch_bounds__(256) void k_fill_u8(uint8_t *__restrict__ p, int64_t n, uint8_t value,
+                                                 const SolveControl *ctl = nullptr, unsigned gate = 0)
 {
+	if (!gate_open(ctl, gate)) {
+		return;
+	}
 	const int64_t stride = (int64_t)gridDim.x * blockDim.x;
 	for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
 		p[i] = value;
 	}
 }
 
-__global__ __launch_bounds__(256) void k_fill_i32(int32_t *__restrict__ p, int64_t n, int32_t value)
+__global__ __launch_bounds__(256) void k_fill_i32(int32_t *__restrict__ p, int64_t n, int32_t value,
+                                                  const SolveControl *ctl = nullptr, unsigned gate = 0)
 {
+	if (!gate_open(ctl, gate)) {
+		return;
+	}
 	const int64_t stride = (int64_t)gridDim.x * blockDim.x;
 	for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
 		p[i] = value;

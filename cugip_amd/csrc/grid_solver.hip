@@ -7,7 +7,6 @@
 #include <cstring>
 
 #include "grid_kernels.h"
-#include "grid_wave.h"
 #include "solver.h"
 #include "tuned_policy.h"
 
@@ -144,22 +143,6 @@ int GridSolver::profile(cgp_kernel_profile *out, int capacity) const
 
 GridSolver::Shape GridSolver::pick_shape(const cgp_policy &policy)
 {
-	const bool wave_shape = (!policy.tile_x && !policy.tile_y && !policy.tile_z) ||
-	                           (policy.tile_x == wave::TX && policy.tile_y == wave::TY && policy.tile_z == wave::TZ);
-	if (policy.engine == CGP_ENGINE_WAVE) {
-		if (mInst.size() != 1) {
-			fail(CGP_ERR_INVALID, "the wave engine solves single grids, not batches");
-		}
-		if (mConn != 6 || !wave_shape) {
-			fail(CGP_ERR_INVALID, "the wave engine handles 6-connected grids with 32x8x4 tiles only");
-		}
-	} else if (policy.engine != CGP_ENGINE_AUTO && policy.engine != CGP_ENGINE_TILES) {
-		fail(CGP_ERR_INVALID, "unknown engine " + std::to_string(policy.engine));
-	}
-	mWave = policy.engine == CGP_ENGINE_WAVE;
-	if (mWave) {
-		return {wave::TX, wave::TY, wave::TZ};
-	}
 	if (policy.tile_x || policy.tile_y || policy.tile_z) {
 		if (!shape_supported(mConn, policy.tile_x, policy.tile_y, policy.tile_z)) {
 			fail(CGP_ERR_INVALID, "unsupported tile shape " + std::to_string(policy.tile_x) + "x" +
@@ -235,6 +218,9 @@ void GridSolver::allocate_grids(int conn, const std::vector<InstanceDesc> &inst,
 	mConn = conn;
 	mVox = total;
 	mHasBack = has_back;
+	// until an init status says otherwise, the device-driven solve stages residuals as int16 (it
+	// switches when the init kernel finds an edge whose capacity pair exceeds 32767)
+	mWideResiduals = false;
 	mCs.allocate(total);
 	mCt.allocate(total);
 	mCn.allocate((size_t)total * half);
@@ -255,7 +241,10 @@ void GridSolver::allocate_grids(int conn, const std::vector<InstanceDesc> &inst,
 	mCounterTotal.allocate(3);
 	mStatus.allocate(1);
 	mCount.allocate(4);  // [0] list count, [1] [2] count_two, [3] closure reached t
-	mHostScratch.allocate(count + 3 > 8 ? count + 3 : 8);  // flows + visit counters
+	// control block of the device-driven solve, then its visit-counter and per-grid flow totals
+	mCtl.allocate(kCtlTotals + 3 + count);
+	mHostScratch.allocate(kCtlTotals + 3 + count);  // read-backs: flows + visit counters, or the control block
+	++mGeneration;
 }
 
 void GridSolver::set_batch(int conn, const GridSpec *specs, int count, bool device_pointers)
@@ -445,11 +434,11 @@ void GridSolver::build_colours(ColourSet &set, int ncolours, bool corners)
 
 void GridSolver::configure_tiles(const Shape &shape)
 {
-	if (mTilesReady && mReadyWave == mWave && mReadyShape.tx == shape.tx && mReadyShape.ty == shape.ty &&
-	    mReadyShape.tz == shape.tz) {
+	if (mTilesReady && mReadyShape.tx == shape.tx && mReadyShape.ty == shape.ty && mReadyShape.tz == shape.tz) {
 		return;
 	}
 	mTilesReady = false;
+	++mGeneration;
 	mShape = shape;
 	int64_t tiles = 0;
 	for (InstanceDesc &d : mInst) {
@@ -524,15 +513,8 @@ void GridSolver::configure_tiles(const Shape &shape)
 		CGP_HIP(hipMemcpyAsync(mTileInst.get(), owner.data(), owner.size() * sizeof(int32_t), hipMemcpyHostToDevice,
 		                       mStream));
 	}
-	if (mWave) {
-		mBits.allocate((size_t)wave::kPlanes * mTiles * wave::R);
-		mSBits.allocate((size_t)mTiles * wave::R);
-		mReach.allocate((size_t)mTiles * wave::R);
-		mFaces.allocate((size_t)mTiles * wave::kFaces);
-	}
 	CGP_HIP(hipStreamSynchronize(mStream));
 	mTilesReady = true;
-	mReadyWave = mWave;
 	mReadyShape = shape;
 }
 
@@ -541,12 +523,9 @@ static GridView make_view(int nx, int ny, int nz, int ntx, int nty, int ntz, int
                           DeviceBuffer<uint8_t> &s, DeviceBuffer<uint32_t> &omask, DeviceBuffer<int32_t> &cs,
                           DeviceBuffer<int32_t> &ct, DeviceBuffer<int32_t> &cn, DeviceBuffer<int32_t> &cnb,
                           DeviceBuffer<unsigned long long> &flow, DeviceBuffer<int32_t> &status,
-                          DeviceBuffer<unsigned long long> &counters, DeviceBuffer<int32_t> &faces,
-                          DeviceBuffer<uint32_t> &reach)
+                          DeviceBuffer<unsigned long long> &counters)
 {
 	GridView g;
-	g.faces = faces.get();
-	g.reach = reach.get();
 	g.nx = nx;
 	g.ny = ny;
 	g.nz = nz;
@@ -574,7 +553,7 @@ static GridView make_view(int nx, int ny, int nz, int ntx, int nty, int ntz, int
 }
 
 #define CGP_VIEW() batch_view(make_view(mNx, mNy, mNz, mNtx, mNty, mNtz, mVox, mE, mRt, mH, mR, mS, mOmask, mCs, \
-                                       mCt, mCn, mCnb, mFlow, mStatus, mCounters, mFaces, mReach),              \
+                                       mCt, mCn, mCnb, mFlow, mStatus, mCounters),                             \
                              mInst.size() > 1 ? mInstDev.get() : nullptr, mTileInst.get())
 
 // A batch's kernels find each tile's instance through the instance table.
@@ -587,53 +566,50 @@ static GridView batch_view(GridView g, const InstanceDesc *inst, const int32_t *
 	return g;
 }
 
-void GridSolver::launch_init()
+void GridSolver::launch_init(const SolveControl *ctl)
 {
 	GridView g = CGP_VIEW();
 	// reads cs, ct and the n-link planes, writes e, rt, the K residual planes and the state word
 	const double bytes = 4.0 * (2 + (mConn / 2) * (mHasBack ? 2 : 1)) + 4.0 * (3 + mConn);
-	if (mWave) {
-		launch(kInit, mVox, bytes + 1.0, [&] {
-			hipLaunchKernelGGL(wave::k_wave_init, dim3(mTiles), dim3(64), 0, mStream, g, mBits.get(), mTiles);
-		});
-		return;
-	}
 	launch(kInit, mVox, bytes, [&] {
 		const bool batch = mInst.size() > 1;
 		const int32_t *block_inst = batch ? mInitBlockInst.get() : nullptr;
 		const int32_t *first = batch ? mInitFirstBlock.get() : nullptr;
 		if (mConn == 6) {
-			hipLaunchKernelGGL(k_grid_init<6>, dim3(mInitBlocks), dim3(kInitThreads), 0, mStream, g, block_inst, first);
+			hipLaunchKernelGGL(k_grid_init<6>, dim3(mInitBlocks), dim3(kInitThreads), 0, mStream, g, block_inst, first,
+			                   ctl);
 		} else {
 			hipLaunchKernelGGL(k_grid_init<26>, dim3(mInitBlocks), dim3(kInitThreads), 0, mStream, g, block_inst,
-			                   first);
+			                   first, ctl);
 		}
 	});
 }
 
-void GridSolver::launch_compact(const ColourSet &set, uint8_t *flags, int colour, bool clear, const uint8_t *work)
+void GridSolver::launch_compact(const ColourSet &set, uint8_t *flags, int colour, bool clear, const uint8_t *work,
+                                const SolveControl *ctl, unsigned gate)
 {
 	if (mTiles <= kCompactBlockTiles || (work && mTiles <= kCompactOrderedTiles)) {
 		launch(kCompact, 0, 0.0, [&] {
 			if (mTiles <= kCompactBlockTiles) {
 				hipLaunchKernelGGL(k_compact_block<1>, dim3(1), dim3(kCompactThreads), 0, mStream, flags, mTiles,
 				                   set.colour.get(), colour, clear ? 1 : 0, mList.get(), mCount.get(),
-				                   (const uint8_t *)nullptr, (int32_t *)nullptr, work);
+				                   (const uint8_t *)nullptr, (int32_t *)nullptr, work, ctl, gate);
 			} else {
 				hipLaunchKernelGGL(k_compact_block<8>, dim3(1), dim3(kCompactThreads), 0, mStream, flags, mTiles,
 				                   set.colour.get(), colour, clear ? 1 : 0, mList.get(), mCount.get(),
-				                   (const uint8_t *)nullptr, (int32_t *)nullptr, work);
+				                   (const uint8_t *)nullptr, (int32_t *)nullptr, work, ctl, gate);
 			}
 		});
 		return;
 	}
 	launch(kFill, 0, 0.0, [&] {
-		hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, mStream, mCount.get(), (int64_t)1, 0);
+		hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, mStream, mCount.get(), (int64_t)1, 0,
+		                   (const SolveControl *)nullptr, 0u);
 	});
 	launch(kCompact, 0, 0.0, [&] {
 		hipLaunchKernelGGL(k_compact_tiles, dim3((mTiles + kCompactTilesPerGroup - 1) / kCompactTilesPerGroup),
-		                   dim3(kCompactThreads), 0, mStream, flags,
-		                   mTiles, set.colour.get(), colour, clear ? 1 : 0, mList.get(), mCount.get());
+		                   dim3(kCompactThreads), 0, mStream, flags, mTiles, set.colour.get(), colour, clear ? 1 : 0,
+		                   mList.get(), mCount.get(), ctl, gate);
 	});
 }
 
@@ -662,29 +638,25 @@ void GridSolver::count_two(uint8_t *a, uint8_t *b, int &na, int &nb)
 		launch(kCompact, 0, 0.0, [&] {
 			hipLaunchKernelGGL(k_compact_block<1>, dim3(1), dim3(kCompactThreads), 0, mStream, a, mTiles,
 			                   mSweep.colour.get(), -1, 0, (int32_t *)nullptr, mCount.get() + 1, (const uint8_t *)b,
-			                   mCount.get() + 2, (const uint8_t *)nullptr);
+			                   mCount.get() + 2, (const uint8_t *)nullptr, (const SolveControl *)nullptr, 0u);
 		});
-		CGP_HIP(hipMemcpyAsync(pinned, mCount.get() + 1, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, mStream));
-		CGP_HIP(hipStreamSynchronize(mStream));
-		if (mStatusPending) {
-			apply_status(pinned[4]);
-		}
-		na = pinned[0];
-		nb = b ? pinned[1] : 0;
-		return;
+	} else {
+		launch(kFill, 0, 0.0, [&] {
+			hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, mStream, mCount.get() + 1, (int64_t)2, 0,
+			                   (const SolveControl *)nullptr, 0u);
+		});
+		launch(kCompact, 0, 0.0, [&] {
+			const int blocks = (mTiles + kCompactTilesPerGroup - 1) / kCompactTilesPerGroup;
+			hipLaunchKernelGGL(k_compact_tiles, dim3(blocks), dim3(kCompactThreads), 0, mStream, a, mTiles,
+			                   mSweep.colour.get(), -1, 0, (int32_t *)nullptr, mCount.get() + 1,
+			                   (const SolveControl *)nullptr, 0u);
+			if (b) {
+				hipLaunchKernelGGL(k_compact_tiles, dim3(blocks), dim3(kCompactThreads), 0, mStream, b, mTiles,
+				                   mSweep.colour.get(), -1, 0, (int32_t *)nullptr, mCount.get() + 2,
+				                   (const SolveControl *)nullptr, 0u);
+			}
+		});
 	}
-	launch(kFill, 0, 0.0, [&] {
-		hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, mStream, mCount.get() + 1, (int64_t)2, 0);
-	});
-	launch(kCompact, 0, 0.0, [&] {
-		const int blocks = (mTiles + kCompactTilesPerGroup - 1) / kCompactTilesPerGroup;
-		hipLaunchKernelGGL(k_compact_tiles, dim3(blocks), dim3(kCompactThreads), 0, mStream, a, mTiles, mSweep.colour.get(), -1, 0,
-		                   (int32_t *)nullptr, mCount.get() + 1);
-		if (b) {
-			hipLaunchKernelGGL(k_compact_tiles, dim3(blocks), dim3(kCompactThreads), 0, mStream, b, mTiles, mSweep.colour.get(), -1,
-			                   0, (int32_t *)nullptr, mCount.get() + 2);
-		}
-	});
 	CGP_HIP(hipMemcpyAsync(pinned, mCount.get() + 1, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, mStream));
 	CGP_HIP(hipStreamSynchronize(mStream));
 	if (mStatusPending) {
@@ -753,97 +725,50 @@ int GridSolver::colour_sweeps(uint8_t *dirty, uint8_t *also_count, int &also, Pa
 	}
 }
 
-int GridSolver::count_flags(const uint8_t *flags, int colour)
-{
-	launch_compact(mSweep, const_cast<uint8_t *>(flags), colour);
-	int32_t *pinned = reinterpret_cast<int32_t *>(mHostScratch.get());
-	CGP_HIP(hipMemcpyAsync(pinned, mCount.get(), sizeof(int32_t), hipMemcpyDeviceToHost, mStream));
-	CGP_HIP(hipStreamSynchronize(mStream));
-	return pinned[0];
-}
-
-int GridSolver::run_global_relabel(int32_t threshold, cgp_stats &stats)
+// One global-relabel pass over `grid` blocks taking tiles from list (count tiles).
+void GridSolver::launch_bfs_pass(const int32_t *list, const int32_t *count, int grid, int32_t threshold,
+                                 int64_t voxels)
 {
 	GridView g = CGP_VIEW();
-	const int64_t tile_voxels = (int64_t)mShape.tx * mShape.ty * mShape.tz;
-	if (!mWave) {
-		launch(kFill, mVox, 4.0, [&] {
-			hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(mVox, 256, 8192)), dim3(256), 0, mStream, mH.get(), mVox,
-			                   kLabelInf);
+	// exact passes read the state word and h and write h; threshold passes read r[K], rt, e, h
+	const double bytes = threshold == 0 ? 12.0 : 4.0 * (mConn + 4);
+	launch(kBfs, voxels, bytes, [&] {
+		dispatch_shape(mConn, mShape.tx, mShape.ty, mShape.tz, [&](auto tag) {
+			using S = decltype(tag);
+			hipLaunchKernelGGL((k_grid_bfs<S::conn, S::tx, S::ty, S::tz>), dim3(grid), dim3(S::tx * S::ty), 0,
+			                   mStream, g, list, count, threshold, mDirtyA.get(), mActive.get(), mTileWork.get());
 		});
-		launch(kFill, 0, 0.0, [&] {
-			hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(mTiles, 256)), dim3(256), 0, mStream, mDirtyA.get(),
-			                   (int64_t)mTiles, (uint8_t)0);
-		});
-		int active = 0;
-		// exact passes read the state word and h and write h; threshold passes read r[K], rt, e, h
-		const double bytes = threshold == 0 ? 12.0 : 4.0 * (mConn + 4);
-		stats.bfs_passes += colour_sweeps(mDirtyA.get(), mActive.get(), active,
-		                                  [&](const int32_t *list, const int32_t *count, int grid, bool first) {
-			                                  launch(kBfs, first ? grid * tile_voxels : -1, bytes, [&] {
-				                                  dispatch_shape(mConn, mShape.tx, mShape.ty, mShape.tz, [&](auto tag) {
-					                                  using S = decltype(tag);
-					                                  hipLaunchKernelGGL((k_grid_bfs<S::conn, S::tx, S::ty, S::tz>),
-					                                                     dim3(grid), dim3(S::tx * S::ty), 0, mStream, g,
-					                                                     list, count, threshold, mDirtyA.get(),
-					                                                     mActive.get(), mTileWork.get());
-				                                  });
-			                                  });
-		                                  });
-		return active;
-	}
-	const int64_t n = (int64_t)mTiles * wave::kFaces;
-	launch(kFill, mVox, 4.0 * n / mVox, [&] {
-		hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(n, 256, 8192)), dim3(256), 0, mStream, mFaces.get(), n,
-		                   kLabelInf);
 	});
-	const int32_t *list = mAllTiles.get();
-	const int32_t *count = mAllTiles.get() + mTiles;
-	int blocks = mTiles;
-	for (;;) {
-		launch(kFill, 0, 0.0, [&] {
-			hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(mTiles, 256)), dim3(256), 0, mStream, mDirtyA.get(),
-			                   (int64_t)mTiles, (uint8_t)0);
-		});
-		// reads the tile's bit planes (8 bits/voxel) and its own face labels (832 per 1024 voxels),
-		// writes face labels and the reach plane
-		launch(kBfs, (int64_t)blocks * tile_voxels, 1.0 + 2.0 * 4.0 * wave::kFaces / wave::NV + 0.125, [&] {
-			hipLaunchKernelGGL(wave::k_wave_bfs, dim3(blocks), dim3(64), 0, mStream, g, mBits.get(), mTiles, list,
-			                   count, threshold, mDirtyA.get(), mActive.get(), (int32_t *)nullptr);
-		});
-		++stats.bfs_passes;
-		const int dirty = count_flags(mDirtyA.get(), -1);
-		if (dirty == 0) {
-			break;
-		}
-		list = mList.get();
-		count = mCount.get();
-		blocks = dirty;
-	}
-	return count_flags(mActive.get(), -1);
 }
 
-void GridSolver::run_discharge_phase(int colour, int max_iters)
+// One S_min closure pass (first: seeded from the excess). reached (may be null) is set when the
+// closure takes in a voxel with a residual sink link.
+void GridSolver::launch_closure_pass(const int32_t *list, const int32_t *count, int grid, bool first,
+                                     int32_t *reached, const SolveControl *ctl, int64_t voxels)
 {
 	GridView g = CGP_VIEW();
-	// heaviest tiles first (work estimates of the last relabel; the wave engine has none)
-	launch_compact(mPhase, mActive.get(), colour, false, mWave ? nullptr : mTileWork.get());
-	// every tile of one colour; blocks beyond the compacted count exit at once
-	const int blocks = mPhase.max_size;
-	if (mWave) {
-		// reads and writes e, rt, r[6] and the bit planes; writes the face labels
-		launch(kDischarge, -1, 8.0 * (mConn + 2) + 2.0 + 4.0 * wave::kFaces / wave::NV, [&] {
-			hipLaunchKernelGGL(wave::k_wave_discharge, dim3(blocks), dim3(64), 0, mStream, g, mBits.get(), mTiles,
-			                   mList.get(), mCount.get(), max_iters, mActive.get());
+	// reads the state word and s, writes s
+	launch(kClosure, voxels, 6.0, [&] {
+		dispatch_shape(mConn, mShape.tx, mShape.ty, mShape.tz, [&](auto tag) {
+			using S = decltype(tag);
+			hipLaunchKernelGGL((k_grid_source_closure<S::conn, S::tx, S::ty, S::tz>), dim3(grid), dim3(S::tx * S::ty),
+			                   0, mStream, g, list, count, first ? 1 : 0, mDirtyB.get(), reached, ctl);
 		});
-		return;
-	}
+	});
+}
+
+// One discharge phase over the compacted list (every tile of one colour at most; blocks beyond
+// the count exit at once).
+void GridSolver::launch_discharge(int max_iters, bool wide)
+{
+	GridView g = CGP_VIEW();
+	const int blocks = mPhase.max_size;
 	// reads and writes e, rt, h and r[K], writes the state word; residuals staged in LDS as int16
 	// unless some edge's capacity pair exceeds 32767
 	launch(kDischarge, -1, 8.0 * (mConn + 3) + 4.0, [&] {
 		dispatch_shape(mConn, mShape.tx, mShape.ty, mShape.tz, [&](auto tag) {
 			using S = decltype(tag);
-			if (mWideResiduals) {
+			if (wide) {
 				hipLaunchKernelGGL((k_grid_discharge<S::conn, S::tx, S::ty, S::tz, int32_t>), dim3(blocks),
 				                   dim3(S::tx * S::ty), 0, mStream, g, mList.get(), mCount.get(), max_iters,
 				                   mActive.get());
@@ -856,72 +781,58 @@ void GridSolver::run_discharge_phase(int colour, int max_iters)
 	});
 }
 
+int GridSolver::run_global_relabel(int32_t threshold, cgp_stats &stats)
+{
+	const int64_t tile_voxels = (int64_t)mShape.tx * mShape.ty * mShape.tz;
+	launch(kFill, mVox, 4.0, [&] {
+		hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(mVox, 256, 8192)), dim3(256), 0, mStream, mH.get(), mVox,
+		                   kLabelInf, (const SolveControl *)nullptr, 0u);
+	});
+	launch(kFill, 0, 0.0, [&] {
+		hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(mTiles, 256)), dim3(256), 0, mStream, mDirtyA.get(),
+		                   (int64_t)mTiles, (uint8_t)0, (const SolveControl *)nullptr, 0u);
+	});
+	int active = 0;
+	stats.bfs_passes += colour_sweeps(mDirtyA.get(), mActive.get(), active,
+	                                  [&](const int32_t *list, const int32_t *count, int grid, bool first) {
+		                                  launch_bfs_pass(list, count, grid, threshold, first ? grid * tile_voxels : -1);
+	                                  });
+	return active;
+}
+
+void GridSolver::run_discharge_phase(int colour, int max_iters)
+{
+	// heaviest tiles first (work estimates of the last relabel)
+	launch_compact(mPhase, mActive.get(), colour, false, mTileWork.get());
+	launch_discharge(max_iters, mWideResiduals);
+}
+
 bool GridSolver::run_source_closure(cgp_stats &stats, bool check)
 {
-	GridView g = CGP_VIEW();
 	const int64_t tile_voxels = (int64_t)mShape.tx * mShape.ty * mShape.tz;
-	if (!mWave) {
-		int32_t *reached = check ? mCount.get() + 3 : nullptr;
-		launch(kFill, 0, 0.0, [&] {
-			hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, mStream, mCount.get() + 3, (int64_t)1, 0);
-		});
-		launch(kFill, mVox, 1.0, [&] {
-			hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(mVox, 256, 8192)), dim3(256), 0, mStream, mS.get(), mVox,
-			                   (uint8_t)0);
-		});
-		launch(kFill, 0, 0.0, [&] {
-			hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(mTiles, 256)), dim3(256), 0, mStream, mDirtyB.get(),
-			                   (int64_t)mTiles, (uint8_t)0);
-		});
-		int unused = 0;
-		bool hit = false;
-		// reads the state word and s, writes s
-		stats.cut_passes += colour_sweeps(
-		    mDirtyB.get(), nullptr, unused,
-		    [&](const int32_t *list, const int32_t *count, int grid, bool first) {
-			    launch(kClosure, first ? grid * tile_voxels : -1, 6.0, [&] {
-				    dispatch_shape(mConn, mShape.tx, mShape.ty, mShape.tz, [&](auto tag) {
-					    using S = decltype(tag);
-					    hipLaunchKernelGGL((k_grid_source_closure<S::conn, S::tx, S::ty, S::tz>), dim3(grid),
-					                       dim3(S::tx * S::ty), 0, mStream, g, list, count, first ? 1 : 0,
-					                       mDirtyB.get(), reached);
-				    });
-			    });
-		    },
-		    check ? &hit : nullptr);
-		// (the last count of the sweeps read the flag after their last pass)
-		return hit;
-	}
-	launch(kFill, mVox, 0.125, [&] {
-		hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for((int64_t)mTiles * wave::R * 4, 256, 8192)), dim3(256), 0,
-		                   mStream, reinterpret_cast<uint8_t *>(mSBits.get()), (int64_t)mTiles * wave::R * 4,
-		                   (uint8_t)0);
+	int32_t *reached = check ? mCount.get() + 3 : nullptr;
+	launch(kFill, 0, 0.0, [&] {
+		hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, mStream, mCount.get() + 3, (int64_t)1, 0,
+		                   (const SolveControl *)nullptr, 0u);
 	});
-	const int32_t *list = mAllTiles.get();
-	const int32_t *count = mAllTiles.get() + mTiles;
-	int blocks = mTiles;
-	int first = 1;
-	for (;;) {
-		launch(kFill, 0, 0.0, [&] {
-			hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(mTiles, 256)), dim3(256), 0, mStream, mDirtyB.get(),
-			                   (int64_t)mTiles, (uint8_t)0);
-		});
-		// reads 7 bit planes and the S bits, writes the S bits
-		launch(kClosure, (int64_t)blocks * tile_voxels, 1.0, [&] {
-			hipLaunchKernelGGL(wave::k_wave_closure, dim3(blocks), dim3(64), 0, mStream, g, mBits.get(), mSBits.get(),
-			                   mTiles, list, count, first, mDirtyB.get());
-		});
-		++stats.cut_passes;
-		first = 0;
-		const int dirty = count_flags(mDirtyB.get(), -1);
-		if (dirty == 0) {
-			break;
-		}
-		list = mList.get();
-		count = mCount.get();
-		blocks = dirty;
-	}
-	return false;  // the wave engine is never asked to check
+	launch(kFill, mVox, 1.0, [&] {
+		hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(mVox, 256, 8192)), dim3(256), 0, mStream, mS.get(), mVox,
+		                   (uint8_t)0, (const SolveControl *)nullptr, 0u);
+	});
+	launch(kFill, 0, 0.0, [&] {
+		hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(mTiles, 256)), dim3(256), 0, mStream, mDirtyB.get(),
+		                   (int64_t)mTiles, (uint8_t)0, (const SolveControl *)nullptr, 0u);
+	});
+	int unused = 0;
+	bool hit = false;
+	stats.cut_passes += colour_sweeps(
+	    mDirtyB.get(), nullptr, unused,
+	    [&](const int32_t *list, const int32_t *count, int grid, bool first) {
+		    launch_closure_pass(list, count, grid, first, reached, nullptr, first ? grid * tile_voxels : -1);
+	    },
+	    check ? &hit : nullptr);
+	// (the last count of the sweeps read the flag after their last pass)
+	return hit;
 }
 
 // Sum of the slots of visit counter `kind` (0 discharge, 1 relabel, 2 closure).
@@ -973,10 +884,193 @@ cgp_policy default_policy()
 	return p;
 }
 
+// ---- device-driven solve ------------------------------------------------------------------------
+
+GridSolver::RoundGraph::~RoundGraph() { reset(); }
+
+void GridSolver::RoundGraph::reset()
+{
+	if (exec) {
+		(void)hipGraphExecDestroy(exec);
+	}
+	exec = nullptr;
+}
+
+// Records init plus one round of stages (relabel, discharge phases, termination closure or final
+// closure), each gated by the control block, then the flow and visit-counter sums. Replayed until
+// the control block says done: a replay whose stage is not the current one costs only launches of
+// kernels that exit at once.
+void GridSolver::capture_round_graph(int phases, int max_iters, bool wide)
+{
+	mGraph.reset();
+	SolveControl *ctl = control();
+	const int64_t tile_voxels = (int64_t)mShape.tx * mShape.ty * mShape.tz;
+	const int ncolours = mConn == 6 ? 2 : 8;
+	const int32_t *sweep_sizes = mSweep.tiles.get() + mTiles;
+	unsigned long long *totals = mCtl.get() + kCtlTotals;
+	const size_t n = mInst.size();
+	hipGraph_t graph = nullptr;
+	CGP_HIP(hipStreamBeginCapture(mStream, hipStreamCaptureModeThreadLocal));
+	try {
+		mCapturing = true;
+		// init (gated: it runs once per solve)
+		hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, mStream, mStatus.get(), (int64_t)1, 0,
+		                   (const SolveControl *)ctl, kGateInit);
+		const int64_t flow_bytes = 8 * (int64_t)n * kSlotWords;
+		hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(flow_bytes, 256)), dim3(256), 0, mStream,
+		                   reinterpret_cast<uint8_t *>(mFlow.get()), flow_bytes, (uint8_t)0, (const SolveControl *)ctl,
+		                   kGateInit);
+		hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(8 * 3 * kSlotWords, 256)), dim3(256), 0, mStream,
+		                   reinterpret_cast<uint8_t *>(mCounters.get()), (int64_t)(8 * 3 * kSlotWords), (uint8_t)0,
+		                   (const SolveControl *)ctl, kGateInit);
+		launch_init(ctl);
+		hipLaunchKernelGGL(k_ctl_after_init, dim3(1), dim3(64), 0, mStream, ctl, (const int32_t *)mStatus.get(),
+		                   wide ? 1 : 0);
+		// global relabel: from scratch, or continued when the last replay left dirty tiles
+		hipLaunchKernelGGL(k_ctl_stage_begin, dim3(1), dim3(64), 0, mStream, ctl, kGateRelabelNew, sweep_sizes);
+		hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(mVox, 256, 8192)), dim3(256), 0, mStream, mH.get(), mVox,
+		                   kLabelInf, (const SolveControl *)ctl, kGateRelabelNew);
+		hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(mTiles, 256)), dim3(256), 0, mStream, mDirtyA.get(),
+		                   (int64_t)mTiles, (uint8_t)0, (const SolveControl *)ctl, kGateRelabelNew);
+		for (int c = 0; c < 2; ++c) {
+			if (mSweep.size[c] > 0) {
+				launch_bfs_pass(mSweep.tiles.get() + mSweep.start[c], &ctl->first[c], mSweep.size[c], 0,
+				                mSweep.size[c] * tile_voxels);
+			}
+		}
+		for (int cycle = 0; cycle < kGraphSweepCycles; ++cycle) {
+			for (int c = 0; c < 2; ++c) {
+				launch_compact(mSweep, mDirtyA.get(), c, true, nullptr, ctl, kGateRelabel);
+				launch_bfs_pass(mList.get(), mCount.get(), mSweep.max_size, 0, -1);
+			}
+		}
+		hipLaunchKernelGGL(k_ctl_relabel_end, dim3(1), dim3(kCtlThreads), 0, mStream, ctl,
+		                   (const uint8_t *)mDirtyA.get(), (const uint8_t *)mActive.get(), mTiles);
+		// discharge phases, heaviest tiles first
+		for (int p = 0; p < phases; ++p) {
+			launch_compact(mPhase, mActive.get(), p % ncolours, false, mTileWork.get(), ctl, kGateDischarge);
+			launch_discharge(max_iters, wide);
+		}
+		hipLaunchKernelGGL(k_ctl_discharge_end, dim3(1), dim3(64), 0, mStream, ctl, phases);
+		// termination test (or the final S_min closure after a relabel found nothing active)
+		hipLaunchKernelGGL(k_ctl_stage_begin, dim3(1), dim3(64), 0, mStream, ctl, kGateClosureNew, sweep_sizes);
+		hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(mVox, 256, 8192)), dim3(256), 0, mStream, mS.get(), mVox,
+		                   (uint8_t)0, (const SolveControl *)ctl, kGateClosureNew);
+		hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(mTiles, 256)), dim3(256), 0, mStream, mDirtyB.get(),
+		                   (int64_t)mTiles, (uint8_t)0, (const SolveControl *)ctl, kGateClosureNew);
+		for (int c = 0; c < 2; ++c) {
+			if (mSweep.size[c] > 0) {
+				launch_closure_pass(mSweep.tiles.get() + mSweep.start[c], &ctl->first[c], mSweep.size[c], true,
+				                    &ctl->reached, ctl, mSweep.size[c] * tile_voxels);
+			}
+		}
+		for (int cycle = 0; cycle < kGraphSweepCycles; ++cycle) {
+			for (int c = 0; c < 2; ++c) {
+				launch_compact(mSweep, mDirtyB.get(), c, true, nullptr, ctl, kGateClosure | kGateUnlessReached);
+				launch_closure_pass(mList.get(), mCount.get(), mSweep.max_size, false, &ctl->reached, ctl, -1);
+			}
+		}
+		hipLaunchKernelGGL(k_ctl_closure_end, dim3(1), dim3(kCtlThreads), 0, mStream, ctl,
+		                   (const uint8_t *)mDirtyB.get(), mTiles);
+		// visit counters and per-grid flows, read back with the control block
+		hipLaunchKernelGGL(k_sum_slots, dim3(1), dim3(64), 0, mStream, mCounters.get(), (int64_t)3, totals);
+		hipLaunchKernelGGL(k_sum_slots, dim3(blocks_for((int64_t)n, 256)), dim3(256), 0, mStream, mFlow.get(),
+		                   (int64_t)n, totals + 3);
+		CGP_LAUNCH_CHECK();
+		mCapturing = false;
+		CGP_HIP(hipStreamEndCapture(mStream, &graph));
+	} catch (...) {
+		mCapturing = false;
+		hipGraph_t partial = nullptr;
+		(void)hipStreamEndCapture(mStream, &partial);
+		if (partial) {
+			(void)hipGraphDestroy(partial);
+		}
+		(void)hipGetLastError();
+		throw;
+	}
+	hipGraphExec_t exec = nullptr;
+	const hipError_t err = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+	(void)hipGraphDestroy(graph);
+	CGP_HIP(err);
+	mGraph.exec = exec;
+	mGraph.generation = mGeneration;
+	mGraph.phases = phases;
+	mGraph.iterations = max_iters;
+	mGraph.wide = wide;
+}
+
+int64_t GridSolver::solve_graph(int phases, int max_iters, int max_rounds, cgp_stats &stats)
+{
+	const int ncolours = mConn == 6 ? 2 : 8;
+	// a replay starts its phases at colour 0, so every colour needs a phase in each replay
+	phases = (phases + ncolours - 1) / ncolours * ncolours;
+	const size_t n = mInst.size();
+	bool wide = mWideResiduals;
+	if (!mGraph.exec || mGraph.generation != mGeneration || mGraph.phases != phases ||
+	    mGraph.iterations != max_iters || mGraph.wide != wide) {
+		capture_round_graph(phases, max_iters, wide);
+	}
+	hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(64), 0, mStream, control(), (int32_t)max_rounds);
+	CGP_LAUNCH_CHECK();
+	unsigned long long *pinned = reinterpret_cast<unsigned long long *>(mHostScratch.get());
+	const SolveControl *host_ctl = reinterpret_cast<const SolveControl *>(pinned);
+	for (int replay = 0;; ++replay) {
+		CGP_HIP(hipGraphLaunch(mGraph.exec, mStream));
+		CGP_HIP(hipMemcpyAsync(pinned, mCtl.get(), (kCtlTotals + 3 + n) * sizeof(unsigned long long),
+		                       hipMemcpyDeviceToHost, mStream));
+		CGP_HIP(hipStreamSynchronize(mStream));
+		++stats.graph_replays;
+		const int mode = host_ctl->mode;
+		if (mode == kModeFailed) {
+			apply_status(host_ctl->status);  // throws
+			fail(CGP_ERR_DEVICE, "device-driven solve failed with status " + std::to_string(host_ctl->status));
+		}
+		if (mode == kModeNeedWide) {
+			// some edge's capacity pair exceeds 32767: stage the residuals as int32 from here on
+			mWideResiduals = true;
+			wide = true;
+			capture_round_graph(phases, max_iters, true);
+			hipLaunchKernelGGL(k_ctl_set_mode, dim3(1), dim3(64), 0, mStream, control(), (int32_t)kModeRelabelNew);
+			CGP_LAUNCH_CHECK();
+			continue;
+		}
+		if (mode == kModeNotConverged) {
+			fail(CGP_ERR_NOT_CONVERGED, "max_rounds (" + std::to_string(max_rounds) + ") reached before convergence");
+		}
+		if (mode == kModeDoneClosed || mode == kModeDoneExact) {
+			mClosedByCheck = mode == kModeDoneClosed;
+			break;
+		}
+		if (replay >= kMaxReplays) {
+			fail(CGP_ERR_NOT_CONVERGED, "device-driven solve still in stage " + std::to_string(mode) + " after " +
+			                                std::to_string(replay + 1) + " replays");
+		}
+	}
+	mWideResiduals = (host_ctl->status & 4) != 0;
+	stats.rounds = host_ctl->rounds;
+	stats.discharge_phases = host_ctl->phases;
+	stats.discharge_tile_visits = (int64_t)pinned[kCtlTotals];
+	stats.bfs_tile_visits = (int64_t)pinned[kCtlTotals + 1];
+	stats.cut_tile_visits = (int64_t)pinned[kCtlTotals + 2];
+	int64_t total = 0;
+	for (size_t i = 0; i < n; ++i) {
+		mFlows[i] = (int64_t)pinned[kCtlTotals + 3 + i];
+		total += mFlows[i];
+	}
+	return total;
+}
+
 int64_t GridSolver::solve(const cgp_policy &policy, cgp_stats *stats_out, const TraceHook &trace)
 {
 	if (mVox == 0) {
 		fail(CGP_ERR_STATE, "no grid set");
+	}
+	if (!std::isfinite(policy.minimal_residual) || policy.minimal_residual < 0.0f) {
+		fail(CGP_ERR_INVALID, "minimal_residual must be finite and non-negative");
+	}
+	if (policy.engine != CGP_ENGINE_AUTO && policy.engine != CGP_ENGINE_TILES && policy.engine != CGP_ENGINE_GRAPH) {
+		fail(CGP_ERR_INVALID, "unknown engine " + std::to_string(policy.engine));
 	}
 	cgp_stats stats;
 	std::memset(&stats, 0, sizeof(stats));
@@ -998,22 +1092,50 @@ int64_t GridSolver::solve(const cgp_policy &policy, cgp_stats *stats_out, const 
 	const int phases = policy.discharges_per_relabel > 0 ? policy.discharges_per_relabel : tp.phases;
 	const int max_iters = policy.max_tile_iterations > 0 ? policy.max_tile_iterations : tp.iterations;
 	const int max_rounds = policy.max_rounds > 0 ? policy.max_rounds : 1 << 20;
-	int32_t threshold = policy.minimal_residual > 0.0f ? (int32_t)std::floor(policy.minimal_residual) : 0;
-	if (policy.minimal_residual < 0.0f || !std::isfinite(policy.minimal_residual)) {
-		fail(CGP_ERR_INVALID, "minimal_residual must be finite and non-negative");
+	// (a threshold of 2^31 or more closes every edge, as INT32_MAX does)
+	int32_t threshold = policy.minimal_residual > 0.0f
+	                        ? (int32_t)std::min<double>(std::floor((double)policy.minimal_residual), 2147483647.0)
+	                        : 0;
+
+	// The device-driven engine takes no host decision per stage: it serves solves without trace
+	// hooks, per-kernel profiling, an edge threshold or a certificate.
+	const bool graph_ok = !trace.fn && !mProfile && threshold == 0 && mCertificate == 0;
+	if (policy.engine == CGP_ENGINE_GRAPH && !graph_ok) {
+		fail(CGP_ERR_INVALID, "the graph engine takes no trace hook, profiling, minimal_residual or certificate");
+	}
+	const bool use_graph = graph_ok && policy.engine != CGP_ENGINE_TILES;
+	if (use_graph) {
+		mStarted = true;
+		const int64_t flow = solve_graph(phases, max_iters, max_rounds, stats);
+		stats.flow = flow;
+		stats.total_ms = ms_since(t_start);
+		stats.voxels = mVox;
+		stats.tile_x = mShape.tx;
+		stats.tile_y = mShape.ty;
+		stats.tile_z = mShape.tz;
+		stats.conn = mConn;
+		mSolved = true;
+		mLabelsExact = !mClosedByCheck;
+		if (stats_out) {
+			*stats_out = stats;
+		}
+		return flow;
 	}
 
-	hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, mStream, mStatus.get(), (int64_t)1, 0);
+	hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, mStream, mStatus.get(), (int64_t)1, 0,
+	                   (const SolveControl *)nullptr, 0u);
 	{
 		const int64_t flow_bytes = 8 * (int64_t)mInst.size() * kSlotWords;
 		hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(flow_bytes, 256)), dim3(256), 0, mStream,
-		                   reinterpret_cast<uint8_t *>(mFlow.get()), flow_bytes, (uint8_t)0);
+		                   reinterpret_cast<uint8_t *>(mFlow.get()), flow_bytes, (uint8_t)0,
+		                   (const SolveControl *)nullptr, 0u);
 		hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(8 * 3 * kSlotWords, 256)), dim3(256), 0, mStream,
-		                   reinterpret_cast<uint8_t *>(mCounters.get()), (int64_t)(8 * 3 * kSlotWords), (uint8_t)0);
+		                   reinterpret_cast<uint8_t *>(mCounters.get()), (int64_t)(8 * 3 * kSlotWords), (uint8_t)0,
+		                   (const SolveControl *)nullptr, 0u);
 	}
-	launch_init();
+	launch_init(nullptr);
 	mStarted = true;
-	if (trace.fn || mWave) {
+	if (trace.fn) {
 		int32_t *pinned = reinterpret_cast<int32_t *>(mHostScratch.get());
 		CGP_HIP(hipMemcpyAsync(pinned, mStatus.get(), sizeof(int32_t), hipMemcpyDeviceToHost, mStream));
 		CGP_HIP(hipStreamSynchronize(mStream));
@@ -1046,12 +1168,14 @@ int64_t GridSolver::solve(const cgp_policy &policy, cgp_stats *stats_out, const 
 			fail(CGP_ERR_NOT_CONVERGED, "max_rounds (" + std::to_string(max_rounds) + ") reached before convergence");
 		}
 		t0 = std::chrono::steady_clock::now();
+		bool certified = false;
 		for (int p = 0; p < phases; ++p, ++phase) {
 			run_discharge_phase((int)(phase % ncolours), max_iters);
 			++stats.discharge_phases;
 			if (mCertificate > 0 && read_flow() >= mCertificate) {
-				// the flow into t equals the capacity of a cut: the preflow is maximum
-				mStoppedEarly = true;
+				// the flow into t reached the capacity of a cut the caller names: the preflow is
+				// maximum if that is a true cut capacity, which the closure test below confirms
+				certified = true;
 				break;
 			}
 		}
@@ -1060,20 +1184,22 @@ int64_t GridSolver::solve(const cgp_policy &policy, cgp_stats *stats_out, const 
 		}
 		stats.discharge_ms += ms_since(t0);
 		++round;
-		if (mStoppedEarly) {
-			break;
-		}
-		if (!mWave && threshold == 0) {
+		if (threshold == 0 || certified) {
 			// termination test by the closure instead of one more global relabel: when the closure of
 			// the excess reaches no voxel with a residual sink link, no excess can reach t, so the
 			// preflow is maximum and the closure is S_min. Otherwise it stops at the first check that
-			// sees t reached and the loop relabels as before.
+			// sees t reached and the loop relabels as before (a certificate that was no cut capacity
+			// is then dropped, and the solve runs to completion).
 			t0 = std::chrono::steady_clock::now();
 			const bool reached = run_source_closure(stats, true);
 			stats.cut_ms += ms_since(t0);
 			if (!reached) {
 				mClosedByCheck = true;
+				mStoppedEarly = certified;
 				break;
+			}
+			if (certified) {
+				mCertificate = 0;
 			}
 		}
 	}
@@ -1095,9 +1221,9 @@ int64_t GridSolver::solve(const cgp_policy &policy, cgp_stats *stats_out, const 
 	stats.tile_z = mShape.tz;
 	stats.conn = mConn;
 	mSolved = true;
-	// labels are those of the last relabel when the solve stopped at a certificate or at the
-	// closure test; T_min and label readbacks relabel once on request
-	mLabelsExact = !mStoppedEarly && !mClosedByCheck;
+	// labels are those of the last relabel when the solve stopped at the closure test; T_min and
+	// label readbacks relabel once on request
+	mLabelsExact = !mClosedByCheck;
 	trace.emit(CGP_TRACE_AFTER_PUSH, round, 1, flow);
 	trace.emit(CGP_TRACE_FINISHED, round, 1, flow);
 	if (stats_out) {
@@ -1126,14 +1252,8 @@ void GridSolver::sink_side(int index, uint8_t *host_out, uint8_t mask, uint8_t b
 	ensure_exact_labels();
 	const InstanceDesc &d = instance(index);
 	DeviceBuffer<uint8_t> tmp(d.nvox);
-	if (mWave) {
-		GridView g = CGP_VIEW();
-		hipLaunchKernelGGL(wave::k_wave_plane_bytes, dim3(mTiles), dim3(64), 0, mStream, g, mReach.get(), tmp.get(),
-		                   mask, background);
-	} else {
-		hipLaunchKernelGGL(k_sink_side_mask, dim3(blocks_for(d.nvox, 256, 8192)), dim3(256), 0, mStream,
-		                   mH.get() + d.vbase, d.nvox, tmp.get(), mask, background);
-	}
+	hipLaunchKernelGGL(k_sink_side_mask, dim3(blocks_for(d.nvox, 256, 8192)), dim3(256), 0, mStream,
+	                   mH.get() + d.vbase, d.nvox, tmp.get(), mask, background);
 	CGP_LAUNCH_CHECK();
 	CGP_HIP(hipMemcpyAsync(host_out, tmp.get(), d.nvox, hipMemcpyDeviceToHost, mStream));
 	CGP_HIP(hipStreamSynchronize(mStream));
@@ -1145,12 +1265,6 @@ void GridSolver::source_side(int index, uint8_t *host_out)
 		fail(CGP_ERR_STATE, "source_set before max_flow");
 	}
 	const InstanceDesc &d = instance(index);
-	if (mWave) {
-		GridView g = CGP_VIEW();
-		hipLaunchKernelGGL(wave::k_wave_plane_bytes, dim3(mTiles), dim3(64), 0, mStream, g, mSBits.get(), mS.get(),
-		                   (uint8_t)1, (uint8_t)0);
-		CGP_LAUNCH_CHECK();
-	}
 	CGP_HIP(hipMemcpyAsync(host_out, mS.get() + d.vbase, d.nvox, hipMemcpyDeviceToHost, mStream));
 	CGP_HIP(hipStreamSynchronize(mStream));
 }
@@ -1162,13 +1276,6 @@ void GridSolver::labels(int index, int32_t *host_out)
 	}
 	ensure_exact_labels();
 	const InstanceDesc &d = instance(index);
-	if (mWave) {
-		// the face labels are final: one pass over every tile yields every voxel's exact label
-		GridView g = CGP_VIEW();
-		hipLaunchKernelGGL(wave::k_wave_bfs, dim3(mTiles), dim3(64), 0, mStream, g, mBits.get(), mTiles,
-		                   mAllTiles.get(), mAllTiles.get() + mTiles, 0, mDirtyA.get(), mActive.get(), mH.get());
-		CGP_LAUNCH_CHECK();
-	}
 	CGP_HIP(hipMemcpyAsync(host_out, mH.get() + d.vbase, d.nvox * sizeof(int32_t), hipMemcpyDeviceToHost, mStream));
 	CGP_HIP(hipStreamSynchronize(mStream));
 }
@@ -1204,13 +1311,24 @@ void GridSolver::state(int index, int32_t *excess, int32_t *labels, uint32_t *sa
 		if (mSolved) {
 			ensure_exact_labels();
 		}
-		if (mWave) {
-			GridView g = CGP_VIEW();
-			hipLaunchKernelGGL(wave::k_wave_bfs, dim3(mTiles), dim3(64), 0, mStream, g, mBits.get(), mTiles,
-			                   mAllTiles.get(), mAllTiles.get() + mTiles, 0, mDirtyA.get(), mActive.get(), mH.get());
-			CGP_LAUNCH_CHECK();
-		}
 		CGP_HIP(hipMemcpyAsync(labels, mH.get() + d.vbase, d.nvox * sizeof(int32_t), hipMemcpyDeviceToHost, mStream));
+	}
+	CGP_HIP(hipStreamSynchronize(mStream));
+}
+
+void GridSolver::residuals(int index, int32_t *r, int32_t *rt)
+{
+	if (!mStarted) {
+		fail(CGP_ERR_STATE, "no solve has started on this grid");
+	}
+	const InstanceDesc &d = instance(index);
+	if (r) {
+		CGP_HIP(hipMemcpyAsync(r, mR.get() + d.vbase * mConn, (size_t)d.nvox * mConn * sizeof(int32_t),
+		                       hipMemcpyDeviceToHost, mStream));
+	}
+	if (rt) {
+		CGP_HIP(hipMemcpyAsync(rt, mRt.get() + d.vbase, (size_t)d.nvox * sizeof(int32_t), hipMemcpyDeviceToHost,
+		                       mStream));
 	}
 	CGP_HIP(hipStreamSynchronize(mStream));
 }
@@ -1230,13 +1348,6 @@ const uint8_t *GridSolver::device_source_side(int index)
 		fail(CGP_ERR_STATE, "the grid is not solved");
 	}
 	const InstanceDesc &d = instance(index);
-	if (mWave) {
-		GridView g = CGP_VIEW();
-		hipLaunchKernelGGL(wave::k_wave_plane_bytes, dim3(mTiles), dim3(64), 0, mStream, g, mSBits.get(), mS.get(),
-		                   (uint8_t)1, (uint8_t)0);
-		CGP_LAUNCH_CHECK();
-		CGP_HIP(hipStreamSynchronize(mStream));
-	}
 	return mS.get() + d.vbase;
 }
 

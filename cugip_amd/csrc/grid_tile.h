@@ -90,8 +90,6 @@ struct GridView {
 	unsigned long long *flow;  // kSlots slots of this instance's flow into t
 	int32_t *status;         // nonzero: input error detected on the device
 	unsigned long long *counters; // visit counters, kSlots slots per kind (see counter_slot)
-	int32_t *faces;          // wave engine: labels of each tile's face voxels
-	uint32_t *reach;         // wave engine: "reaches t" bit plane of the last global relabel
 	const InstanceDesc *inst;  // batch: instance table (null for a single grid)
 	const int32_t *tile_inst;  // batch: instance of every global tile
 	int32_t tbase;             // global id of this view's first tile

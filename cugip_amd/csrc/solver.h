@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "common.h"
+#include "grid_control.h"
 #include "grid_tile.h"
 
 namespace cgp {
@@ -44,6 +45,7 @@ public:
 	                     const cgp_volume_weights &weights, bool device_pointers);
 	void capacities(int index, int32_t *cs, int32_t *ct, int32_t *cn, int32_t *cnb);
 	void state(int index, int32_t *excess, int32_t *labels, uint32_t *saturated);
+	void residuals(int index, int32_t *r, int32_t *rt);
 	int64_t cut_capacity(int index, const uint8_t *host_side);
 	int64_t cut_capacity_device(int index, const uint8_t *device_side);
 	// Device views for the learned cut proposal.
@@ -93,9 +95,8 @@ private:
 	};
 	Shape pick_shape(const cgp_policy &policy);
 	void configure_tiles(const Shape &shape);
-	void launch_init();
+	void launch_init(const SolveControl *ctl);
 	int run_global_relabel(int32_t threshold, cgp_stats &stats);
-	int count_flags(const uint8_t *flags, int colour);
 	// A colouring of the tiles: per-tile colour, the tile ids grouped by colour (then one count per
 	// colour), the groups' starts and sizes, and the largest group.
 	struct ColourSet {
@@ -105,7 +106,11 @@ private:
 		int colours = 0, max_size = 0;
 	};
 	void launch_compact(const ColourSet &set, uint8_t *flags, int colour, bool clear = false,
-	                    const uint8_t *work = nullptr);
+	                    const uint8_t *work = nullptr, const SolveControl *ctl = nullptr, unsigned gate = 0);
+	void launch_bfs_pass(const int32_t *list, const int32_t *count, int grid, int32_t threshold, int64_t voxels);
+	void launch_closure_pass(const int32_t *list, const int32_t *count, int grid, bool first, int32_t *reached,
+	                         const SolveControl *ctl, int64_t voxels);
+	void launch_discharge(int max_iters, bool wide);
 	void count_two(uint8_t *a, uint8_t *b, int &na, int &nb);
 	template <typename Pass>
 	int colour_sweeps(uint8_t *dirty, uint8_t *also_count, int &also, Pass &&pass, bool *stop = nullptr);
@@ -114,6 +119,25 @@ private:
 	// preflow is then not maximum and the closure is not S_min); it stops early when it does
 	bool run_source_closure(cgp_stats &stats, bool check = false);
 	int64_t read_flow(unsigned long long *visits = nullptr);
+	// Device-driven solve: init and one round of gated stages captured in a HIP graph, replayed with
+	// one read of the control block per replay until it says done.
+	struct RoundGraph {
+		hipGraphExec_t exec = nullptr;
+		int64_t generation = -1;
+		int phases = 0, iterations = 0;
+		bool wide = false;
+		RoundGraph() = default;
+		RoundGraph(const RoundGraph &) = delete;
+		RoundGraph &operator=(const RoundGraph &) = delete;
+		~RoundGraph();
+		void reset();
+	};
+	static constexpr int kGraphSweepCycles = 3;  // colour-sweep cycles per stage and replay
+	static constexpr int kMaxReplays = 1 << 20;
+	static constexpr size_t kCtlTotals = sizeof(SolveControl) / sizeof(unsigned long long);
+	void capture_round_graph(int phases, int max_iters, bool wide);
+	int64_t solve_graph(int phases, int max_iters, int max_rounds, cgp_stats &stats);
+	SolveControl *control() const { return reinterpret_cast<SolveControl *>(mCtl.get()); }
 	unsigned long long read_counter(int kind);
 	const InstanceDesc &instance(int index) const;
 	std::vector<InstanceDesc> describe(int conn, const GridSpec *specs, int count);
@@ -129,8 +153,8 @@ private:
 	DeviceBuffer<int32_t> mInitBlockInst;      // batches only: instance of every init workgroup
 	DeviceBuffer<int32_t> mInitFirstBlock;     // batches only: first init workgroup of every instance
 	int mInitBlocks = 0;
-	// configure_tiles' tables are kept while the grids, the tile shape and the engine stay the same
-	bool mTilesReady = false, mReadyWave = false;
+	// configure_tiles' tables are kept while the grids and the tile shape stay the same
+	bool mTilesReady = false;
 	Shape mReadyShape{0, 0, 0};
 
 	Shape mShape{0, 0, 0};
@@ -163,9 +187,12 @@ private:
 	DeviceBuffer<int32_t> mStatus, mCount, mList, mAllTiles;
 	DeviceBuffer<uint8_t> mActive, mDirtyA, mDirtyB;
 	DeviceBuffer<uint8_t> mTileWork;  // per tile: discharge work estimate from the last relabel
-	DeviceBuffer<uint32_t> mBits, mSBits, mReach;  // wave engine: bit planes, S_min and reach planes
-	DeviceBuffer<int32_t> mFaces;                  // wave engine: face labels
-	bool mWave = false;                    // wave engine (6-connected, 32x8x4 tiles)
+	// device-driven solve: control block (kCtlTotals words), then 3 visit-counter totals and one flow
+	// total per grid
+	DeviceBuffer<unsigned long long> mCtl;
+	RoundGraph mGraph;
+	int64_t mGeneration = 0;  // bumped whenever buffers or tile tables a captured graph uses change
+	bool mCapturing = false;
 	PinnedBuffer<int64_t> mHostScratch;
 	bool mProfile = false;
 	KernelRecord mProf[kKernCount];

@@ -64,12 +64,17 @@ typedef struct cgp_policy {
 	int32_t reserved[7];
 } cgp_policy;
 
-/* Grid engines. AUTO picks TILES (the faster engine on the measured 6- and 26-connected workloads).
-   WAVE: one wave64 per 32x8x4 tile, tile-major bit planes, bit-parallel BFS (6-connected only).
-   TILES: one workgroup per tile of a policy-chosen shape, voxel-parallel LDS relaxation (6 and 26). */
+/* Grid engines: the same tile kernels, driven two ways.
+   TILES: the host runs the relabel / discharge / closure loop and reads tile counts between stages
+   (MinCut::compute's host-driven loop, advanced_operations/detail/graph_cut_implementation.hpp:122-198).
+   GRAPH: device-driven. A control block in device memory holds the solve's stage, every kernel is
+   gated by it and controller kernels move it on; init plus one round of stages is captured in a HIP
+   graph once and replayed, with one read of the control block per replay, until it says done.
+   Solves with trace hooks, profiling, minimal_residual > 0 or a certificate run on TILES.
+   AUTO: GRAPH where it can, else TILES. */
 #define CGP_ENGINE_AUTO 0
 #define CGP_ENGINE_TILES 1
-#define CGP_ENGINE_WAVE 2
+#define CGP_ENGINE_GRAPH 2
 
 /* Per-solve counters and device timings. */
 typedef struct cgp_stats {
@@ -87,7 +92,8 @@ typedef struct cgp_stats {
 	double init_ms, relabel_ms, discharge_ms, cut_ms, total_ms;
 	int64_t voxels;            /* vertices (grid: voxels) */
 	int32_t tile_x, tile_y, tile_z, conn;
-	int32_t reserved[8];
+	int32_t graph_replays;     /* GRAPH engine: replays of the round graph (0 on TILES) */
+	int32_t reserved[7];
 } cgp_stats;
 
 /* Trace hooks of the reference's trace-object concept (advanced_operations/graph_cut.hpp:26-45). */
@@ -183,6 +189,12 @@ int cgp_labels(cgp_graph graph, int32_t *out);
    neighbour in residual direction d has capacity > 0 and residual 0; d < conn/2 are the offsets of
    cap_n, d >= conn/2 their mirrors). Any output may be NULL. */
 int cgp_grid_state(cgp_graph graph, int32_t index, int32_t *excess, int32_t *labels, uint32_t *saturated);
+/* Residual capacities of grid `index` (0 for a single grid) once a solve has started: the residuals
+   of the reference's GraphCutData (advanced_operations/detail/graph_cut_data.hpp:20-140) in the grid
+   layout. residuals receives conn planes of voxel-count int32, plane d = residual v -> v + o_d
+   (d < conn/2: the offsets of cap_n; d >= conn/2: their mirrors; 0 where the neighbour is outside);
+   sink_residual the residual v -> t. Either may be NULL. */
+int cgp_grid_residuals(cgp_graph graph, int32_t index, int32_t *residuals, int32_t *sink_residual);
 /* Capacity of the cut whose source side is {s} + {v : source_side[v] != 0} of grid `index`
    (host buffer of its voxel count): the sum of cs over voxels outside, ct over voxels inside,
    and the capacities of the n-links from inside to outside. Exact int64. A cut whose capacity

@@ -157,6 +157,38 @@ def grid_edges(nx, ny, nz, conn, cap_n, cap_nb=None):
     return ea[:m].copy(), eb[:m].copy(), fwd[:m].copy(), bwd[:m].copy()
 
 
+def half_offsets(conn):
+    """(dx, dy, dz) of the negative half neighbourhood in the grid layout's plane order
+    (VonNeumannNeighborhood<3>::offset(1..3), MooreNeighborhood<3>::offset(1..13),
+    include/cugip/neighborhood.hpp:51-91, 160-262)."""
+    if conn == 6:
+        return [(-1, 0, 0), (0, -1, 0), (0, 0, -1)]
+    return [(j % 3 - 1, (j // 3) % 3 - 1, j // 9 - 1) for j in range(13)]
+
+
+def residual_labels(nx, ny, nz, conn, residuals, sink_residual):
+    """Labels of an exact global relabel of a grid's residual graph: BFS distance to t of every
+    voxel (1 where the residual v -> t is positive, then 1 + the lowest label of a neighbour u with
+    residual v -> u > 0), 0x3fffffff where t is unreachable. residuals: conn planes, plane d the
+    residual v -> v + o_d (d < conn/2 the half offsets, d >= conn/2 their mirrors). The BFS is the
+    oracle's restatement of the reference's relabel (orc_initial_labels) on the edge list of the
+    residual planes."""
+    H = conn // 2
+    r = _i32(residuals).reshape(conn, nz, ny, nx)
+    back = np.zeros((H, nz, ny, nx), np.int32)
+    for j, (dx, dy, dz) in enumerate(half_offsets(conn)):
+        # back[j] at v = residual (v + o_j) -> v, which plane j + H holds at voxel v + o_j
+        zd = slice(max(0, -dz), nz - max(0, dz))
+        yd = slice(max(0, -dy), ny - max(0, dy))
+        xd = slice(max(0, -dx), nx - max(0, dx))
+        zs = slice(zd.start + dz, zd.stop + dz)
+        ys = slice(yd.start + dy, yd.stop + dy)
+        xs = slice(xd.start + dx, xd.stop + dx)
+        back[j][zd, yd, xd] = r[j + H][zs, ys, xs]
+    ea, eb, fwd, bwd = grid_edges(nx, ny, nz, conn, r[:H], back)
+    return initial_labels(nx * ny * nz, ea, eb, fwd, bwd, _i32(sink_residual).reshape(-1))
+
+
 def grid_solve(method, nx, ny, nz, conn, cs, ct, cap_n, cap_nb=None):
     """Solve a grid instance with one of refpr/dinic/bk; returns (flow, smin, tmin) with the
     masks as flat uint8 arrays in voxel order."""

@@ -9,7 +9,8 @@ oracle agree on it (Boykov-Kolmogorov and Dinic everywhere; the restatement of t
 relabel/push loop where it finishes in seconds), and, where the total fits int32, when
 scipy.sparse.csgraph.maximum_flow (a third-party implementation) returns the same flow value.
 
-Run from the repository root: python tests/golden/make_golden.py
+Run from the repository root: python tests/golden/make_golden.py (everything), or
+python tests/golden/make_golden.py --large-26c (only the 256x256x192 26-connected record).
 """
 import hashlib
 import json
@@ -74,7 +75,27 @@ def solve_instance(nx, ny, nz, conn, seed, instance, with_refpr=True):
     return rec, s_bk, t_bk
 
 
+LARGE_26C = ((256, 256, 192), 26, 0, 0)
+
+
+def append(dims, conn, seed, inst):
+    """Solves one more instance (BK and Dinic) and adds its record to grid_fixtures.json."""
+    path = os.path.join(HERE, "grid_fixtures.json")
+    data = json.load(open(path))
+    data["instances"] = [r for r in data["instances"]
+                         if not (r["dims"] == list(dims) and r["conn"] == conn and r["seed"] == seed
+                                 and r["instance"] == inst)]
+    rec, _, _ = solve_instance(*dims, conn, seed, inst, with_refpr=False)
+    data["instances"].append(rec)
+    with open(path, "w") as f:
+        json.dump(data, f, indent=1)
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--large-26c":
+        # the largest 26-connected instance the two CPU solvers finish here (about 12.6 M voxels)
+        append(*LARGE_26C)
+        return
     records = []
     masks = {}
     for seed in range(16):
@@ -88,6 +109,8 @@ def main():
         rec, _, _ = solve_instance(*dims, conn, seed, inst, with_refpr=np.prod(dims) <= 600000)
         records.append(rec)
     rec, _, _ = solve_instance(256, 256, 256, 6, 0, 0, with_refpr=False)
+    records.append(rec)
+    rec, _, _ = solve_instance(*LARGE_26C[0], *LARGE_26C[1:], with_refpr=False)
     records.append(rec)
     out = dict(
         description="Synthetic grid graph-cut instances: exact max flow and SHA-256 of np.packbits of the "

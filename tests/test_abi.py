@@ -35,7 +35,7 @@ def test_every_declared_symbol_is_exported():
 
 
 def test_abi_version():
-    assert cugip_amd.abi_version() == 1
+    assert cugip_amd.abi_version() == 2
 
 
 def test_struct_layouts_match_header():

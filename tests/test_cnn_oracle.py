@@ -78,3 +78,33 @@ def test_bf16_rounding():
     assert out[2] == 1.0 + 4 * 2 ** -8      # tie -> even (up)
     assert out[3] == -2.5 and out[5] == 0.0
     assert abs(out[4] - 1e-3) <= 1e-3 * 2 ** -8
+
+
+def test_forward_matches_torch_conv3d():
+    """The float64 forward equals torch.nn.functional.conv3d (zero padding 1, cross-correlation)
+    with the tap order t = (dz+1)*9 + (dy+1)*3 + (dx+1) mapped to kernel index [dz+1, dy+1, dx+1],
+    on random weights where every tap of every layer is nonzero and a grid small enough that every
+    voxel touches a border: a transposed tap order, a flipped tap or missing padding all fail it."""
+    import torch
+    import torch.nn.functional as F
+    rng = np.random.default_rng(7)
+    shape = (4, 5, 6)
+    vol = rng.integers(0, 256, shape).astype(np.uint8)
+    cs = rng.integers(0, 1000, shape)
+    ct = rng.integers(0, 1000, shape)
+    p = rng.normal(0, 0.4, cnn.N_PARAMS)
+    p[:cnn.N_W1].reshape(27, 4, 16)[:, 3, :] = 0
+    z, _ = cnn.forward(p, vol, cs, ct, bf=False)
+    W, b, wh, bh = cnn.unpack(p)
+    x = torch.from_numpy(cnn.network_input(vol, cs, ct, bf=False)).permute(3, 0, 1, 2)[None]  # (1, 4, z, y, x)
+    for layer in range(4):
+        w = np.asarray(W[layer])  # (27, cin, cout)
+        k = torch.from_numpy(w.reshape(3, 3, 3, w.shape[1], w.shape[2]).transpose(4, 3, 0, 1, 2).copy())
+        x = F.relu(F.conv3d(x, k, torch.from_numpy(np.asarray(b[layer])), padding=1))
+    ref = (x[0].permute(1, 2, 3, 0).numpy() @ wh) + bh
+    assert np.allclose(z, ref, rtol=1e-12, atol=1e-12)
+    # the same network with the tap order read as (dx, dy, dz) instead differs
+    Wt = [np.asarray(w).reshape(3, 3, 3, *np.asarray(w).shape[1:]).transpose(2, 1, 0, 3, 4).reshape(np.asarray(w).shape)
+          for w in W]
+    zt, _ = cnn.forward(cnn.pack(Wt, b, wh, bh), vol, cs, ct, bf=False)
+    assert not np.allclose(zt, ref)

@@ -92,6 +92,43 @@ def test_batch_matches_single_solves_on_batch_workload():
         assert grid_cut_capacity(smin[i], cs, ct, cn) == flows[i], i
 
 
+def test_batch_workload_512():
+    """The full batch workload: 512 grids with edge lengths log-uniform in [32, 128] per axis and
+    grid (165.6 M voxels), solved as one batch by the device-driven engine. Every grid's flow and
+    S_min equal the single-grid solver's bit for bit; the 16 smallest grids also match the CPU
+    oracle (Boykov-Kolmogorov) in flow, S_min and T_min."""
+    import time
+    say("batch workload, 512 grids: building")
+    grids = cugip_amd.make_batch(512, seed=0)
+    vox = sum(cs.size for cs, _, _ in grids)
+    with cugip_amd.GridBatch() as b:
+        b.set_batch(grids, 6)
+        t0 = time.perf_counter()
+        flows = b.solve()
+        stats = b.stats.as_dict()
+        say(f"  batch solve {1e3 * (time.perf_counter() - t0):.1f} ms for {vox} voxels, "
+            f"{stats['graph_replays']} replays, {stats['rounds']} rounds")
+        assert stats["graph_replays"] > 0
+        smin = [b.batch_source_set(i).reshape(-1) for i in range(len(grids))]
+        order = sorted(range(len(grids)), key=lambda i: grids[i][0].size)
+        small = order[:16]
+        tmin = {i: b.batch_fill_segments(i, 1, 0).reshape(-1) for i in small}
+    say("  single-grid solves")
+    with cugip_amd.GridGraph() as g:
+        for i, (cs, ct, cn) in enumerate(grids):
+            g.set_grid(cs, ct, cn)
+            assert g.max_flow() == flows[i], i
+            assert np.array_equal(g.source_set().reshape(-1), smin[i]), i
+    say("  oracle on the 16 smallest grids")
+    for i in small:
+        cs, ct, cn = grids[i]
+        nz, ny, nx = cs.shape
+        ref_flow, ref_smin, ref_tmin = oracle.grid_solve("bk", nx, ny, nz, 6, cs, ct, cn)
+        assert flows[i] == ref_flow, i
+        assert np.array_equal(smin[i], ref_smin), i
+        assert np.array_equal(tmin[i], ref_tmin), i
+
+
 def test_batch_of_one_and_errors():
     say("batch of one, errors")
     vol, cs, ct, cn = cugip_amd.make_instance(33, 20, 17, 9, 0, 6)
@@ -113,7 +150,7 @@ def test_batch_of_one_and_errors():
         with pytest.raises(cugip_amd.CgpError) as e:
             b.source_set()  # a batch must name its grid
         assert e.value.code == cugip_amd.CGP_ERR_INVALID
-        b.set_policy(engine=cugip_amd.ENGINE_WAVE)
+        b.set_policy(engine=7)  # no such engine
         with pytest.raises(cugip_amd.CgpError) as e:
             b.solve()
         assert e.value.code == cugip_amd.CGP_ERR_INVALID

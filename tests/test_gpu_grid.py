@@ -55,30 +55,46 @@ def test_parity_synthetic(dims, conn, seed, instance, policy):
     check_against_oracle(cs, ct, cn, None, conn, **policy)
 
 
-@pytest.mark.parametrize("engine", [cugip_amd.ENGINE_WAVE, cugip_amd.ENGINE_TILES])
+@pytest.mark.parametrize("engine", [cugip_amd.ENGINE_GRAPH, cugip_amd.ENGINE_TILES])
 @pytest.mark.parametrize("dims,seed,policy", [
     ((64, 64, 64), 0, {}),
     ((96, 40, 28), 1, dict(max_tile_iterations=1)),
     ((33, 17, 9), 2, dict(discharges_per_relabel=1)),
     ((128, 64, 48), 3, dict(max_tile_iterations=64, discharges_per_relabel=6)),
-    ((61, 45, 37), 4, dict(minimal_residual=2.0)),
+    ((61, 45, 37), 4, dict(tile_x=32, tile_y=8, tile_z=8)),
+    ((20, 300, 3), 5, dict(tile_x=64, tile_y=4, tile_z=4)),
 ])
 def test_parity_engines(engine, dims, seed, policy):
+    """The host-driven loop and the device-driven graph replay give the oracle's flow, S_min and
+    T_min (T_min after a graph solve comes from one exact relabel on request)."""
     nx, ny, nz = dims
     say(f"grid engine {engine} {dims} seed {seed} policy {policy}")
     vol, cs, ct, cn = cugip_amd.make_instance(nx, ny, nz, seed, 0, 6)
-    check_against_oracle(cs, ct, cn, None, 6, engine=engine, **policy)
+    stats = check_against_oracle(cs, ct, cn, None, 6, engine=engine, **policy)
+    assert (stats["graph_replays"] > 0) == (engine == cugip_amd.ENGINE_GRAPH)
 
 
-def test_wave_engine_rejects_other_shapes():
-    say("wave engine shape check")
-    vol, cs, ct, cn = cugip_amd.make_instance(16, 16, 16, 1, 0, 26)
+def test_graph_engine_refuses_host_decisions():
+    """The graph engine takes no per-stage host decision: minimal_residual, trace hooks and an
+    unknown engine are refused; AUTO then falls back to the host loop."""
+    say("graph engine argument checks")
+    vol, cs, ct, cn = cugip_amd.make_instance(16, 16, 16, 1, 0, 6)
     with cugip_amd.GridGraph() as g:
-        g.set_grid(cs, ct, cn, conn=26)
-        g.set_policy(engine=cugip_amd.ENGINE_WAVE)
+        g.set_grid(cs, ct, cn)
+        g.set_policy(engine=cugip_amd.ENGINE_GRAPH, minimal_residual=2.0)
         with pytest.raises(cugip_amd.CgpError) as e:
             g.max_flow()
         assert e.value.code == cugip_amd.CGP_ERR_INVALID
+        g.set_policy(engine=7)
+        with pytest.raises(cugip_amd.CgpError) as e:
+            g.max_flow()
+        assert e.value.code == cugip_amd.CGP_ERR_INVALID
+        g.set_policy(minimal_residual=2.0)
+        flow = g.max_flow()
+        assert g.stats.graph_replays == 0
+        g.set_policy()
+        assert g.max_flow() == flow
+        assert g.stats.graph_replays > 0
 
 
 @pytest.mark.parametrize("conn", [6, 26])
@@ -434,7 +450,8 @@ def test_6c_above_the_one_workgroup_compaction():
     """A 6-connected grid with more tiles than one workgroup compacts (61440 tiles of the 16x8x4
     shape): the relabel and closure lists take the multi-workgroup compaction, the discharge lists
     the ordered pieces path. Capacities on the device from the volume; S_min and V minus T_min are
-    both minimum cuts, nested, and the flow matches a solve with 32x4x4 tiles (122880 tiles)."""
+    both minimum cuts, nested, and the flow and S_min match a solve with 16x4x4 tiles (122880 tiles,
+    four pieces a thread in the ordered compaction)."""
     nx, ny, nz = 384, 320, 256
     say("grid 6C 384x320x256")
     vol = cugip_amd.generate_volume(nx, ny, nz, seed=3)
@@ -447,7 +464,7 @@ def test_6c_above_the_one_workgroup_compaction():
         assert g.cut_capacity(smin) == flow
         assert g.cut_capacity(1 - tmin) == flow
         assert not (smin.astype(bool) & tmin.astype(bool)).any()
-        g.set_policy(tile_x=32, tile_y=4, tile_z=4)
+        g.set_policy(tile_x=16, tile_y=4, tile_z=4)
         assert g.max_flow() == flow
         assert digest(g.source_set()) == digest(smin)
 
@@ -472,3 +489,62 @@ def test_full_size_26c_512x512x384():
         assert not (smin.astype(bool) & tmin.astype(bool)).any()
         assert g.max_flow() == flow
         assert digest(g.source_set()) == digest(smin)
+
+
+@pytest.mark.parametrize("conn,dims,seed,engine", [
+    (6, (48, 40, 36), 0, cugip_amd.ENGINE_GRAPH),
+    (6, (48, 40, 36), 1, cugip_amd.ENGINE_TILES),
+    (6, (64, 64, 64), 2, cugip_amd.ENGINE_AUTO),
+    (26, (32, 28, 20), 3, cugip_amd.ENGINE_GRAPH),
+    (26, (30, 22, 17), 4, cugip_amd.ENGINE_TILES),
+])
+def test_labels_are_sink_distances(conn, dims, seed, engine):
+    """cgp_labels after a solve are the exact BFS distances to t in the final residual graph: the
+    oracle's relabel run on the residual planes the device holds (cgp_grid_residuals). The
+    residuals also keep every edge's capacity pair: r(v->u) + r(u->v) = c(v->u) + c(u->v)."""
+    nx, ny, nz = dims
+    say(f"labels vs residual BFS {dims} conn {conn} seed {seed} engine {engine}")
+    vol, cs, ct, cn = cugip_amd.make_instance(nx, ny, nz, seed, 0, conn)
+    with cugip_amd.GridGraph() as g:
+        g.set_grid(cs, ct, cn, conn=conn)
+        g.set_policy(engine=engine)
+        g.max_flow()
+        labels = g.labels().reshape(-1)
+        r, rt = g.residuals()
+    ref = oracle.residual_labels(nx, ny, nz, conn, r, rt)
+    assert np.array_equal(labels, ref), int((labels != ref).sum())
+    assert (ref < cugip_amd.LABEL_UNREACHABLE).any() and (ref == cugip_amd.LABEL_UNREACHABLE).any()
+    H = conn // 2
+    for j, (dx, dy, dz) in enumerate(oracle.half_offsets(conn)):
+        zd = slice(max(0, -dz), nz - max(0, dz))
+        yd = slice(max(0, -dy), ny - max(0, dy))
+        xd = slice(max(0, -dx), nx - max(0, dx))
+        zs, ys, xs = (slice(a.start + o, a.stop + o) for a, o in ((zd, dz), (yd, dy), (xd, dx)))
+        pair = r[j][zd, yd, xd].astype(np.int64) + r[j + H][zs, ys, xs]
+        assert np.array_equal(pair, 2 * cn[j][zd, yd, xd].astype(np.int64)), j
+
+
+@pytest.mark.parametrize("conn,dims,seed", [(6, (40, 36, 28), 5), (26, (24, 20, 16), 6)])
+def test_first_relabel_labels(conn, dims, seed):
+    """The labels of the first global relabel (read from the trace hook after it, before any push)
+    equal the oracle's restatement of the reference's first BFS (initBFSKernel + level sweeps,
+    graph_cut_relabeling.hpp:47-112) on the initial residual graph, the sink t-links taken after the
+    per-voxel cancellation min(cs, ct) that the init kernel sends straight to t."""
+    nx, ny, nz = dims
+    say(f"first relabel labels {dims} conn {conn} seed {seed}")
+    vol, cs, ct, cn = cugip_amd.make_instance(nx, ny, nz, seed, 0, conn)
+    captured = {}
+    with cugip_amd.GridGraph() as g:
+        g.set_grid(cs, ct, cn, conn=conn)
+
+        def hook(event, iteration, done, flow, starts):
+            if event == cugip_amd.TRACE_AFTER_RELABEL and iteration == 0 and "labels" not in captured:
+                captured["labels"] = g.grid_state()[1].copy()
+
+        g.set_trace(hook)
+        g.max_flow()
+        g.set_trace(None)
+    ct2 = (ct.astype(np.int64) - np.minimum(cs, ct)).astype(np.int32)
+    ea, eb, fwd, bwd = oracle.grid_edges(nx, ny, nz, conn, cn)
+    ref = oracle.initial_labels(nx * ny * nz, ea, eb, fwd, bwd, ct2.reshape(-1))
+    assert np.array_equal(captured["labels"], ref), int((captured["labels"] != ref).sum())

@@ -209,3 +209,40 @@ def test_preflow_closure_equals_source_reachability():
         r = oracle.refpr(*g)
         f, s, t = oracle.dinic(*g)
         assert r["flow"] == f and np.array_equal(r["smin"], s) and np.array_equal(r["tmin"], t)
+
+
+@pytest.mark.parametrize("conn,seed", [(6, 0), (6, 1), (26, 2), (26, 3)])
+def test_residual_labels_vs_explicit_bfs(conn, seed):
+    """oracle.residual_labels (the C relabel on the residual planes' edge list) equals a BFS written
+    out voxel by voxel over the neighbour offsets, on random residual planes with zeros (entries
+    whose neighbour lies outside the volume carry values that must be ignored)."""
+    from collections import deque
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = (int(v) for v in rng.integers(2, 7, 3))
+    r = (rng.integers(0, 5, (conn, nz, ny, nx)) * (rng.random((conn, nz, ny, nx)) < 0.45)).astype(np.int32)
+    rt = (rng.integers(0, 3, (nz, ny, nx)) * (rng.random((nz, ny, nx)) < 0.2)).astype(np.int32)
+    offs = oracle.half_offsets(conn)
+    offs = offs + [(-dx, -dy, -dz) for dx, dy, dz in offs]
+    inf = 0x3FFFFFFF
+    lab = np.full((nz, ny, nx), inf, np.int64)
+    q = deque()
+    for z, y, x in zip(*np.nonzero(rt > 0)):
+        lab[z, y, x] = 1
+        q.append((z, y, x))
+    while q:
+        z, y, x = q.popleft()
+        for (vz, vy, vx) in [(z - dz, y - dy, x - dx) for dx, dy, dz in offs]:
+            if not (0 <= vx < nx and 0 <= vy < ny and 0 <= vz < nz) or lab[vz, vy, vx] != inf:
+                continue
+            # v reaches u = (z, y, x) when the residual v -> u > 0: u = v + o_d for the d with that offset
+            d = offs.index((x - vx, y - vy, z - vz))
+            if r[d, vz, vy, vx] > 0:
+                lab[vz, vy, vx] = lab[z, y, x] + 1
+                q.append((vz, vy, vx))
+    got = oracle.residual_labels(nx, ny, nz, conn, r, rt)
+    assert np.array_equal(got, lab.reshape(-1).astype(np.int32))
+    # a planted mistake (the mirror planes read at v instead of v + o_j) is caught
+    if conn == 6:
+        wrong = r.copy()
+        wrong[3:] = np.roll(wrong[3:], 1, axis=3)
+        assert not np.array_equal(oracle.residual_labels(nx, ny, nz, conn, wrong, rt), got) or r[3:].sum() == 0
