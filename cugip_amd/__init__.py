@@ -149,6 +149,7 @@ _lib.cgp_set_grid_volume.argtypes = [_vp] + [ctypes.c_int32] * 4 + [_vp, _vp, ct
 _lib.cgp_grid_capacities.argtypes = [_vp, ctypes.c_int32, _i32p, _i32p, _i32p, _i32p]
 _lib.cgp_grid_state.argtypes = [_vp, ctypes.c_int32, _i32p, _i32p, ctypes.POINTER(ctypes.c_uint32)]
 _lib.cgp_grid_residuals.argtypes = [_vp, ctypes.c_int32, _i32p, _i32p]
+_lib.cgp_grid_max_flow_noalloc.argtypes = [ctypes.c_int32] * 4 + [_vp] * 5 + [ctypes.POINTER(ctypes.c_int64), _vp]
 _lib.cgp_save_graphml.argtypes = [_vp, ctypes.c_char_p]
 _lib.cgp_cut_capacity.argtypes = [_vp, ctypes.c_int32, _u8p, ctypes.POINTER(ctypes.c_int64)]
 _lib.cgp_cnn_create.argtypes = [ctypes.POINTER(_vp)]
@@ -711,6 +712,19 @@ def batch_shapes(count, seed=0, lo=32, hi=128):
         nx, ny, nz = dims
         shapes.append((nz, ny, nx))
     return shapes
+
+
+def max_flow_noalloc(shape, conn, cs_ptr, ct_ptr, cn_ptr, cnb_ptr=None, source_set_ptr=None, stream=None):
+    """cgp_grid_max_flow_noalloc: one grid solve on device capacity arrays (raw device pointers,
+    layout of set_grid) whose working memory is the library's fixed per-device scratch; S_min goes
+    to source_set_ptr (device bytes) when given. Returns the flow."""
+    nz, ny, nx = shape
+    flow = ctypes.c_int64(0)
+    rc = _lib.cgp_grid_max_flow_noalloc(nx, ny, nz, conn, _vp(cs_ptr), _vp(ct_ptr), _vp(cn_ptr), _vp(cnb_ptr),
+                                        _vp(source_set_ptr), ctypes.byref(flow), _vp(stream))
+    if rc != CGP_OK:
+        raise CgpError(rc, _lib.cgp_last_error(None).decode(errors="replace"))
+    return int(flow.value)
 
 
 def make_instance(nx, ny, nz, seed, instance=0, conn=6):

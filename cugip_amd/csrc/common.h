@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "cugip_hip.h"
 
@@ -33,7 +34,26 @@ inline void hip_check(hipError_t err, const char *expr, const char *file, int li
 #define CGP_HIP(expr) ::cgp::hip_check((expr), #expr, __FILE__, __LINE__)
 #define CGP_LAUNCH_CHECK() ::cgp::hip_check(hipGetLastError(), "kernel launch", __FILE__, __LINE__)
 
-// Owned device allocation (hipMalloc / hipFree), movable, not copyable.
+// A fixed block of device memory handed out front to back (256-byte aligned pieces), never freed
+// piecewise: the memory of the no-alloc entry point. take() fails with CGP_ERR_INVALID when the
+// block is exhausted.
+struct Arena {
+	char *base = nullptr;
+	size_t size = 0, used = 0;
+	void *take(size_t bytes)
+	{
+		const size_t start = (used + 255) / 256 * 256;
+		if (start > size || bytes > size - start) {
+			fail(CGP_ERR_INVALID, "scratch exhausted: " + std::to_string(start + bytes) + " bytes needed, " +
+			                          std::to_string(size) + " available");
+		}
+		used = start + bytes;
+		return base + start;
+	}
+};
+
+// Device allocation (hipMalloc / hipFree), movable, not copyable; or a borrowed piece of memory
+// it does not free (an arena piece, a caller's array).
 template <typename T>
 class DeviceBuffer {
 public:
@@ -41,7 +61,7 @@ public:
 	explicit DeviceBuffer(size_t count) { allocate(count); }
 	DeviceBuffer(const DeviceBuffer &) = delete;
 	DeviceBuffer &operator=(const DeviceBuffer &) = delete;
-	DeviceBuffer(DeviceBuffer &&other) noexcept : mPtr(other.mPtr), mCount(other.mCount)
+	DeviceBuffer(DeviceBuffer &&other) noexcept : mPtr(other.mPtr), mCount(other.mCount), mOwned(other.mOwned)
 	{
 		other.mPtr = nullptr;
 		other.mCount = 0;
@@ -52,6 +72,7 @@ public:
 			release();
 			mPtr = other.mPtr;
 			mCount = other.mCount;
+			mOwned = other.mOwned;
 			other.mPtr = nullptr;
 			other.mCount = 0;
 		}
@@ -59,9 +80,29 @@ public:
 	}
 	~DeviceBuffer() { release(); }
 
+	// count elements from `arena` when it is given (no allocation), else from hipMalloc
+	void allocate(size_t count, Arena *arena)
+	{
+		if (!arena) {
+			allocate(count);
+			return;
+		}
+		release();
+		if (count) {
+			borrow(static_cast<T *>(arena->take(count * sizeof(T))), count);
+		}
+	}
+	// use memory owned elsewhere (not freed here)
+	void borrow(T *ptr, size_t count)
+	{
+		release();
+		mPtr = ptr;
+		mCount = ptr ? count : 0;
+		mOwned = false;
+	}
 	void allocate(size_t count)
 	{
-		if (count == mCount && mPtr) {
+		if (count == mCount && mPtr && mOwned) {
 			return;
 		}
 		release();
@@ -77,14 +118,16 @@ public:
 		}
 		mPtr = static_cast<T *>(p);
 		mCount = count;
+		mOwned = true;
 	}
 	void release()
 	{
-		if (mPtr) {
+		if (mPtr && mOwned) {
 			(void)hipFree(mPtr);
 		}
 		mPtr = nullptr;
 		mCount = 0;
+		mOwned = true;
 	}
 	T *get() const { return mPtr; }
 	size_t size() const { return mCount; }
@@ -93,41 +136,52 @@ public:
 private:
 	T *mPtr = nullptr;
 	size_t mCount = 0;
+	bool mOwned = true;
 };
 
-// Pinned host staging buffer (hipHostMalloc).
+// Host staging buffer for device reads: pinned (hipHostMalloc), or with `pageable` plain host
+// memory (no HIP allocation; copies into it are slower).
 template <typename T>
 class PinnedBuffer {
 public:
 	PinnedBuffer() = default;
 	PinnedBuffer(const PinnedBuffer &) = delete;
 	PinnedBuffer &operator=(const PinnedBuffer &) = delete;
-	~PinnedBuffer()
+	~PinnedBuffer() { release(); }
+	void allocate(size_t count, bool pageable = false)
 	{
-		if (mPtr) {
-			(void)hipHostFree(mPtr);
-		}
-	}
-	void allocate(size_t count)
-	{
-		if (count <= mCount && mPtr) {
+		if (count <= mCount && mPtr && pageable == mPageable) {
 			return;
 		}
-		if (mPtr) {
-			(void)hipHostFree(mPtr);
-			mPtr = nullptr;
+		release();
+		if (pageable) {
+			mHost.assign(count, T());
+			mPtr = mHost.data();
+		} else {
+			void *p = nullptr;
+			CGP_HIP(hipHostMalloc(&p, count * sizeof(T), hipHostMallocDefault));
+			mPtr = static_cast<T *>(p);
 		}
-		void *p = nullptr;
-		CGP_HIP(hipHostMalloc(&p, count * sizeof(T), hipHostMallocDefault));
-		mPtr = static_cast<T *>(p);
 		mCount = count;
+		mPageable = pageable;
 	}
 	T *get() const { return mPtr; }
 	T &operator[](size_t i) const { return mPtr[i]; }
 
 private:
+	void release()
+	{
+		if (mPtr && !mPageable) {
+			(void)hipHostFree(mPtr);
+		}
+		mHost.clear();
+		mPtr = nullptr;
+		mCount = 0;
+	}
 	T *mPtr = nullptr;
 	size_t mCount = 0;
+	bool mPageable = false;
+	std::vector<T> mHost;
 };
 
 // ---- device helpers (wave64) ----------------------------------------------------------------

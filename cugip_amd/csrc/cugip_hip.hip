@@ -579,6 +579,19 @@ int cgp_max_flow_certified(cgp_graph g, int64_t certificate, int64_t *flow, cgp_
 	});
 }
 
+int cgp_grid_max_flow_noalloc(int32_t nx, int32_t ny, int32_t nz, int32_t conn, const int32_t *cap_source,
+                              const int32_t *cap_sink, const int32_t *cap_n, const int32_t *cap_n_back,
+                              uint8_t *source_set, int64_t *flow, void *stream)
+{
+	return guarded_global([&] {
+		if (!flow) {
+			cgp::fail(CGP_ERR_INVALID, "flow pointer is null");
+		}
+		*flow = cgp::noalloc_max_flow(nx, ny, nz, conn, cap_source, cap_sink, cap_n, cap_n_back, source_set,
+		                              static_cast<hipStream_t>(stream));
+	});
+}
+
 int cgp_set_grid_batch(cgp_graph g, int32_t count, int32_t conn, const cgp_grid_instance *grids,
                        int32_t device_pointers)
 {

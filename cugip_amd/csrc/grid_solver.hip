@@ -5,6 +5,8 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <memory>
+#include <mutex>
 
 #include "grid_kernels.h"
 #include "solver.h"
@@ -198,7 +200,7 @@ std::vector<InstanceDesc> GridSolver::describe(int conn, const GridSpec *specs, 
 	return inst;
 }
 
-void GridSolver::allocate_grids(int conn, const std::vector<InstanceDesc> &inst, bool has_back)
+void GridSolver::allocate_grids(int conn, const std::vector<InstanceDesc> &inst, bool has_back, const GridSpec *borrow)
 {
 	int64_t total = 0;
 	for (const InstanceDesc &d : inst) {
@@ -221,29 +223,41 @@ void GridSolver::allocate_grids(int conn, const std::vector<InstanceDesc> &inst,
 	// until an init status says otherwise, the device-driven solve stages residuals as int16 (it
 	// switches when the init kernel finds an edge whose capacity pair exceeds 32767)
 	mWideResiduals = false;
-	mCs.allocate(total);
-	mCt.allocate(total);
-	mCn.allocate((size_t)total * half);
-	if (mHasBack) {
-		mCnb.allocate((size_t)total * half);
-	} else {
-		mCnb.release();
+	if (mArena) {
+		mArena->used = 0;  // everything of the last grid is re-carved
 	}
-	mE.allocate(total);
-	mRt.allocate(total);
-	mH.allocate(total);
-	mR.allocate((size_t)total * conn);
-	mS.allocate(total);
-	mOmask.allocate(total);
-	mFlow.allocate(count * kSlotWords);
-	mFlowTotal.allocate(count);
-	mCounters.allocate(3 * kSlotWords);
-	mCounterTotal.allocate(3);
-	mStatus.allocate(1);
-	mCount.allocate(4);  // [0] list count, [1] [2] count_two, [3] closure reached t
+	if (borrow) {
+		// the caller's capacity arrays, read in place (kernels never write them)
+		mCs.borrow(const_cast<int32_t *>(borrow->cs), total);
+		mCt.borrow(const_cast<int32_t *>(borrow->ct), total);
+		mCn.borrow(const_cast<int32_t *>(borrow->cn), (size_t)total * half);
+		mCnb.borrow(const_cast<int32_t *>(borrow->cnb), mHasBack ? (size_t)total * half : 0);
+	} else {
+		mCs.allocate(total, mArena);
+		mCt.allocate(total, mArena);
+		mCn.allocate((size_t)total * half, mArena);
+		if (mHasBack) {
+			mCnb.allocate((size_t)total * half, mArena);
+		} else {
+			mCnb.release();
+		}
+	}
+	mE.allocate(total, mArena);
+	mRt.allocate(total, mArena);
+	mH.allocate(total, mArena);
+	mR.allocate((size_t)total * conn, mArena);
+	mS.allocate(total, mArena);
+	mOmask.allocate(total, mArena);
+	mFlow.allocate(count * kSlotWords, mArena);
+	mFlowTotal.allocate(count, mArena);
+	mCounters.allocate(3 * kSlotWords, mArena);
+	mCounterTotal.allocate(3, mArena);
+	mStatus.allocate(1, mArena);
+	mCount.allocate(4, mArena);  // [0] list count, [1] [2] count_two, [3] closure reached t
 	// control block of the device-driven solve, then its visit-counter and per-grid flow totals
-	mCtl.allocate(kCtlTotals + 3 + count);
-	mHostScratch.allocate(kCtlTotals + 3 + count);  // read-backs: flows + visit counters, or the control block
+	mCtl.allocate(kCtlTotals + 3 + count, mArena);
+	// read-backs: flows + visit counters, or the control block (pageable host memory in arena mode)
+	mHostScratch.allocate(kCtlTotals + 3 + count, mArena != nullptr);
 	++mGeneration;
 }
 
@@ -274,6 +288,17 @@ void GridSolver::set_batch(int conn, const GridSpec *specs, int count, bool devi
 		}
 	}
 	CGP_HIP(hipStreamSynchronize(mStream));
+}
+
+void GridSolver::set_grid_in_place(int nx, int ny, int nz, int conn, const int32_t *cs, const int32_t *ct,
+                                   const int32_t *cn, const int32_t *cnb)
+{
+	if (!cs || !ct || !cn) {
+		fail(CGP_ERR_INVALID, "capacity pointers must not be null");
+	}
+	const GridSpec spec{nx, ny, nz, cs, ct, cn, cnb};
+	const std::vector<InstanceDesc> inst = describe(conn, &spec, 1);
+	allocate_grids(conn, inst, cnb != nullptr, &spec);
 }
 
 void GridSolver::set_grid_volume(int nx, int ny, int nz, int conn, const uint8_t *volume, const uint8_t *markers,
@@ -411,7 +436,7 @@ void GridSolver::build_colours(ColourSet &set, int ncolours, bool corners)
 	// largest colour class: the grid of a pass must cover every tile of one colour
 	set.max_size = *std::max_element(per_colour.begin(), per_colour.end());
 	set.colours = ncolours;
-	set.colour.allocate(((size_t)mTiles + 31) / 32 * 32);
+	set.colour.allocate(((size_t)mTiles + 31) / 32 * 32, mArena);
 	CGP_HIP(hipMemcpyAsync(set.colour.get(), colour.data(), colour.size(), hipMemcpyHostToDevice, mStream));
 	set.start.assign(ncolours, 0);
 	set.size.assign(per_colour.begin(), per_colour.end());
@@ -426,7 +451,7 @@ void GridSolver::build_colours(ColourSet &set, int ncolours, bool corners)
 	for (int c = 0; c < ncolours; ++c) {
 		grouped[(size_t)mTiles + c] = set.size[c];
 	}
-	set.tiles.allocate(grouped.size());
+	set.tiles.allocate(grouped.size(), mArena);
 	CGP_HIP(hipMemcpyAsync(set.tiles.get(), grouped.data(), grouped.size() * sizeof(int32_t), hipMemcpyHostToDevice,
 	                       mStream));
 	CGP_HIP(hipStreamSynchronize(mStream));  // the host vectors go out of scope
@@ -465,12 +490,12 @@ void GridSolver::configure_tiles(const Shape &shape)
 	}
 	// flag arrays padded to whole 32-byte chunks: the one-workgroup compaction reads them 16 bytes at a time
 	const int64_t padded = ((int64_t)mTiles + 31) / 32 * 32;
-	mActive.allocate(padded);
-	mTileWork.allocate(padded);
-	mDirtyA.allocate(padded);
-	mDirtyB.allocate(padded);
-	mList.allocate(mTiles);
-	mAllTiles.allocate(mTiles + 1);
+	mActive.allocate(padded, mArena);
+	mTileWork.allocate(padded, mArena);
+	mDirtyA.allocate(padded, mArena);
+	mDirtyB.allocate(padded, mArena);
+	mList.allocate(mTiles, mArena);
+	mAllTiles.allocate(mTiles + 1, mArena);
 	std::vector<int32_t> all(mTiles + 1);
 	for (int t = 0; t < mTiles; ++t) {
 		all[t] = t;
@@ -496,8 +521,8 @@ void GridSolver::configure_tiles(const Shape &shape)
 					owner_block[(size_t)b] = (int32_t)i;
 				}
 			}
-			mInitBlockInst.allocate((size_t)blocks);
-			mInitFirstBlock.allocate(mInst.size());
+			mInitBlockInst.allocate((size_t)blocks, mArena);
+			mInitFirstBlock.allocate(mInst.size(), mArena);
 			CGP_HIP(hipMemcpyAsync(mInitBlockInst.get(), owner_block.data(), owner_block.size() * sizeof(int32_t),
 			                       hipMemcpyHostToDevice, mStream));
 			CGP_HIP(hipMemcpyAsync(mInitFirstBlock.get(), first.data(), first.size() * sizeof(int32_t),
@@ -506,8 +531,8 @@ void GridSolver::configure_tiles(const Shape &shape)
 		}
 	}
 	if (mInst.size() > 1) {
-		mInstDev.allocate(mInst.size());
-		mTileInst.allocate(mTiles);
+		mInstDev.allocate(mInst.size(), mArena);
+		mTileInst.allocate(mTiles, mArena);
 		CGP_HIP(hipMemcpyAsync(mInstDev.get(), mInst.data(), mInst.size() * sizeof(InstanceDesc),
 		                       hipMemcpyHostToDevice, mStream));
 		CGP_HIP(hipMemcpyAsync(mTileInst.get(), owner.data(), owner.size() * sizeof(int32_t), hipMemcpyHostToDevice,
@@ -1269,6 +1294,16 @@ void GridSolver::source_side(int index, uint8_t *host_out)
 	CGP_HIP(hipStreamSynchronize(mStream));
 }
 
+void GridSolver::source_side_device(int index, uint8_t *out)
+{
+	if (!mSolved) {
+		fail(CGP_ERR_STATE, "source_set before max_flow");
+	}
+	const InstanceDesc &d = instance(index);
+	CGP_HIP(hipMemcpyAsync(out, mS.get() + d.vbase, d.nvox, hipMemcpyDeviceToDevice, mStream));
+	CGP_HIP(hipStreamSynchronize(mStream));
+}
+
 void GridSolver::labels(int index, int32_t *host_out)
 {
 	if (!mSolved) {
@@ -1349,6 +1384,78 @@ const uint8_t *GridSolver::device_source_side(int index)
 	}
 	const InstanceDesc &d = instance(index);
 	return mS.get() + d.vbase;
+}
+
+// ---- no-alloc solve ------------------------------------------------------------------------------
+// A grid solve that allocates nothing: every device buffer of the solver (per-voxel state, tile
+// flags and lists, counters) is a piece of `scratch`, a file-scope device array of a fixed size
+// that each GPU the library runs on holds until the process ends. In host code `scratch` names a
+// host-side stand-in; the device address of each GPU's copy comes from hipGetSymbolAddress, called
+// once per device with that device current, and only that address is ever used.
+
+namespace {
+
+__device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+constexpr int kNoAllocMaxDevices = 64;
+
+struct NoAllocDevice {
+	std::mutex lock;           // calls on one device run one at a time
+	bool ready = false;
+	Arena arena;               // over this device's copy of `scratch`
+	std::unique_ptr<GridSolver> solver;
+};
+
+NoAllocDevice g_noalloc[kNoAllocMaxDevices];
+
+NoAllocDevice &noalloc_device(int device)
+{
+	if (device < 0 || device >= kNoAllocMaxDevices) {
+		fail(CGP_ERR_DEVICE, "device " + std::to_string(device) + " out of range for the no-alloc entry point");
+	}
+	return g_noalloc[device];
+}
+
+}  // namespace
+
+int64_t noalloc_max_flow(int nx, int ny, int nz, int conn, const int32_t *cs, const int32_t *ct, const int32_t *cn,
+                         const int32_t *cnb, uint8_t *source_set, hipStream_t stream)
+{
+	int device = 0;
+	CGP_HIP(hipGetDevice(&device));
+	NoAllocDevice &nd = noalloc_device(device);
+	std::lock_guard<std::mutex> guard(nd.lock);
+	if (!nd.ready) {
+		// set-up, once per device, with that device current: the address of its copy of the array
+		void *ptr = nullptr;
+		CGP_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+		nd.arena.base = static_cast<char *>(ptr);
+		nd.arena.size = kNoAllocScratchBytes;
+		nd.arena.used = 0;
+		nd.solver.reset(new GridSolver(stream, &nd.arena));
+		nd.ready = true;
+	}
+	GridSolver &solver = *nd.solver;
+	solver.set_stream(stream);
+	try {
+		solver.set_grid_in_place(nx, ny, nz, conn, cs, ct, cn, cnb);
+		cgp_policy policy = default_policy();
+		policy.engine = CGP_ENGINE_TILES;  // plain launches: capturing a HIP graph may allocate
+		cgp_stats stats;
+		const int64_t flow = solver.solve(policy, &stats, TraceHook());
+		if (source_set) {
+			solver.source_side_device(0, source_set);
+		}
+		return flow;
+	} catch (const Error &e) {
+		if (e.code == CGP_ERR_INVALID && std::string(e.what()).rfind("scratch exhausted", 0) == 0) {
+			fail(CGP_ERR_INVALID, "a " + std::to_string(nx) + "x" + std::to_string(ny) + "x" + std::to_string(nz) +
+			                          " " + std::to_string(conn) + "-connected grid does not fit the " +
+			                          std::to_string(kNoAllocScratchBytes >> 20) + " MiB scratch of the no-alloc "
+			                          "entry point (" + e.what() + ")");
+		}
+		throw;
+	}
 }
 
 }  // namespace cgp

@@ -36,6 +36,18 @@ struct GridSpec {
 class GridSolver {
 public:
 	explicit GridSolver(hipStream_t stream) : mStream(stream) {}
+	// A solver whose device memory all comes from `arena` (a fixed block it never frees), with
+	// pageable host staging: it calls no allocating HIP routine. The memory of one grid is carved
+	// afresh by every set_grid_in_place.
+	GridSolver(hipStream_t stream, Arena *arena) : mStream(stream), mArena(arena) {}
+	void set_stream(hipStream_t stream) { mStream = stream; }
+	// A single grid whose capacities are device arrays the solver reads in place (no copies); they
+	// must stay valid and unchanged while the solver uses them.
+	void set_grid_in_place(int nx, int ny, int nz, int conn, const int32_t *cs, const int32_t *ct, const int32_t *cn,
+	                       const int32_t *cnb);
+	// The source side S_min of grid `index` after a solve, as device bytes (copied on the solver's
+	// stream into `out`, device memory of its voxel count).
+	void source_side_device(int index, uint8_t *out);
 
 	void set_grid(int nx, int ny, int nz, int conn, const int32_t *cs, const int32_t *ct, const int32_t *cn,
 	              const int32_t *cnb, bool device_pointers);
@@ -141,9 +153,12 @@ private:
 	unsigned long long read_counter(int kind);
 	const InstanceDesc &instance(int index) const;
 	std::vector<InstanceDesc> describe(int conn, const GridSpec *specs, int count);
-	void allocate_grids(int conn, const std::vector<InstanceDesc> &inst, bool has_back);
+	// borrow (single grid): read the caller's device capacity arrays in place instead of copies
+	void allocate_grids(int conn, const std::vector<InstanceDesc> &inst, bool has_back,
+	                    const GridSpec *borrow = nullptr);
 
 	hipStream_t mStream;
+	Arena *mArena = nullptr;  // no-alloc mode: every device buffer is a piece of this block
 	int mNx = 0, mNy = 0, mNz = 0, mConn = 0;  // dimensions of a single grid (instance 0)
 	int64_t mVox = 0;                          // voxels of all instances
 	std::vector<InstanceDesc> mInst;           // host copy of the instance table
@@ -198,6 +213,15 @@ private:
 	KernelRecord mProf[kKernCount];
 	hipEvent_t mEvStart = nullptr, mEvStop = nullptr;
 };
+
+// Scratch of the no-alloc grid solve, per device (CGP_NOALLOC_SCRATCH_BYTES in the C ABI).
+constexpr size_t kNoAllocScratchBytes = (size_t)CGP_NOALLOC_SCRATCH_MIB << 20;
+
+// Solves one grid whose capacities are device arrays, with the solver's device memory taken from
+// a fixed per-device scratch array instead of allocations; writes S_min (device bytes, may be
+// null) and returns the flow. Calls on one device are serialised.
+int64_t noalloc_max_flow(int nx, int ny, int nz, int conn, const int32_t *cs, const int32_t *ct, const int32_t *cn,
+                         const int32_t *cnb, uint8_t *source_set, hipStream_t stream);
 
 // Generic graph (CSR), the data layout of the reference's Graph<TFlow>.
 class CsrSolver {

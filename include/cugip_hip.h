@@ -324,6 +324,25 @@ int cgp_generate_volume(int32_t nx, int32_t ny, int32_t nz, uint64_t seed, uint6
 int cgp_caps_from_volume(int32_t nx, int32_t ny, int32_t nz, int32_t conn, const uint8_t *volume,
                          int32_t *cap_source, int32_t *cap_sink, int32_t *cap_n);
 
+/* ---- solve without allocating ---------------------------------------------------------------
+   No counterpart in the reference; for callers that cannot allocate device memory (tight serving
+   loops, harnesses that forbid hipMalloc). One grid solve whose working memory (per-voxel residual
+   state, tile flags and lists, counters) is a fixed scratch array of CGP_NOALLOC_SCRATCH_MIB MiB
+   that each GPU the library runs on holds from its first such call until the process ends; the
+   call itself makes no device or pinned allocation. Capacities: device arrays in the layout of
+   cgp_set_grid, read in place (not copied). source_set (device memory of nx*ny*nz bytes, may be
+   NULL) receives S_min as 0/1 bytes, flow the exact max-flow value. stream: a hipStream_t (NULL =
+   the default stream); the call returns when its work on it is done. A grid whose state does not
+   fit the scratch (41 bytes per voxel 6-connected, 121 bytes 26-connected, plus a few bytes per
+   tile: about 1.6 M or 0.55 M voxels) fails with CGP_ERR_INVALID. Every call on one GPU uses that
+   GPU's one scratch array, which keeps the last call's contents (the solve clears what it needs);
+   the library runs such calls one at a time, so calls from several host threads queue. Errors:
+   cgp_last_error(NULL). */
+#define CGP_NOALLOC_SCRATCH_MIB 64
+int cgp_grid_max_flow_noalloc(int32_t nx, int32_t ny, int32_t nz, int32_t conn, const int32_t *cap_source,
+                              const int32_t *cap_sink, const int32_t *cap_n, const int32_t *cap_n_back,
+                              uint8_t *source_set, int64_t *flow, void *stream);
+
 /* ---- device ------------------------------------------------------------------------------- */
 int cgp_device_count(int32_t *count);
 /* Version of this ABI; bumps on any incompatible change. */
