@@ -189,6 +189,69 @@ def batch_workload(cugip_amd, count, seed, repeats=3, cpu=True):
     return out
 
 
+def random_csr_graph(n, seed):
+    """A sparse irregular graph for the generic (CSR) path, shaped like a region adjacency graph:
+    the vertices sit on a k x k x k lattice (k = round(n^(1/3)), n = k^3 used), each joined to its
+    +x, +y and +z lattice neighbours and to one random vertex within 2 steps per axis, with
+    capacities 1..99 each way; a tenth of the vertices get a source t-link and a tenth a sink t-link
+    (0..399). Edge records are normalised to first < second (EdgeRecord, edge_record.hpp:10-29),
+    fwd being first -> second."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    k = max(2, int(round(n ** (1.0 / 3.0))))
+    n = k ** 3
+    z, y, x = (c.reshape(-1) for c in np.indices((k, k, k)))
+    v = np.arange(n, dtype=np.int64)
+    ends = []
+    for dx, dy, dz in [(1, 0, 0), (0, 1, 0), (0, 0, 1)]:
+        ok = (x + dx < k) & (y + dy < k) & (z + dz < k)
+        ends.append((v[ok], v[ok] + dx + dy * k + dz * k * k))
+    off = rng.integers(-2, 3, (n, 3))
+    xx, yy, zz = np.clip(x + off[:, 0], 0, k - 1), np.clip(y + off[:, 1], 0, k - 1), np.clip(z + off[:, 2], 0, k - 1)
+    u = (zz * k + yy) * k + xx
+    ok = u != v
+    ends.append((v[ok], u[ok]))
+    a = np.concatenate([e[0] for e in ends])
+    b = np.concatenate([e[1] for e in ends])
+    fwd = rng.integers(1, 100, a.size).astype(np.int32)
+    bwd = rng.integers(1, 100, a.size).astype(np.int32)
+    lo, hi = np.minimum(a, b), np.maximum(a, b)
+    f = np.where(a < b, fwd, bwd)
+    bw = np.where(a < b, bwd, fwd)
+    cs = (rng.integers(0, 400, n) * (rng.random(n) < 0.1)).astype(np.int32)
+    ct = (rng.integers(0, 400, n) * (rng.random(n) < 0.1)).astype(np.int32)
+    return n, lo.astype(np.int32), hi.astype(np.int32), f.astype(np.int32), bw.astype(np.int32), cs, ct
+
+
+def generic_graph(cugip_amd, n, seed, repeats=3, cpu=True):
+    """The generic (CSR) path on a random sparse graph: median solve time from HBM-resident arcs
+    (set_nweights excluded), checked against the CPU Boykov-Kolmogorov flow."""
+    import numpy as np
+    n, a, b, fwd, bwd, cs, ct = random_csr_graph(n, seed)
+    with cugip_amd.Graph() as g:
+        g.set_vertex_count(n)
+        g.set_nweights(np.stack([a, b], 1), fwd, bwd)
+        g.set_tweights(cs, ct)
+        g.max_flow()
+        ts = []
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            flow = g.max_flow()
+            ts.append(time.perf_counter() - t0)
+        st = g.stats.as_dict()
+    sec = statistics.median(ts)
+    out = dict(workload=f"random region-adjacency-like graph, {n} vertices on a lattice, {a.size} edges (lattice "
+                        f"neighbours plus one random edge within 2 steps per vertex), seed {seed}",
+               ms=round(sec * 1e3, 3), mvertices_s=round(n / sec / 1e6, 2), medges_s=round(a.size / sec / 1e6, 2),
+               flow=flow, rounds=st["rounds"], bfs_levels=st["bfs_passes"])
+    if cpu:
+        import oracle
+        f, _, _, t = oracle.bk(n, a, b, fwd, bwd, cs, ct)
+        out["cpu_bk"] = dict(seconds=round(t, 3), mvertices_s=round(n / t / 1e6, 3), cores=1, kind="port")
+        out["flow_matches_cpu"] = bool(f == flow)
+    return out
+
+
 def learned_vs_plain(cugip_amd, n, seed, count, weights, repeats=3):
     """The learned cut proposal on the batched-comparison grids: per grid, inference + certificate
     (capacity of the predicted cut) + the certified solve, against the plain solve; flows and S_min
@@ -314,6 +377,8 @@ def main():
     ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch", type=int, default=8, help="grids of the batched-vs-plain comparison (0: skip)")
+    ap.add_argument("--generic", type=int, default=1000000,
+                    help="vertices of the generic-graph (CSR) leg, about 4 edges each (0: skip)")
     ap.add_argument("--batch-workload", type=int, default=512,
                     help="grids of the batch workload leg, edge lengths log-uniform in [32, 128] (0: skip)")
     ap.add_argument("--calibration-copies", action="store_true",
@@ -444,6 +509,12 @@ def main():
         batch512 = batch_workload(cugip_amd, args.batch_workload, args.seed, cpu=not args.no_cpu)
         log(f"batch workload: {batch512}")
 
+    generic = None
+    if args.generic > 0 and not args.volume:
+        log(f"generic graph: {args.generic} vertices")
+        generic = generic_graph(cugip_amd, args.generic, args.seed, cpu=not args.no_cpu)
+        log(f"generic graph: {generic}")
+
     cpu = None
     if not args.no_cpu and not args.volume:
         cpu = cpu_baseline(nx, ny, nz, args.conn, cs, ct, cn, args.cpu_runs)
@@ -481,6 +552,7 @@ def main():
             "learned_vs_plain": learned,
             "learned_batched_speedup": learned_batched,
             "batch512": batch512,
+            "generic_graph": generic,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -149,6 +149,9 @@ _lib.cgp_set_grid_volume.argtypes = [_vp] + [ctypes.c_int32] * 4 + [_vp, _vp, ct
 _lib.cgp_grid_capacities.argtypes = [_vp, ctypes.c_int32, _i32p, _i32p, _i32p, _i32p]
 _lib.cgp_grid_state.argtypes = [_vp, ctypes.c_int32, _i32p, _i32p, ctypes.POINTER(ctypes.c_uint32)]
 _lib.cgp_grid_residuals.argtypes = [_vp, ctypes.c_int32, _i32p, _i32p]
+_lib.cgp_region_graph_info.argtypes = [ctypes.c_char_p, _i32p, _i32p]
+_lib.cgp_region_graph_read.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_double, ctypes.c_int32,
+                                       ctypes.POINTER(Edge), _i32p, _i32p, _i32p]
 _lib.cgp_grid_max_flow_noalloc.argtypes = [ctypes.c_int32] * 4 + [_vp] * 5 + [ctypes.POINTER(ctypes.c_int64), _vp]
 _lib.cgp_save_graphml.argtypes = [_vp, ctypes.c_char_p]
 _lib.cgp_cut_capacity.argtypes = [_vp, ctypes.c_int32, _u8p, ctypes.POINTER(ctypes.c_int64)]
@@ -725,6 +728,27 @@ def max_flow_noalloc(shape, conn, cs_ptr, ct_ptr, cn_ptr, cnb_ptr=None, source_s
     if rc != CGP_OK:
         raise CgpError(rc, _lib.cgp_last_error(None).decode(errors="replace"))
     return int(flow.value)
+
+
+def read_region_graph(graph_path, markers_path=None, weight_scale=1.0, marker_capacity=10000):
+    """A region adjacency graph file (and marker file) of the reference's region benchmarks, as the
+    arrays of Graph.set_nweights / set_tweights: (vertex_count, edges (m, 2) as (first, second),
+    weights (m,), cap_source, cap_sink); cgp_region_graph_read."""
+    n = ctypes.c_int32(0)
+    m = ctypes.c_int32(0)
+    _global_check(_lib.cgp_region_graph_info(os.fsencode(graph_path), ctypes.byref(n), ctypes.byref(m)))
+    n, m = int(n.value), int(m.value)
+    rec = (Edge * max(m, 1))()
+    w = np.empty(max(m, 1), np.int32)
+    cs = np.empty(n, np.int32)
+    ct = np.empty(n, np.int32)
+    _global_check(_lib.cgp_region_graph_read(os.fsencode(graph_path),
+                                             os.fsencode(markers_path) if markers_path else None,
+                                             float(weight_scale), int(marker_capacity), rec, _p32(w), _p32(cs),
+                                             _p32(ct)))
+    arr = np.frombuffer(rec, dtype=np.uint32, count=2 * max(m, 1)).reshape(-1, 2)[:m]
+    edges = np.stack([arr[:, 1], arr[:, 0]], axis=1).astype(np.int64)
+    return n, edges, w[:m].copy(), cs, ct
 
 
 def make_instance(nx, ny, nz, seed, instance=0, conn=6):

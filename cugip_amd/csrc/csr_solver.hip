@@ -79,6 +79,28 @@ __global__ __launch_bounds__(256) void k_csr_bfs_seed(int n, const int32_t *rt, 
 	}
 }
 
+// Append v to queue where pred holds, among the lanes active at the call (also inside loops whose
+// trip counts differ per lane): the lowest active lane reserves the entries of the whole wave with
+// one atomic, and each lane stores at its rank.
+__device__ inline void wave_append_active(bool pred, int v, int32_t *queue, int32_t *tail)
+{
+	const unsigned long long active = __ballot(1);
+	const unsigned long long ballot = __ballot(pred);
+	if (!ballot) {
+		return;  // (the same for every active lane)
+	}
+	const int lane = threadIdx.x & 63;
+	const int leader = __ffsll((long long)active) - 1;
+	int base = 0;
+	if (lane == leader) {
+		base = atomicAdd(tail, (int)__popcll(ballot));
+	}
+	base = __shfl(base, leader, 64);
+	if (pred) {
+		queue[base + (int)__popcll(ballot & ((1ull << lane) - 1ull))] = v;
+	}
+}
+
 // Label the unlabelled neighbours u of frontier vertex v that can reach v (residual u -> v > threshold).
 __device__ inline void expand_towards_sink(int v, int32_t level, const int32_t *first, const int32_t *head,
                                            const int32_t *mate, const int32_t *res, int32_t threshold, int32_t *h,
@@ -86,11 +108,34 @@ __device__ inline void expand_towards_sink(int v, int32_t level, const int32_t *
 {
 	for (int a = first[v]; a < first[v + 1]; ++a) {
 		const int u = head[a];
+		bool claimed = false;
 		if (h[u] == kLabelInf && res[mate[a]] > threshold) {
-			if (atomicCAS(&h[u], kLabelInf, level + 1) == kLabelInf) {
-				queue[atomicAdd(tail, 1)] = u;
-			}
+			claimed = atomicCAS(&h[u], kLabelInf, level + 1) == kLabelInf;
 		}
+		wave_append_active(claimed, u, queue, tail);
+	}
+}
+
+// The flow into t is summed in kSlots slots on separate cache lines, picked by workgroup: one
+// word taking an atomic from every workgroup serialises them in the L2.
+constexpr int kFlowSlots = 32;
+constexpr int kFlowSlotStride = 16;
+
+__device__ inline void add_flow(unsigned long long *flow, long long total)
+{
+	if (total) {
+		atomicAdd(flow + (blockIdx.x & (kFlowSlots - 1)) * kFlowSlotStride, (unsigned long long)total);
+	}
+}
+
+__global__ void k_csr_sum_flow(const unsigned long long *slots, unsigned long long *total)
+{
+	if (threadIdx.x == 0 && blockIdx.x == 0) {
+		unsigned long long sum = 0;
+		for (int j = 0; j < kFlowSlots; ++j) {
+			sum += slots[j * kFlowSlotStride];
+		}
+		*total = sum;
 	}
 }
 
@@ -184,8 +229,8 @@ __global__ __launch_bounds__(256) void k_csr_push_level(int32_t qs, int32_t qe, 
 		*pushed = 1;
 	}
 	const long long total = block_sum_i64<256>(sink, red);
-	if (threadIdx.x == 0 && total) {
-		atomicAdd(flow, (unsigned long long)total);
+	if (threadIdx.x == 0) {
+		add_flow(flow, total);
 	}
 }
 
@@ -217,9 +262,7 @@ __global__ __launch_bounds__(kMultiLevelThreads) void k_csr_push_multi(int32_t l
 	}
 	const long long total = block_sum_i64<kMultiLevelThreads>(sink, red);
 	if (threadIdx.x == 0) {
-		if (total) {
-			atomicAdd(flow, (unsigned long long)total);
-		}
+		add_flow(flow, total);
 		*next_level = level;
 	}
 }
@@ -250,9 +293,8 @@ __global__ __launch_bounds__(256) void k_csr_closure_level(int32_t qs, int32_t q
 		const int v = queue[i];
 		for (int a = first[v]; a < first[v + 1]; ++a) {
 			const int u = head[a];
-			if (res[a] > 0 && mark[u] == 0 && atomicCAS(&mark[u], 0, 1) == 0) {
-				queue[atomicAdd(tail, 1)] = u;
-			}
+			const bool claimed = res[a] > 0 && mark[u] == 0 && atomicCAS(&mark[u], 0, 1) == 0;
+			wave_append_active(claimed, u, queue, tail);
 		}
 	}
 }
@@ -305,7 +347,7 @@ void CsrSolver::set_vertex_count(int n)
 	mPushed.allocate(1);
 	mStatus.allocate(1);
 	mLevelStarts.allocate(kMaxLevelsPerLaunch + 2);
-	mFlow.allocate(1);
+	mFlow.allocate(kFlowSlots * kFlowSlotStride + 1);  // slots, then their sum
 	mPinned.allocate(kMaxLevelsPerLaunch + 8);
 	mHead.release();
 	mMate.release();
@@ -436,6 +478,18 @@ void CsrSolver::export_graph(std::vector<int32_t> &tail, std::vector<int32_t> &h
 	}
 }
 
+int64_t CsrSolver::read_flow()
+{
+	unsigned long long *total = mFlow.get() + kFlowSlots * kFlowSlotStride;
+	hipLaunchKernelGGL(k_csr_sum_flow, dim3(1), dim3(64), 0, mStream, (const unsigned long long *)mFlow.get(), total);
+	CGP_LAUNCH_CHECK();
+	CGP_HIP(hipMemcpyAsync(mPinned.get(), total, sizeof(unsigned long long), hipMemcpyDeviceToHost, mStream));
+	CGP_HIP(hipStreamSynchronize(mStream));
+	int64_t f;
+	std::memcpy(&f, mPinned.get(), sizeof(f));
+	return f;
+}
+
 int32_t CsrSolver::read_i32(const int32_t *device_ptr)
 {
 	CGP_HIP(hipMemcpyAsync(mPinned.get(), device_ptr, sizeof(int32_t), hipMemcpyDeviceToHost, mStream));
@@ -556,9 +610,12 @@ int64_t CsrSolver::solve(const cgp_policy &policy, cgp_stats *stats_out, const T
 	const auto t_start = std::chrono::steady_clock::now();
 	mSolved = false;
 	const int max_rounds = policy.max_rounds > 0 ? policy.max_rounds : 1 << 20;
-	int32_t threshold = policy.minimal_residual > 0.0f ? (int32_t)std::floor(policy.minimal_residual) : 0;
+	// (a threshold of 2^31 or more closes every arc, as INT32_MAX does)
+	int32_t threshold = policy.minimal_residual > 0.0f
+	                        ? (int32_t)std::min<double>(std::floor((double)policy.minimal_residual), 2147483647.0)
+	                        : 0;
 
-	CGP_HIP(hipMemsetAsync(mFlow.get(), 0, sizeof(unsigned long long), mStream));
+	CGP_HIP(hipMemsetAsync(mFlow.get(), 0, mFlow.bytes(), mStream));
 	hipLaunchKernelGGL(k_csr_init_vertices, dim3(grid_blocks(mN)), dim3(256), 0, mStream, mN, mCs.get(), mCt.get(),
 	                   mE.get(), mRt.get());
 	if (mArcs > 0) {
@@ -586,11 +643,7 @@ int64_t CsrSolver::solve(const cgp_policy &policy, cgp_stats *stats_out, const T
 		stats.discharge_ms += ms_since(t0);
 		const bool done = !pushed && threshold == 0;
 		if (trace.fn) {
-			CGP_HIP(hipMemcpyAsync(mPinned.get(), mFlow.get(), sizeof(unsigned long long), hipMemcpyDeviceToHost, mStream));
-			CGP_HIP(hipStreamSynchronize(mStream));
-			int64_t f;
-			std::memcpy(&f, mPinned.get(), sizeof(f));
-			trace.emit(CGP_TRACE_AFTER_PUSH, iteration, done ? 1 : 0, f);
+			trace.emit(CGP_TRACE_AFTER_PUSH, iteration, done ? 1 : 0, read_flow());
 		}
 		++iteration;
 		if (!pushed) {
@@ -603,10 +656,7 @@ int64_t CsrSolver::solve(const cgp_policy &policy, cgp_stats *stats_out, const T
 	}
 	auto t0 = std::chrono::steady_clock::now();
 	source_closure(stats);
-	CGP_HIP(hipMemcpyAsync(mPinned.get(), mFlow.get(), sizeof(unsigned long long), hipMemcpyDeviceToHost, mStream));
-	CGP_HIP(hipStreamSynchronize(mStream));
-	int64_t flow;
-	std::memcpy(&flow, mPinned.get(), sizeof(flow));
+	const int64_t flow = read_flow();
 	stats.cut_ms = ms_since(t0);
 	stats.flow = flow;
 	stats.total_ms = ms_since(t_start);

@@ -326,6 +326,38 @@ int cgp_write_mfi(const char *path, int32_t nx, int32_t ny, int32_t nz, int32_t 
 	});
 }
 
+int cgp_region_graph_info(const char *graph_path, int32_t *vertex_count, int32_t *edge_count)
+{
+	return guarded_global([&] {
+		if (!graph_path || !vertex_count || !edge_count) {
+			cgp::fail(CGP_ERR_INVALID, "null argument");
+		}
+		const cgp::RegionGraph g = cgp::read_region_graph(graph_path);
+		*vertex_count = g.vertices;
+		*edge_count = (int32_t)g.a.size();
+	});
+}
+
+int cgp_region_graph_read(const char *graph_path, const char *markers_path, double weight_scale,
+                          int32_t marker_capacity, cgp_edge *edges, int32_t *weights, int32_t *cap_source,
+                          int32_t *cap_sink)
+{
+	return guarded_global([&] {
+		if (!graph_path || !cap_source || !cap_sink) {
+			cgp::fail(CGP_ERR_INVALID, "null argument");
+		}
+		const cgp::RegionGraph g = cgp::read_region_graph(graph_path);
+		if (!g.a.empty() && (!edges || !weights)) {
+			cgp::fail(CGP_ERR_INVALID, "null edge arrays");
+		}
+		std::vector<std::pair<bool, int32_t>> markers;
+		if (markers_path) {
+			markers = cgp::read_region_markers(markers_path);
+		}
+		cgp::region_graph_arrays(g, markers, weight_scale, marker_capacity, edges, weights, cap_source, cap_sink);
+	});
+}
+
 int cgp_default_volume_weights(cgp_volume_weights *w)
 {
 	return guarded_global([&] {

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <cstdint>
+#include <utility>
 #include <vector>
 
 #include "common.h"
@@ -223,6 +224,19 @@ constexpr size_t kNoAllocScratchBytes = (size_t)CGP_NOALLOC_SCRATCH_MIB << 20;
 int64_t noalloc_max_flow(int nx, int ny, int nz, int conn, const int32_t *cs, const int32_t *ct, const int32_t *cn,
                          const int32_t *cnb, uint8_t *source_set, hipStream_t stream);
 
+// Region adjacency graph of the reference's region benchmarks (region_graph.cpp).
+struct RegionGraph {
+	int32_t vertices = 0;
+	std::vector<int32_t> a, b;  // 0-based region ids of each edge record
+	std::vector<double> sum;    // its boundary statistic
+};
+RegionGraph read_region_graph(const char *path);
+// (object marker?, 0-based region id) per marker line
+std::vector<std::pair<bool, int32_t>> read_region_markers(const char *path);
+void region_graph_arrays(const RegionGraph &g, const std::vector<std::pair<bool, int32_t>> &markers,
+                         double weight_scale, int32_t marker_capacity, cgp_edge *edges, int32_t *weights,
+                         int32_t *cap_source, int32_t *cap_sink);
+
 // Generic graph (CSR), the data layout of the reference's Graph<TFlow>.
 class CsrSolver {
 public:
@@ -247,6 +261,7 @@ private:
 	bool push_sweep(const std::vector<int32_t> &level_starts, cgp_stats &stats);
 	void source_closure(cgp_stats &stats);
 	int32_t read_i32(const int32_t *device_ptr);
+	int64_t read_flow();
 
 	hipStream_t mStream;
 	int mN = 0;

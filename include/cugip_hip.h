@@ -253,6 +253,24 @@ int cgp_write_mfi(const char *path, int32_t nx, int32_t ny, int32_t nz, int32_t 
                   const int32_t *cap_sink, const int32_t *cap_n, const int32_t *cap_n_back, int64_t maxflow,
                   const char *metadata);
 
+/* ---- region adjacency graphs ------------------------------------------------------------------
+   The text format of the reference's region benchmarks (benchmarks/graph_cut/graph.hpp:48-104,
+   loadGraph / saveGraph; marker files of benchmarks/graph_cut/segmentation_regions.cpp:53-80):
+   graph lines "n;<id>;<count>;<sum>" (regions, ids 1..n) and "e;<a>;<b>;<count>;<sum>" (adjacent
+   regions and their boundary statistic), marker lines "f <id>" (object) and "b <id>" (background);
+   '#' lines and empty lines are skipped, and a later edge record of the same (a, b) replaces the
+   earlier one. As computeCudaGraphCut builds its GraphData (benchmarks/graph_cut/graph_cut.cpp:
+   100-154), region id i is vertex i - 1, every edge has weight round(sum * weight_scale) in both
+   directions, background markers get marker_capacity on the source link and object markers on the
+   sink link; the object segmentation is then cgp_fill_segments (the vertices that still reach t). */
+int cgp_region_graph_info(const char *graph_path, int32_t *vertex_count, int32_t *edge_count);
+/* Fills edges and weights (edge_count entries) and the t-links (vertex_count entries) of a graph
+   file and a marker file (markers_path may be NULL: no markers), ready for cgp_set_vertex_count,
+   cgp_set_nweights(edges, weights, weights) and cgp_set_tweights. Host buffers. */
+int cgp_region_graph_read(const char *graph_path, const char *markers_path, double weight_scale,
+                          int32_t marker_capacity, cgp_edge *edges, int32_t *weights, int32_t *cap_source,
+                          int32_t *cap_sink);
+
 /* ---- on-device image -> graph front end -----------------------------------------------------
    Capacities computed on the device from a uint8 volume, so that they never exist on the host:
    the counterpart of the caller-side construction in benchmarks/graph_cut/graph_cut.cpp:32-98

@@ -107,3 +107,29 @@ def test_product_does_not_use_the_oracle():
                 if f.endswith((".py", ".hip", ".h", ".hpp", ".cpp", "Makefile")) or f == "Makefile":
                     text = open(os.path.join(dirpath, f), errors="ignore").read()
                     assert "liboracle" not in text and "import oracle" not in text and "orc_" not in text, f
+
+
+def test_region_graph_reader(tmp_path):
+    """cgp_region_graph_read gives the edges, weights and t-links an independent Python reading of
+    the same files gives (ids shifted to 0, round(sum * scale) both ways, background markers on the
+    source link, a repeated edge record replacing the earlier one); malformed files are refused."""
+    import numpy as np
+
+    from tests.region_util import parse_region_graph, write_random_region_graph
+    g, mk = tmp_path / "g.txt", tmp_path / "m.txt"
+    write_random_region_graph(g, mk, 200, seed=1)
+    n, edges, w, cs, ct = cugip_amd.read_region_graph(str(g), str(mk), 1000.0, 10000)
+    rn, ea, eb, rw, rcs, rct = parse_region_graph(g, mk, 1000.0, 10000)
+    assert n == rn
+    got = sorted(zip(edges[:, 0].tolist(), edges[:, 1].tolist(), w.tolist()))
+    ref = sorted(zip(ea.tolist(), eb.tolist(), rw.tolist()))
+    assert got == ref
+    assert np.array_equal(cs, rcs) and np.array_equal(ct, rct)
+    bad = tmp_path / "bad.txt"
+    bad.write_text("n;1;3;2.0\nn;3;3;2.0\ne;1;3;1;0.5\n")  # region 2 has no node record
+    with pytest.raises(cugip_amd.CgpError) as e:
+        cugip_amd.read_region_graph(str(bad))
+    assert e.value.code == cugip_amd.CGP_ERR_INVALID
+    bad.write_text("n;1;3;2.0\nn;2;3;2.0\ne;1;2;1;abc\n")
+    with pytest.raises(cugip_amd.CgpError):
+        cugip_amd.read_region_graph(str(bad))

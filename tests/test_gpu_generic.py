@@ -158,3 +158,60 @@ def test_trace_hooks():
     assert kinds[0] == cugip_amd.TRACE_STARTED and kinds[-1] == cugip_amd.TRACE_FINISHED
     assert kinds.count(cugip_amd.TRACE_BEGIN_ITERATION) == kinds.count(cugip_amd.TRACE_AFTER_RELABEL)
     assert events[-1][3] == want and events[-1][2] == 1
+
+
+@pytest.mark.parametrize("n,seed", [(300, 2), (5000, 3)])
+def test_region_graph_file(tmp_path, n, seed):
+    """A region adjacency graph file and marker file in the reference's format, read by
+    cgp_region_graph_read and solved by the generic GPU path: flow, S_min and the object mask
+    (fill_segments, the regions that still reach t) equal Dinic's on the independently parsed
+    graph."""
+    from tests.region_util import parse_region_graph, write_random_region_graph
+    say(f"region graph file n={n} seed {seed}")
+    g, mk = tmp_path / "g.txt", tmp_path / "m.txt"
+    write_random_region_graph(g, mk, n, seed)
+    nv, edges, w, cs, ct = cugip_amd.read_region_graph(str(g), str(mk), 1000.0, 10000)
+    with cugip_amd.Graph() as gr:
+        gr.set_vertex_count(nv)
+        gr.set_nweights(edges, w, w)
+        gr.set_tweights(cs, ct)
+        flow = gr.max_flow()
+        smin, tmin = gr.source_set(), gr.fill_segments(1, 0)
+    rn, ea, eb, rw, rcs, rct = parse_region_graph(g, mk, 1000.0, 10000)
+    ref_flow, ref_smin, ref_tmin = oracle.dinic(rn, ea, eb, rw, rw, rcs, rct)
+    assert flow == ref_flow
+    assert np.array_equal(smin, ref_smin) and np.array_equal(tmin, ref_tmin)
+    assert 0 < tmin.sum() < nv
+
+
+def test_large_lattice_graph_multi_workgroup_levels():
+    """A 64^3-vertex lattice graph with one random extra edge per vertex (parallel edges and both
+    orientations among them, asymmetric capacities): BFS and push levels far above one workgroup's
+    size, so the multi-workgroup level kernels and their wave-aggregated queue appends and slotted
+    flow sums run; flow, S_min and T_min equal Boykov-Kolmogorov's."""
+    rng = np.random.default_rng(11)
+    k = 64
+    n = k ** 3
+    z, y, x = (c.reshape(-1) for c in np.indices((k, k, k)))
+    v = np.arange(n)
+    ea, eb = [], []
+    for d, ok in ((1, x + 1 < k), (k, y + 1 < k), (k * k, z + 1 < k)):
+        ea.append(v[ok])
+        eb.append(v[ok] + d)
+    off = rng.integers(-2, 3, (n, 3))
+    u = (np.clip(z + off[:, 2], 0, k - 1) * k + np.clip(y + off[:, 1], 0, k - 1)) * k + np.clip(x + off[:, 0], 0, k - 1)
+    ea.append(u[u != v])  # (reversed orientation on purpose)
+    eb.append(v[u != v])
+    ea, eb = np.concatenate(ea), np.concatenate(eb)
+    fwd = rng.integers(1, 100, ea.size).astype(np.int32)
+    bwd = rng.integers(1, 100, ea.size).astype(np.int32)
+    cs = (rng.integers(0, 400, n) * (rng.random(n) < 0.1)).astype(np.int32)
+    ct = (rng.integers(0, 400, n) * (rng.random(n) < 0.1)).astype(np.int32)
+    swap = ea > eb
+    g = (n, np.where(swap, eb, ea).astype(np.int32), np.where(swap, ea, eb).astype(np.int32),
+         np.where(swap, bwd, fwd).astype(np.int32), np.where(swap, fwd, bwd).astype(np.int32), cs, ct)
+    say(f"generic lattice graph n {n} m {ea.size}")
+    flow, smin, tmin = solve(g)
+    ref_flow, ref_smin, ref_tmin, _ = oracle.bk(*g)
+    assert flow == ref_flow
+    assert np.array_equal(smin, ref_smin) and np.array_equal(tmin, ref_tmin)
