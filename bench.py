@@ -48,20 +48,25 @@ def cpu_model():
 def cpu_baseline(nx, ny, nz, conn, cs, ct, cn, runs):
     """Boykov-Kolmogorov (the reference's CPU comparator role) on the same instance, one core."""
     import oracle
+    log(f"cpu baseline: edge list of the {nx}x{ny}x{nz} {conn}-connected grid")
     ea, eb, fwd, bwd = oracle.grid_edges(nx, ny, nz, conn, cn)
+    log(f"cpu baseline: {len(ea)} edges; Boykov-Kolmogorov runs")
     n = nx * ny * nz
     secs = []
     flow = None
-    for i in range(runs + 1):
+    # runs == 1: a single timed run without warm-up (for instances whose one run takes minutes)
+    warm = 1 if runs > 1 else 0
+    for i in range(runs + warm):
         f, _, _, t = oracle.bk(n, ea, eb, fwd, bwd, cs.reshape(-1), ct.reshape(-1))
         flow = f
-        if i > 0:  # first run is the warm-up
+        if i >= warm:
             secs.append(t)
         log(f"cpu baseline run {i}: BK maxflow {t:.3f} s")
     med = statistics.median(secs)
+    how = f"median of {runs} runs after 1 warm-up" if warm else "one run"
     return dict(value=round(n / med / 1e6, 3), unit="Mvoxels/s", cores=1, kind="port",
                 sample=f"full {nx}x{ny}x{nz} {conn}-connected seed-0 instance; Boykov-Kolmogorov maxflow() only "
-                       f"(graph construction excluded), median of {runs} runs after 1 warm-up",
+                       f"(graph construction excluded), {how}",
                 seconds=round(med, 4), flow=flow, host_cpu=cpu_model(), host_nproc=os.cpu_count())
 
 
@@ -368,7 +373,9 @@ def main():
                     help="grid dimensions (default: --size in every axis)")
     ap.add_argument("--volume", action="store_true",
                     help="capacities built on the device from a generated uint8 volume (the image front end), "
-                         "as for the 512x512x384 26-connected configuration; no CPU baseline or batch")
+                         "as for the 512x512x384 26-connected configuration; no batch legs (the CPU baseline "
+                         "computes the same capacities on the host; at 512x512x384 26-connected it needs about "
+                         "90 GB of host memory: use --cpu-runs 1)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--conn", type=int, default=6)
     ap.add_argument("--engine", choices=["auto", "tiles", "graph"], default="auto",
@@ -516,7 +523,10 @@ def main():
         log(f"generic graph: {generic}")
 
     cpu = None
-    if not args.no_cpu and not args.volume:
+    if not args.no_cpu:
+        if args.volume:
+            # the CPU solver needs the capacities on the host: the same tables, computed here
+            cs, ct, cn = cugip_amd.caps_from_volume(vol, args.conn)
         cpu = cpu_baseline(nx, ny, nz, args.conn, cs, ct, cn, args.cpu_runs)
         if cpu["flow"] != flow:
             log(f"CPU baseline flow {cpu['flow']} differs from GPU flow {flow}")

@@ -154,6 +154,8 @@ def grid_edges(nx, ny, nz, conn, cap_n, cap_nb=None):
                              _p32(ea), _p32(eb), _p32(fwd), _p32(bwd))
     if m < 0:
         raise ValueError("grid_edges: bad connectivity")
+    if m == cap:
+        return ea, eb, fwd, bwd
     return ea[:m].copy(), eb[:m].copy(), fwd[:m].copy(), bwd[:m].copy()
 
 

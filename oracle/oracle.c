@@ -942,18 +942,22 @@ int64_t orc_grid_edges(int32_t nx, int32_t ny, int32_t nz, int32_t conn, const i
 		return -1;
 	}
 	const int64_t vox = (int64_t)nx * ny * nz;
-	int64_t m = 0;
+	int32_t off[13][3];
 	for (int32_t j = 0; j < conn / 2; ++j) {
-		int32_t dx, dy, dz;
-		half_offset(conn, j, &dx, &dy, &dz);
-		for (int32_t z = 0; z < nz; ++z) {
-			for (int32_t y = 0; y < ny; ++y) {
-				for (int32_t x = 0; x < nx; ++x) {
-					const int32_t ux = x + dx, uy = y + dy, uz = z + dz;
+		half_offset(conn, j, &off[j][0], &off[j][1], &off[j][2]);
+	}
+	/* voxel-major (the half neighbourhood of each voxel in turn): the arcs of one vertex are then
+	   written close together when a network is built from the list */
+	int64_t m = 0;
+	for (int32_t z = 0; z < nz; ++z) {
+		for (int32_t y = 0; y < ny; ++y) {
+			for (int32_t x = 0; x < nx; ++x) {
+				const int64_t v = ((int64_t)z * ny + y) * nx + x;
+				for (int32_t j = 0; j < conn / 2; ++j) {
+					const int32_t ux = x + off[j][0], uy = y + off[j][1], uz = z + off[j][2];
 					if (ux < 0 || ux >= nx || uy < 0 || uy >= ny || uz < 0 || uz >= nz) {
 						continue;
 					}
-					const int64_t v = ((int64_t)z * ny + y) * nx + x;
 					const int64_t u = ((int64_t)uz * ny + uy) * nx + ux;
 					const int32_t f = cap_n[j * vox + v];
 					const int32_t b = cap_nb ? cap_nb[j * vox + v] : f;
