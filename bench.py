@@ -149,7 +149,7 @@ def cpu_batch_baseline(grids, threads):
                 seconds=round(longest, 3), wall_s=round(wall, 3), flows=flows)
 
 
-def batch_workload(cugip_amd, count, seed, repeats=3, cpu=True):
+def batch_workload(cugip_amd, count, seed, repeats=3, cpu=True, plain=True):
     """The batch workload: `count` grids with edge lengths log-uniform in [32, 128] per axis and grid,
     solved as one batch (device-driven engine, capacities resident in HBM) against the plain
     single-grid solver run on each grid in turn; every grid's flow and S_min compared bit for bit.
@@ -171,7 +171,7 @@ def batch_workload(cugip_amd, count, seed, repeats=3, cpu=True):
     plain_s = 0.0
     same = True
     with cugip_amd.GridGraph() as g:
-        for i, (cs, ct, cn) in enumerate(grids):
+        for i, (cs, ct, cn) in enumerate(grids if plain else []):
             g.set_grid(cs, ct, cn)
             g.max_flow()
             tg = []
@@ -184,9 +184,11 @@ def batch_workload(cugip_amd, count, seed, repeats=3, cpu=True):
     out = dict(workload=f"{count} grids, edge lengths log-uniform in [32, 128] per axis (seed {seed}), "
                         f"6-connected, {vox} voxels",
                batched_ms=round(batch_s * 1e3, 3), batched_mvox_s=round(vox / batch_s / 1e6, 1),
-               plain_ms=round(plain_s * 1e3, 3), plain_mvox_s=round(vox / plain_s / 1e6, 1),
-               speedup=round(plain_s / batch_s, 3), bit_exact=bool(same), replays=stats["graph_replays"],
-               rounds=stats["rounds"], tile=[stats["tile_x"], stats["tile_y"], stats["tile_z"]])
+               replays=stats["graph_replays"], rounds=stats["rounds"],
+               tile=[stats["tile_x"], stats["tile_y"], stats["tile_z"]])
+    if plain:
+        out.update(plain_ms=round(plain_s * 1e3, 3), plain_mvox_s=round(vox / plain_s / 1e6, 1),
+                   speedup=round(plain_s / batch_s, 3), bit_exact=bool(same))
     if cpu:
         c = cpu_batch_baseline(grids, host_cores())
         c["flows_match_gpu"] = c.pop("flows") == flows
@@ -388,6 +390,8 @@ def main():
                     help="vertices of the generic-graph (CSR) leg, about 4 edges each (0: skip)")
     ap.add_argument("--batch-workload", type=int, default=512,
                     help="grids of the batch workload leg, edge lengths log-uniform in [32, 128] (0: skip)")
+    ap.add_argument("--batch-plain", type=int, default=1,
+                    help="0: skip the batch leg's one-by-one plain solves (each grid captures its own round graph)")
     ap.add_argument("--calibration-copies", action="store_true",
                     help="start with device-to-device copies of known size (for rocprofv3 --pmc)")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
@@ -430,6 +434,7 @@ def main():
 
     engine = {"auto": cugip_amd.ENGINE_AUTO, "tiles": cugip_amd.ENGINE_TILES,
               "graph": cugip_amd.ENGINE_GRAPH}[args.engine]
+    log("instance ready; capacities to the device")
     g = cugip_amd.GridGraph()
     t0 = time.perf_counter()
     if args.volume:
@@ -438,6 +443,7 @@ def main():
         g.set_grid(cs, ct, cn, conn=args.conn)
     h2d_ms = (time.perf_counter() - t0) * 1e3
     g.set_policy(engine=engine)
+    log(f"grid set ({h2d_ms:.1f} ms); warm-up solves")
 
     for i in range(args.warmup):
         g.max_flow()
@@ -513,7 +519,8 @@ def main():
     batch512 = None
     if args.batch_workload > 0 and not args.volume and args.conn == 6:
         log(f"batch workload: {args.batch_workload} grids")
-        batch512 = batch_workload(cugip_amd, args.batch_workload, args.seed, cpu=not args.no_cpu)
+        batch512 = batch_workload(cugip_amd, args.batch_workload, args.seed, cpu=not args.no_cpu,
+                                  plain=args.batch_plain != 0)
         log(f"batch workload: {batch512}")
 
     generic = None

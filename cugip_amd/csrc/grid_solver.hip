@@ -1040,6 +1040,7 @@ int64_t GridSolver::solve_graph(int phases, int max_iters, int max_rounds, cgp_s
 	CGP_LAUNCH_CHECK();
 	unsigned long long *pinned = reinterpret_cast<unsigned long long *>(mHostScratch.get());
 	const SolveControl *host_ctl = reinterpret_cast<const SolveControl *>(pinned);
+	int round_seen = -1, replays_in_round = 0;
 	for (int replay = 0;; ++replay) {
 		CGP_HIP(hipGraphLaunch(mGraph.exec, mStream));
 		CGP_HIP(hipMemcpyAsync(pinned, mCtl.get(), (kCtlTotals + 3 + n) * sizeof(unsigned long long),
@@ -1067,9 +1068,17 @@ int64_t GridSolver::solve_graph(int phases, int max_iters, int max_rounds, cgp_s
 			mClosedByCheck = mode == kModeDoneClosed;
 			break;
 		}
-		if (replay >= kMaxReplays) {
-			fail(CGP_ERR_NOT_CONVERGED, "device-driven solve still in stage " + std::to_string(mode) + " after " +
-			                                std::to_string(replay + 1) + " replays");
+		// a global relabel or a closure takes a bounded number of colour-sweep passes; a round that
+		// needs far more replays than any sweep can take has stopped making progress
+		if (host_ctl->rounds != round_seen) {
+			round_seen = host_ctl->rounds;
+			replays_in_round = 0;
+		}
+		if (++replays_in_round > kMaxReplaysPerRound) {
+			fail(CGP_ERR_NOT_CONVERGED, "device-driven solve made no progress: stage " + std::to_string(mode) +
+			                                ", round " + std::to_string(host_ctl->rounds) + ", " +
+			                                std::to_string(host_ctl->dirty) + " dirty tiles after " +
+			                                std::to_string(replays_in_round) + " replays of the round");
 		}
 	}
 	mWideResiduals = (host_ctl->status & 4) != 0;

@@ -146,7 +146,7 @@ private:
 		void reset();
 	};
 	static constexpr int kGraphSweepCycles = 3;  // colour-sweep cycles per stage and replay
-	static constexpr int kMaxReplays = 1 << 20;
+	static constexpr int kMaxReplaysPerRound = 1024;
 	static constexpr size_t kCtlTotals = sizeof(SolveControl) / sizeof(unsigned long long);
 	void capture_round_graph(int phases, int max_iters, bool wide);
 	int64_t solve_graph(int phases, int max_iters, int max_rounds, cgp_stats &stats);
