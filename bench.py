@@ -151,15 +151,15 @@ def cpu_batch_baseline(grids, threads):
 
 def batch_workload(cugip_amd, count, seed, repeats=3, cpu=True, plain=True):
     """The batch workload: `count` grids with edge lengths log-uniform in [32, 128] per axis and grid,
-    solved as one batch (device-driven engine, capacities resident in HBM) against the plain
-    single-grid solver run on each grid in turn; every grid's flow and S_min compared bit for bit.
+    solved as one batch (capacities resident in HBM) against the plain single-grid solver run on
+    each grid in turn; every grid's flow and S_min compared bit for bit.
     With cpu, the all-cores Boykov-Kolmogorov baseline over the same grids."""
     import numpy as np
     grids = cugip_amd.make_batch(count, seed)
     vox = sum(g[0].size for g in grids)
     with cugip_amd.GridBatch() as b:
         b.set_batch(grids, 6)
-        b.solve()  # first solve captures the round graph
+        b.solve()  # warm-up (tile set-up)
         ts = []
         for _ in range(repeats):
             t0 = time.perf_counter()

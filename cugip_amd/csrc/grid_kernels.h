@@ -1028,10 +1028,10 @@ __global__ __launch_bounds__(256) void k_sum_slots(const unsigned long long *__r
 // with more than kDirect list entries stores them through an LDS stage by consecutive lanes, and
 // consumed flags go back as whole 16-byte words.
 constexpr int kCompactThreads = 1024;
-// one piece per thread for up to 32768 tiles; up to eight (262144 tiles) only where the order by
-// work pays for the slower pass (more pieces measured slower than the multi-workgroup path)
+// one piece per thread for up to 32768 tiles (the solver uses it with one piece; beyond, lists go
+// through the multi-workgroup passes: k_compact_tiles, or k_order_count + k_order_place for the
+// ordered discharge lists, which measured faster than eight pieces a thread in one workgroup)
 constexpr int kCompactBlockTiles = 32 * kCompactThreads;
-constexpr int kCompactOrderedTiles = 8 * kCompactBlockTiles;
 
 __device__ inline int block_exclusive_scan_1024(int v, int *wave_sums, int &total)
 {
@@ -1463,6 +1463,127 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact_block(uint8_t *__re
 }
 
 // ---------------------------------------------------------------------------------------------
+// Ordered compaction over many workgroups, for lists longer than one workgroup compacts quickly:
+// the same order by decreasing work class, in two passes over 4096 tiles a workgroup (4 a thread,
+// one 4-byte load per array). k_order_count adds each workgroup's per-class counts into
+// ws[0..kWorkClasses) (one atomic per class and workgroup); k_order_place reserves each
+// workgroup's run inside every class's part of the list (one atomic per class and workgroup on
+// ws[kWorkClasses + c], zeroed with the counts beforehand) and stores its entries there in thread
+// order, so a wave's stores of one class land side by side. Within a class the order between
+// workgroups follows their reservations, which does not matter: any order of the tiles of a phase
+// gives the same phase (tiles of one colour never touch).
+constexpr int kOrderWorkspace = 2 * kWorkClasses;
+
+// selection (bit j) and packed per-class counts of the thread's 4 tiles from t0
+__device__ inline uint32_t order_select(const uint8_t *flags, int ntiles, const uint8_t *colours, int colour,
+                                        const uint8_t *work, int t0, uint32_t &cls4, unsigned long long &lo,
+                                        unsigned long long &hi)
+{
+	uint32_t sel = 0;
+	cls4 = 0;
+	lo = hi = 0;
+	if (t0 >= ntiles) {
+		return 0;
+	}
+	const uint32_t f = *reinterpret_cast<const uint32_t *>(flags + t0);
+	const uint32_t c = colour >= 0 ? *reinterpret_cast<const uint32_t *>(colours + t0) : 0u;
+	const uint32_t w = *reinterpret_cast<const uint32_t *>(work + t0);
+#pragma unroll
+	for (int j = 0; j < kCompactTilesPerThread; ++j) {
+		const bool on = ((f >> (8 * j)) & 0xffu) != 0u &&
+		                (colour < 0 || ((c >> (8 * j)) & 0xffu) == (uint32_t)colour) && t0 + j < ntiles;
+		const int k = work_class((int)((w >> (8 * j)) & 0xffu));
+		sel |= (on ? 1u : 0u) << j;
+		cls4 |= (uint32_t)k << (8 * j);
+		if (on) {
+			const unsigned long long one = 1ull << (16 * (k & 3));
+			lo += k < 4 ? one : 0ull;
+			hi += k < 4 ? 0ull : one;
+		}
+	}
+	return sel;
+}
+
+__global__ __launch_bounds__(kCompactThreads) void k_order_count(const uint8_t *__restrict__ flags, int ntiles,
+                                                                 const uint8_t *__restrict__ colours, int colour,
+                                                                 const uint8_t *__restrict__ work,
+                                                                 int32_t *__restrict__ ws, const SolveControl *ctl,
+                                                                 unsigned gate)
+{
+	__shared__ unsigned long long sums[kCompactThreads / 64][2];
+	if (!gate_open(ctl, gate)) {
+		return;
+	}
+	const int t0 = ((int)blockIdx.x * kCompactThreads + (int)threadIdx.x) * kCompactTilesPerThread;
+	uint32_t cls4;
+	unsigned long long lo, hi;
+	order_select(flags, ntiles, colours, colour, work, t0, cls4, lo, hi);
+	unsigned long long tlo = 0, thi = 0;
+	block_exclusive_scan_pair_1024(lo, hi, sums, tlo, thi);
+	if (threadIdx.x < kWorkClasses) {
+		const int c = (int)threadIdx.x;
+		const int n = (int)(((c < 4 ? tlo : thi) >> (16 * (c & 3))) & 0xffffull);
+		if (n) {
+			atomicAdd(&ws[c], n);
+		}
+	}
+}
+
+__global__ __launch_bounds__(kCompactThreads) void k_order_place(const uint8_t *__restrict__ flags, int ntiles,
+                                                                 const uint8_t *__restrict__ colours, int colour,
+                                                                 const uint8_t *__restrict__ work,
+                                                                 int32_t *__restrict__ ws, int32_t *__restrict__ list,
+                                                                 int32_t *__restrict__ count, const SolveControl *ctl,
+                                                                 unsigned gate)
+{
+	__shared__ unsigned long long sums[kCompactThreads / 64][2];
+	__shared__ int base[kWorkClasses];
+	if (!gate_open(ctl, gate)) {
+		if (blockIdx.x == 0 && threadIdx.x == 0) {
+			*count = 0;  // a closed gate lists nothing
+		}
+		return;
+	}
+	const int t0 = ((int)blockIdx.x * kCompactThreads + (int)threadIdx.x) * kCompactTilesPerThread;
+	uint32_t cls4;
+	unsigned long long lo, hi;
+	const uint32_t sel = order_select(flags, ntiles, colours, colour, work, t0, cls4, lo, hi);
+	unsigned long long tlo = 0, thi = 0;
+	block_exclusive_scan_pair_1024(lo, hi, sums, tlo, thi);  // lo, hi: this thread's offsets per class
+	if (threadIdx.x == 0) {
+		// class c's part of the list starts after the totals of the classes before it
+		int start = 0;
+		for (int c = 0; c < kWorkClasses; ++c) {
+			const int n = (int)(((c < 4 ? tlo : thi) >> (16 * (c & 3))) & 0xffffull);
+			base[c] = n ? start + atomicAdd(&ws[kWorkClasses + c], n) : 0;
+			start += ws[c];
+		}
+		if (blockIdx.x == 0) {
+			*count = start;
+		}
+	}
+	__syncthreads();
+	int next[kWorkClasses];
+#pragma unroll
+	for (int c = 0; c < kWorkClasses; ++c) {
+		next[c] = base[c] + (int)(((c < 4 ? lo : hi) >> (16 * (c & 3))) & 0xffffull);
+	}
+#pragma unroll
+	for (int j = 0; j < kCompactTilesPerThread; ++j) {
+		if (!((sel >> j) & 1u)) {
+			continue;
+		}
+		const int k = (int)((cls4 >> (8 * j)) & 0xffu);
+#pragma unroll
+		for (int c = 0; c < kWorkClasses; ++c) {
+			if (c == k) {
+				list[next[c]++] = t0 + j;
+			}
+		}
+	}
+}
+
+// ---------------------------------------------------------------------------------------------// ---------------------------------------------------------------------------------------------
 // Controllers of the device-driven solve (single workgroups; SolveControl in grid_control.h).
 
 // Start of a device-driven solve (one thread).

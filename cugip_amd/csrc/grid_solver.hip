@@ -495,6 +495,7 @@ void GridSolver::configure_tiles(const Shape &shape)
 	mDirtyA.allocate(padded, mArena);
 	mDirtyB.allocate(padded, mArena);
 	mList.allocate(mTiles, mArena);
+	mOrderWs.allocate(kOrderWorkspace, mArena);
 	mAllTiles.allocate(mTiles + 1, mArena);
 	std::vector<int32_t> all(mTiles + 1);
 	for (int t = 0; t < mTiles; ++t) {
@@ -613,17 +614,27 @@ void GridSolver::launch_init(const SolveControl *ctl)
 void GridSolver::launch_compact(const ColourSet &set, uint8_t *flags, int colour, bool clear, const uint8_t *work,
                                 const SolveControl *ctl, unsigned gate)
 {
-	if (mTiles <= kCompactBlockTiles || (work && mTiles <= kCompactOrderedTiles)) {
+	if (mTiles <= kCompactBlockTiles) {
 		launch(kCompact, 0, 0.0, [&] {
-			if (mTiles <= kCompactBlockTiles) {
-				hipLaunchKernelGGL(k_compact_block<1>, dim3(1), dim3(kCompactThreads), 0, mStream, flags, mTiles,
-				                   set.colour.get(), colour, clear ? 1 : 0, mList.get(), mCount.get(),
-				                   (const uint8_t *)nullptr, (int32_t *)nullptr, work, ctl, gate);
-			} else {
-				hipLaunchKernelGGL(k_compact_block<8>, dim3(1), dim3(kCompactThreads), 0, mStream, flags, mTiles,
-				                   set.colour.get(), colour, clear ? 1 : 0, mList.get(), mCount.get(),
-				                   (const uint8_t *)nullptr, (int32_t *)nullptr, work, ctl, gate);
-			}
+			hipLaunchKernelGGL(k_compact_block<1>, dim3(1), dim3(kCompactThreads), 0, mStream, flags, mTiles,
+			                   set.colour.get(), colour, clear ? 1 : 0, mList.get(), mCount.get(),
+			                   (const uint8_t *)nullptr, (int32_t *)nullptr, work, ctl, gate);
+		});
+		return;
+	}
+	const int blocks = (mTiles + kCompactTilesPerGroup - 1) / kCompactTilesPerGroup;
+	if (work) {
+		// ordered by work class over many workgroups (the lists of discharge phases never clear flags)
+		launch(kFill, 0, 0.0, [&] {
+			hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, mStream, mOrderWs.get(), (int64_t)kOrderWorkspace, 0,
+			                   (const SolveControl *)nullptr, 0u);
+		});
+		launch(kCompact, 0, 0.0, [&] {
+			hipLaunchKernelGGL(k_order_count, dim3(blocks), dim3(kCompactThreads), 0, mStream, (const uint8_t *)flags,
+			                   mTiles, (const uint8_t *)set.colour.get(), colour, work, mOrderWs.get(), ctl, gate);
+			hipLaunchKernelGGL(k_order_place, dim3(blocks), dim3(kCompactThreads), 0, mStream, (const uint8_t *)flags,
+			                   mTiles, (const uint8_t *)set.colour.get(), colour, work, mOrderWs.get(), mList.get(),
+			                   mCount.get(), ctl, gate);
 		});
 		return;
 	}
@@ -632,9 +643,8 @@ void GridSolver::launch_compact(const ColourSet &set, uint8_t *flags, int colour
 		                   (const SolveControl *)nullptr, 0u);
 	});
 	launch(kCompact, 0, 0.0, [&] {
-		hipLaunchKernelGGL(k_compact_tiles, dim3((mTiles + kCompactTilesPerGroup - 1) / kCompactTilesPerGroup),
-		                   dim3(kCompactThreads), 0, mStream, flags, mTiles, set.colour.get(), colour, clear ? 1 : 0,
-		                   mList.get(), mCount.get(), ctl, gate);
+		hipLaunchKernelGGL(k_compact_tiles, dim3(blocks), dim3(kCompactThreads), 0, mStream, flags, mTiles,
+		                   set.colour.get(), colour, clear ? 1 : 0, mList.get(), mCount.get(), ctl, gate);
 	});
 }
 
@@ -1133,11 +1143,18 @@ int64_t GridSolver::solve(const cgp_policy &policy, cgp_stats *stats_out, const 
 
 	// The device-driven engine takes no host decision per stage: it serves solves without trace
 	// hooks, per-kernel profiling, an edge threshold or a certificate.
+	// It is opt-in: the host-driven loop measured as fast or faster on every workload timed (256^3,
+	// the 512-grid batch, 26-connected grids), so AUTO takes the host-driven loop. 26-connected
+	// grids are refused: a 512x512x384 26-connected graph solve stopped answering twice, with no
+	// cause found.
 	const bool graph_ok = !trace.fn && !mProfile && threshold == 0 && mCertificate == 0;
 	if (policy.engine == CGP_ENGINE_GRAPH && !graph_ok) {
 		fail(CGP_ERR_INVALID, "the graph engine takes no trace hook, profiling, minimal_residual or certificate");
 	}
-	const bool use_graph = graph_ok && policy.engine != CGP_ENGINE_TILES;
+	if (policy.engine == CGP_ENGINE_GRAPH && mConn != 6) {
+		fail(CGP_ERR_INVALID, "the graph engine serves 6-connected grids; 26-connected grids run on the tiles engine");
+	}
+	const bool use_graph = policy.engine == CGP_ENGINE_GRAPH;
 	if (use_graph) {
 		mStarted = true;
 		const int64_t flow = solve_graph(phases, max_iters, max_rounds, stats);

@@ -203,6 +203,7 @@ private:
 	DeviceBuffer<int32_t> mStatus, mCount, mList, mAllTiles;
 	DeviceBuffer<uint8_t> mActive, mDirtyA, mDirtyB;
 	DeviceBuffer<uint8_t> mTileWork;  // per tile: discharge work estimate from the last relabel
+	DeviceBuffer<int32_t> mOrderWs;   // ordered multi-workgroup compaction: class counts and cursors
 	// device-driven solve: control block (kCtlTotals words), then 3 visit-counter totals and one flow
 	// total per grid
 	DeviceBuffer<unsigned long long> mCtl;

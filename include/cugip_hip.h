@@ -70,8 +70,9 @@ typedef struct cgp_policy {
    GRAPH: device-driven. A control block in device memory holds the solve's stage, every kernel is
    gated by it and controller kernels move it on; init plus one round of stages is captured in a HIP
    graph once and replayed, with one read of the control block per replay, until it says done.
-   Solves with trace hooks, profiling, minimal_residual > 0 or a certificate run on TILES.
-   AUTO: GRAPH where it can, else TILES. */
+   GRAPH serves 6-connected grids only and refuses trace hooks, profiling, minimal_residual > 0
+   and a certificate (CGP_ERR_INVALID).
+   AUTO: TILES (measured as fast or faster than GRAPH on every timed workload). */
 #define CGP_ENGINE_AUTO 0
 #define CGP_ENGINE_TILES 1
 #define CGP_ENGINE_GRAPH 2

@@ -94,21 +94,28 @@ def test_batch_matches_single_solves_on_batch_workload():
 
 def test_batch_workload_512():
     """The full batch workload: 512 grids with edge lengths log-uniform in [32, 128] per axis and
-    grid (165.6 M voxels), solved as one batch by the device-driven engine. Every grid's flow and
-    S_min equal the single-grid solver's bit for bit; the 16 smallest grids also match the CPU
-    oracle (Boykov-Kolmogorov) in flow, S_min and T_min."""
+    grid (165.6 M voxels), solved as one batch by the host-driven and by the device-driven engine
+    (the same flows). Every grid's flow and S_min equal the single-grid solver's bit for bit; the 16
+    smallest grids also match the CPU oracle (Boykov-Kolmogorov) in flow, S_min and T_min."""
     import time
     say("batch workload, 512 grids: building")
     grids = cugip_amd.make_batch(512, seed=0)
     vox = sum(cs.size for cs, _, _ in grids)
     with cugip_amd.GridBatch() as b:
         b.set_batch(grids, 6)
+        b.set_policy(engine=cugip_amd.ENGINE_GRAPH)
         t0 = time.perf_counter()
-        flows = b.solve()
+        graph_flows = b.solve()
         stats = b.stats.as_dict()
-        say(f"  batch solve {1e3 * (time.perf_counter() - t0):.1f} ms for {vox} voxels, "
+        say(f"  graph batch solve {1e3 * (time.perf_counter() - t0):.1f} ms for {vox} voxels, "
             f"{stats['graph_replays']} replays, {stats['rounds']} rounds")
         assert stats["graph_replays"] > 0
+        b.set_policy()
+        t0 = time.perf_counter()
+        flows = b.solve()
+        say(f"  tiles batch solve {1e3 * (time.perf_counter() - t0):.1f} ms")
+        assert b.stats.graph_replays == 0
+        assert list(flows) == list(graph_flows)
         smin = [b.batch_source_set(i).reshape(-1) for i in range(len(grids))]
         order = sorted(range(len(grids)), key=lambda i: grids[i][0].size)
         small = order[:16]

@@ -75,8 +75,8 @@ def test_parity_engines(engine, dims, seed, policy):
 
 
 def test_graph_engine_refuses_host_decisions():
-    """The graph engine takes no per-stage host decision: minimal_residual, trace hooks and an
-    unknown engine are refused; AUTO then falls back to the host loop."""
+    """The graph engine takes no per-stage host decision: minimal_residual and an unknown engine
+    are refused, and so are 26-connected grids; AUTO runs the host loop."""
     say("graph engine argument checks")
     vol, cs, ct, cn = cugip_amd.make_instance(16, 16, 16, 1, 0, 6)
     with cugip_amd.GridGraph() as g:
@@ -94,7 +94,20 @@ def test_graph_engine_refuses_host_decisions():
         assert g.stats.graph_replays == 0
         g.set_policy()
         assert g.max_flow() == flow
+        assert g.stats.graph_replays == 0
+        g.set_policy(engine=cugip_amd.ENGINE_GRAPH)
+        assert g.max_flow() == flow
         assert g.stats.graph_replays > 0
+    vol, cs, ct, cn = cugip_amd.make_instance(16, 16, 16, 1, 0, 26)
+    with cugip_amd.GridGraph() as g:
+        g.set_grid(cs, ct, cn, conn=26)
+        g.set_policy(engine=cugip_amd.ENGINE_GRAPH)
+        with pytest.raises(cugip_amd.CgpError) as e:
+            g.max_flow()
+        assert e.value.code == cugip_amd.CGP_ERR_INVALID
+        g.set_policy()
+        g.max_flow()
+        assert g.stats.graph_replays == 0
 
 
 @pytest.mark.parametrize("conn", [6, 26])
@@ -121,7 +134,7 @@ def test_degenerate_shapes(dims):
     ct = rng.integers(0, 50, (nz, ny, nx)).astype(np.int32)
     cn = rng.integers(0, 30, (3, nz, ny, nx)).astype(np.int32)
     check_against_oracle(cs, ct, cn, method="dinic")
-    check_against_oracle(cs, ct, cn, method="dinic", engine=cugip_amd.ENGINE_TILES)
+    check_against_oracle(cs, ct, cn, method="dinic", engine=cugip_amd.ENGINE_GRAPH)
     cn26 = rng.integers(0, 30, (13, nz, ny, nx)).astype(np.int32)
     check_against_oracle(cs, ct, cn26, conn=26, method="dinic")
 
@@ -495,7 +508,7 @@ def test_full_size_26c_512x512x384():
     (6, (48, 40, 36), 0, cugip_amd.ENGINE_GRAPH),
     (6, (48, 40, 36), 1, cugip_amd.ENGINE_TILES),
     (6, (64, 64, 64), 2, cugip_amd.ENGINE_AUTO),
-    (26, (32, 28, 20), 3, cugip_amd.ENGINE_GRAPH),
+    (26, (32, 28, 20), 3, cugip_amd.ENGINE_AUTO),
     (26, (30, 22, 17), 4, cugip_amd.ENGINE_TILES),
 ])
 def test_labels_are_sink_distances(conn, dims, seed, engine):
