@@ -168,6 +168,18 @@ def batch_workload(cugip_amd, count, seed, repeats=3, cpu=True, plain=True):
         stats = b.stats.as_dict()
         batch_s = statistics.median(ts)
         smin = [np.packbits(b.batch_source_set(i)) for i in range(count)]
+        # the same batch on the device-driven engine (one captured round graph, replayed)
+        b.set_policy(engine=cugip_amd.ENGINE_GRAPH)
+        b.solve()  # captures the round graph
+        tg = []
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            graph_flows = b.solve()
+            tg.append(time.perf_counter() - t0)
+        graph_stats = b.stats.as_dict()
+        graph_s = statistics.median(tg)
+        graph_same = list(graph_flows) == list(flows) and all(
+            np.array_equal(np.packbits(b.batch_source_set(i)), smin[i]) for i in range(count))
     plain_s = 0.0
     same = True
     with cugip_amd.GridGraph() as g:
@@ -184,8 +196,9 @@ def batch_workload(cugip_amd, count, seed, repeats=3, cpu=True, plain=True):
     out = dict(workload=f"{count} grids, edge lengths log-uniform in [32, 128] per axis (seed {seed}), "
                         f"6-connected, {vox} voxels",
                batched_ms=round(batch_s * 1e3, 3), batched_mvox_s=round(vox / batch_s / 1e6, 1),
-               replays=stats["graph_replays"], rounds=stats["rounds"],
-               tile=[stats["tile_x"], stats["tile_y"], stats["tile_z"]])
+               rounds=stats["rounds"], tile=[stats["tile_x"], stats["tile_y"], stats["tile_z"]],
+               graph_engine=dict(ms=round(graph_s * 1e3, 3), mvox_s=round(vox / graph_s / 1e6, 1),
+                                 replays=graph_stats["graph_replays"], same_flows_and_smin=bool(graph_same)))
     if plain:
         out.update(plain_ms=round(plain_s * 1e3, 3), plain_mvox_s=round(vox / plain_s / 1e6, 1),
                    speedup=round(plain_s / batch_s, 3), bit_exact=bool(same))
@@ -262,11 +275,13 @@ def generic_graph(cugip_amd, n, seed, repeats=3, cpu=True):
 def learned_vs_plain(cugip_amd, n, seed, count, weights, repeats=3):
     """The learned cut proposal on the batched-comparison grids: per grid, inference + certificate
     (capacity of the predicted cut) + the certified solve, against the plain solve; flows and S_min
-    must be identical. Also the learned+batched pipeline: inference for every grid plus one batched
-    solve (the certificate applies to single grids, so the batch solve runs to completion)."""
+    must be identical. exact_certificate_ms: the certified solve given the max flow itself (no
+    inference), the bound on what any proposal can save. Also the learned+batched pipeline:
+    inference for every grid plus one batched solve (the certificate applies to single grids, so
+    the batch solve runs to completion)."""
     import numpy as np
     params = np.load(weights)
-    plain = learned = infer_total = 0.0
+    plain = learned = infer_total = ceiling = 0.0
     early = 0
     exact = True
     gaps = []
@@ -299,10 +314,19 @@ def learned_vs_plain(cugip_amd, n, seed, count, weights, repeats=3):
                 early += int(stopped)
                 gaps.append((cert - flow) / flow)
                 exact &= f == flow and np.array_equal(np.packbits(g.source_set()), smin)
+                # the most any proposal could save: the max flow itself as the certificate, no inference
+                ts = []
+                for _ in range(repeats):
+                    t0 = time.perf_counter()
+                    f, stopped = g.max_flow_certified(flow)
+                    ts.append(time.perf_counter() - t0)
+                ceiling += min(ts)
+                exact &= f == flow and bool(stopped)
     return dict(workload=f"{count} grids of {n}^3 6-connected (seed {seed}), weights {os.path.basename(weights)} "
                          "(trained on 2000 grids of 128^3)",
                 speedup=round(plain / learned, 3), plain_ms=round(plain * 1e3, 3), learned_ms=round(learned * 1e3, 3),
                 inference_ms=round(infer_total * 1e3, 3), stopped_early=early,
+                exact_certificate_ms=round(ceiling * 1e3, 3), exact_certificate_speedup=round(plain / ceiling, 3),
                 median_relative_certificate_gap=float(np.median(gaps)), bit_exact=bool(exact))
 
 

@@ -138,3 +138,54 @@ def test_resident_volume_and_trained_weights():
         assert flow <= c_dev <= flow * 1.01, (flow, c_dev)
         f, _ = g.max_flow_certified(c_dev)
         assert f == flow and np.array_equal(g.source_set(), smin)
+
+
+def test_learned_certificate_on_held_out_192_and_256_grids():
+    """The committed weights on held-out grids of the learned workload's shapes (8 of 192^3 and 2
+    of 256^3, seeds outside the training range 100000-101999): every certified solve returns the
+    plain solve's flow and S_min bit for bit; the trained network's certificate is an upper bound
+    on the max flow and tighter than the untrained network's on every grid (what training gains);
+    early stops, and the solve times of plain, trained-certificate, untrained-certificate and
+    exact-certificate solves (the last is the most any proposal could save), are reported."""
+    import os
+    weights = os.path.join(os.path.dirname(cugip_amd.__file__), "cnn_weights.npy")
+    grids = [(192, 900000 + i) for i in range(8)] + [(256, 900100 + i) for i in range(2)]
+    trained, untrained = np.load(weights), cugip_amd.Cnn.init_params(0)
+    early = 0
+    times = {"plain": 0.0, "trained": 0.0, "untrained": 0.0, "exact": 0.0}
+
+    def timed(fn):
+        best = None
+        for _ in range(2):
+            t0 = time.perf_counter()
+            out = fn()
+            dt = time.perf_counter() - t0
+            best = dt if best is None or dt < best else best
+        return out, best
+
+    with cugip_amd.Cnn() as net:
+        for n, seed in grids:
+            say(f"learned held-out {n}^3 seed {seed}")
+            vol = cugip_amd.generate_volume(n, n, n, seed)
+            with cugip_amd.GridGraph() as g:
+                g.set_grid_volume(vol)
+                flow, dt = timed(g.max_flow)
+                times["plain"] += dt
+                smin = g.source_set()
+                net.set_params(untrained)
+                cert0 = net.certificate(g)
+                net.set_params(trained)
+                cert1 = net.certificate(g)
+                say(f"  flow {flow}, certificate gap trained {cert1 - flow}, untrained {cert0 - flow}")
+                assert flow <= cert1 < cert0
+                for name, cert in (("trained", cert1), ("untrained", cert0), ("exact", flow)):
+                    (f, stopped), dt = timed(lambda: g.max_flow_certified(cert))
+                    times[name] += dt
+                    assert f == flow, name
+                    assert np.array_equal(g.source_set(), smin), name
+                    if name == "trained":
+                        early += int(stopped)
+                    if name == "exact":
+                        assert stopped
+    say(f"  {len(grids)} grids: trained certificate stopped {early} solves early; total ms " +
+        ", ".join(f"{k} {1e3 * v:.2f}" for k, v in times.items()))
