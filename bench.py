@@ -366,7 +366,7 @@ def traffic_from_counters(fetch_csv, write_csv, out_path, copy_bytes):
     last_init = [i for i in ids if "k_grid_init" in fa[i][0]][-1]
     names = {"k_grid_discharge": "grid_discharge", "k_grid_bfs": "grid_bfs", "k_grid_init": "grid_init",
              "k_grid_source_closure": "grid_source_closure"}
-    out = dict(workload="last solve of `bench.py --calibration-copies --steps 1 --warmup 1 --no-cpu --batch 0`",
+    out = dict(workload="last solve of `bench.py --calibration-copies --steps 1 --warmup 1 --no-cpu --batch 0 --generic 0 --batch-workload 0`",
                calibration=dict(fetch_factor=round(fcal, 4), write_factor=round(wcal, 4),
                                 how="bytes of the three device-to-device copies / their FETCH_SIZE and WRITE_SIZE"))
     for i in ids:
@@ -416,6 +416,8 @@ def main():
                     help="grids of the batch workload leg, edge lengths log-uniform in [32, 128] (0: skip)")
     ap.add_argument("--batch-plain", type=int, default=1,
                     help="0: skip the batch leg's one-by-one plain solves (each grid captures its own round graph)")
+    ap.add_argument("--batch-workload-only", action="store_true",
+                    help="run only the batch workload leg and print its record (for kernel traces of that leg)")
     ap.add_argument("--calibration-copies", action="store_true",
                     help="start with device-to-device copies of known size (for rocprofv3 --pmc)")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
@@ -443,6 +445,12 @@ def main():
     if cugip_amd.device_count() < 1:
         log("no HIP device visible")
         sys.exit(1)
+
+    if args.batch_workload_only:
+        rec = batch_workload(cugip_amd, args.batch_workload, args.seed, cpu=not args.no_cpu,
+                             plain=bool(args.batch_plain))
+        print(json.dumps(rec))
+        return
 
     n = args.size
     nx, ny, nz = args.shape if args.shape else (n, n, n)

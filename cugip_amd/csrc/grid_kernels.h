@@ -33,6 +33,26 @@ namespace cgp {
 constexpr uint32_t kStateSink = 1u << 29;
 constexpr uint32_t kStateExcess = 1u << 30;
 
+// Tile of discharge block b. A contiguous list (class_stride 0) has its length in count[0]; a list
+// in class regions (class_stride > 0) keeps class c's tiles at list + c * class_stride and their
+// number in count[c], and blocks take the classes in order. -1: no tile for this block.
+__device__ inline int discharge_tile(const int32_t *list, const int32_t *count, int class_stride, int b)
+{
+	if (class_stride == 0) {
+		return b < *count ? list[b] : -1;
+	}
+	int base = 0;
+#pragma unroll
+	for (int c = 0; c < kWorkClasses; ++c) {
+		const int n = count[c];
+		if (b < base + n) {
+			return list[c * class_stride + (b - base)];
+		}
+		base += n;
+	}
+	return -1;
+}
+
 // ---------------------------------------------------------------------------------------------
 // init: residual planes, cancellation, excess, residual sink link. Sets status bit 0 for a negative
 // capacity, bit 1 when a voxel's inflow or an edge's capacity pair does not fit int32, and bit 2
@@ -511,15 +531,15 @@ struct DirtyBits {
 
 template <int kConn, int kTX, int kTY, int kTZ, typename RT>
 __global__ __launch_bounds__(kTX *kTY, 6) void k_grid_discharge(GridView gv, const int32_t *__restrict__ list,
-                                                             const int32_t *__restrict__ count, int max_iters,
-                                                             uint8_t *__restrict__ tile_active)
+                                                             const int32_t *__restrict__ count, int class_stride,
+                                                             int max_iters, uint8_t *__restrict__ tile_active)
 {
 	using T = Tile<kConn, kTX, kTY, kTZ>;
 	using D = typename T::D;
 	using Dirty = DirtyBits<kConn>;
 	constexpr int K = T::K;
-	const int tile = list[blockIdx.x];  // in bounds: the grid never exceeds the list's capacity
-	if ((int)blockIdx.x >= *count) {
+	const int tile = discharge_tile(list, count, class_stride, (int)blockIdx.x);
+	if (tile < 0) {
 		return;
 	}
 	int local;
@@ -1029,8 +1049,8 @@ __global__ __launch_bounds__(256) void k_sum_slots(const unsigned long long *__r
 // consumed flags go back as whole 16-byte words.
 constexpr int kCompactThreads = 1024;
 // one piece per thread for up to 32768 tiles (the solver uses it with one piece; beyond, lists go
-// through the multi-workgroup passes: k_compact_tiles, or k_order_count + k_order_place for the
-// ordered discharge lists, which measured faster than eight pieces a thread in one workgroup)
+// through the multi-workgroup passes: k_compact_tiles, or k_order_regions for the ordered
+// discharge lists, which measured faster than eight pieces a thread in one workgroup)
 constexpr int kCompactBlockTiles = 32 * kCompactThreads;
 
 __device__ inline int block_exclusive_scan_1024(int v, int *wave_sums, int &total)
@@ -1064,9 +1084,10 @@ __device__ inline int block_exclusive_scan_1024(int v, int *wave_sums, int &tota
 // tiles: workgroups of 1024 threads take 4096 tiles (4 per thread, one 4-byte load per array; the
 // flag arrays are padded to whole 32-byte pieces), scan their selections and reserve their part
 // of the list with one atomic each (per-wave atomics on the one count serialise in the L2). The
-// count must be zero beforehand. colours[t] is the colour of tile t inside its instance. With
-// clear set, the selected flags are reset (the pass that takes the list consumes them); with list
-// null, only the count is taken.
+// count must be zero beforehand; zero_next (if set) is cleared, so a solver that alternates
+// between two count words needs no fill before a compaction. colours[t] is the colour of tile t
+// inside its instance. With clear set, the selected flags are reset (the pass that takes the list
+// consumes them); with list null, only the count is taken.
 constexpr int kCompactTilesPerThread = 4;
 constexpr int kCompactTilesPerGroup = kCompactTilesPerThread * kCompactThreads;
 
@@ -1074,12 +1095,16 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact_tiles(uint8_t *__re
                                                                    const uint8_t *__restrict__ colours, int colour,
                                                                    int clear, int32_t *__restrict__ list,
                                                                    int32_t *__restrict__ count,
+                                                                   int32_t *__restrict__ zero_next,
                                                                    const SolveControl *ctl, unsigned gate)
 {
 	__shared__ int sums[kCompactThreads / 64];
 	__shared__ int base;
+	if (zero_next && blockIdx.x == 0 && threadIdx.x == 0) {
+		*zero_next = 0;  // the count of the next compaction (it alternates between two words)
+	}
 	if (!gate_open(ctl, gate)) {
-		return;  // the count stays at the zero it was filled with
+		return;  // the count stays at zero
 	}
 	const int t0 = ((int)blockIdx.x * kCompactThreads + (int)threadIdx.x) * kCompactTilesPerThread;
 	uint32_t sel = 0;
@@ -1165,7 +1190,6 @@ __device__ inline void compact_piece(const uint8_t *flags, const uint8_t *colour
 // open-ended), in tile order within a class: the heaviest tiles start first and do not finish
 // after everyone else. The per-class counts travel packed, 16 bits a class in two 64-bit words, so
 // one workgroup scan orders all classes.
-constexpr int kWorkClasses = 8;
 constexpr int kOrderMin = 2048;
 
 __device__ inline int work_class(int w)
@@ -1463,17 +1487,16 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact_block(uint8_t *__re
 }
 
 // ---------------------------------------------------------------------------------------------
-// Ordered compaction over many workgroups, for lists longer than one workgroup compacts quickly:
-// the same order by decreasing work class, in two passes over 4096 tiles a workgroup (4 a thread,
-// one 4-byte load per array). k_order_count adds each workgroup's per-class counts into
-// ws[0..kWorkClasses) (one atomic per class and workgroup); k_order_place reserves each
-// workgroup's run inside every class's part of the list (one atomic per class and workgroup on
-// ws[kWorkClasses + c], zeroed with the counts beforehand) and stores its entries there in thread
-// order, so a wave's stores of one class land side by side. Within a class the order between
-// workgroups follows their reservations, which does not matter: any order of the tiles of a phase
-// gives the same phase (tiles of one colour never touch).
-constexpr int kOrderWorkspace = 2 * kWorkClasses;
-
+// Ordered compaction over many workgroups in one pass, for lists longer than one workgroup
+// compacts quickly: the list is laid out in class regions (class c's tiles at list + c * stride,
+// stride = the largest colour class, their number in cursors[c]), which the discharge walks in
+// class order (discharge_tile), so no pass needs the totals of the classes before it. Workgroups
+// take 4096 tiles (4 a thread, one 4-byte load per array), scan their per-class counts, and
+// reserve their run in each class's region with one atomic per class (the eight issued together by
+// eight lanes). The cursors must be zero beforehand; zero_next, the other set of a solver that
+// alternates between two, is cleared. Within a class the order between workgroups follows their
+// reservations, which does not matter: any order of the tiles of a phase gives the same phase
+// (tiles of one colour never touch).
 // selection (bit j) and packed per-class counts of the thread's 4 tiles from t0
 __device__ inline uint32_t order_select(const uint8_t *flags, int ntiles, const uint8_t *colours, int colour,
                                         const uint8_t *work, int t0, uint32_t &cls4, unsigned long long &lo,
@@ -1504,45 +1527,21 @@ __device__ inline uint32_t order_select(const uint8_t *flags, int ntiles, const 
 	return sel;
 }
 
-__global__ __launch_bounds__(kCompactThreads) void k_order_count(const uint8_t *__restrict__ flags, int ntiles,
-                                                                 const uint8_t *__restrict__ colours, int colour,
-                                                                 const uint8_t *__restrict__ work,
-                                                                 int32_t *__restrict__ ws, const SolveControl *ctl,
-                                                                 unsigned gate)
-{
-	__shared__ unsigned long long sums[kCompactThreads / 64][2];
-	if (!gate_open(ctl, gate)) {
-		return;
-	}
-	const int t0 = ((int)blockIdx.x * kCompactThreads + (int)threadIdx.x) * kCompactTilesPerThread;
-	uint32_t cls4;
-	unsigned long long lo, hi;
-	order_select(flags, ntiles, colours, colour, work, t0, cls4, lo, hi);
-	unsigned long long tlo = 0, thi = 0;
-	block_exclusive_scan_pair_1024(lo, hi, sums, tlo, thi);
-	if (threadIdx.x < kWorkClasses) {
-		const int c = (int)threadIdx.x;
-		const int n = (int)(((c < 4 ? tlo : thi) >> (16 * (c & 3))) & 0xffffull);
-		if (n) {
-			atomicAdd(&ws[c], n);
-		}
-	}
-}
-
-__global__ __launch_bounds__(kCompactThreads) void k_order_place(const uint8_t *__restrict__ flags, int ntiles,
-                                                                 const uint8_t *__restrict__ colours, int colour,
-                                                                 const uint8_t *__restrict__ work,
-                                                                 int32_t *__restrict__ ws, int32_t *__restrict__ list,
-                                                                 int32_t *__restrict__ count, const SolveControl *ctl,
-                                                                 unsigned gate)
+__global__ __launch_bounds__(kCompactThreads) void k_order_regions(const uint8_t *__restrict__ flags, int ntiles,
+                                                                   const uint8_t *__restrict__ colours, int colour,
+                                                                   const uint8_t *__restrict__ work,
+                                                                   int32_t *__restrict__ list, int stride,
+                                                                   int32_t *__restrict__ cursors,
+                                                                   int32_t *__restrict__ zero_next,
+                                                                   const SolveControl *ctl, unsigned gate)
 {
 	__shared__ unsigned long long sums[kCompactThreads / 64][2];
 	__shared__ int base[kWorkClasses];
+	if (blockIdx.x == 0 && threadIdx.x < kWorkClasses) {
+		zero_next[threadIdx.x] = 0;
+	}
 	if (!gate_open(ctl, gate)) {
-		if (blockIdx.x == 0 && threadIdx.x == 0) {
-			*count = 0;  // a closed gate lists nothing
-		}
-		return;
+		return;  // the cursors stay at zero: an empty list
 	}
 	const int t0 = ((int)blockIdx.x * kCompactThreads + (int)threadIdx.x) * kCompactTilesPerThread;
 	uint32_t cls4;
@@ -1550,17 +1549,10 @@ __global__ __launch_bounds__(kCompactThreads) void k_order_place(const uint8_t *
 	const uint32_t sel = order_select(flags, ntiles, colours, colour, work, t0, cls4, lo, hi);
 	unsigned long long tlo = 0, thi = 0;
 	block_exclusive_scan_pair_1024(lo, hi, sums, tlo, thi);  // lo, hi: this thread's offsets per class
-	if (threadIdx.x == 0) {
-		// class c's part of the list starts after the totals of the classes before it
-		int start = 0;
-		for (int c = 0; c < kWorkClasses; ++c) {
-			const int n = (int)(((c < 4 ? tlo : thi) >> (16 * (c & 3))) & 0xffffull);
-			base[c] = n ? start + atomicAdd(&ws[kWorkClasses + c], n) : 0;
-			start += ws[c];
-		}
-		if (blockIdx.x == 0) {
-			*count = start;
-		}
+	if (threadIdx.x < kWorkClasses) {
+		const int c = (int)threadIdx.x;
+		const int n = (int)(((c < 4 ? tlo : thi) >> (16 * (c & 3))) & 0xffffull);
+		base[c] = c * stride + (n ? atomicAdd(&cursors[c], n) : 0);
 	}
 	__syncthreads();
 	int next[kWorkClasses];
@@ -1583,7 +1575,7 @@ __global__ __launch_bounds__(kCompactThreads) void k_order_place(const uint8_t *
 	}
 }
 
-// ---------------------------------------------------------------------------------------------// ---------------------------------------------------------------------------------------------
+// ---------------------------------------------------------------------------------------------
 // Controllers of the device-driven solve (single workgroups; SolveControl in grid_control.h).
 
 // Start of a device-driven solve (one thread).

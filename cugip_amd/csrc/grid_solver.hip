@@ -249,11 +249,14 @@ void GridSolver::allocate_grids(int conn, const std::vector<InstanceDesc> &inst,
 	mS.allocate(total, mArena);
 	mOmask.allocate(total, mArena);
 	mFlow.allocate(count * kSlotWords, mArena);
-	mFlowTotal.allocate(count, mArena);
 	mCounters.allocate(3 * kSlotWords, mArena);
-	mCounterTotal.allocate(3, mArena);
 	mStatus.allocate(1, mArena);
-	mCount.allocate(4, mArena);  // [0] list count, [1] [2] count_two, [3] closure reached t
+	// one region read back whole by the closure's checks: the counts (int32 [0] list count, [1] [2]
+	// count_two, [3] closure reached t), then the 3 visit-counter totals and the per-grid flow totals
+	mReadback.allocate(kReadbackHead + count, mArena);
+	mCount.borrow(reinterpret_cast<int32_t *>(mReadback.get()), 4);
+	mCounterTotal.borrow(mReadback.get() + 2, 3);
+	mFlowTotal.borrow(mReadback.get() + kReadbackHead, count);
 	// control block of the device-driven solve, then its visit-counter and per-grid flow totals
 	mCtl.allocate(kCtlTotals + 3 + count, mArena);
 	// read-backs: flows + visit counters, or the control block (pageable host memory in arena mode)
@@ -494,8 +497,7 @@ void GridSolver::configure_tiles(const Shape &shape)
 	mTileWork.allocate(padded, mArena);
 	mDirtyA.allocate(padded, mArena);
 	mDirtyB.allocate(padded, mArena);
-	mList.allocate(mTiles, mArena);
-	mOrderWs.allocate(kOrderWorkspace, mArena);
+	mCursors.allocate(kCursorWords, mArena);
 	mAllTiles.allocate(mTiles + 1, mArena);
 	std::vector<int32_t> all(mTiles + 1);
 	for (int t = 0; t < mTiles; ++t) {
@@ -505,6 +507,12 @@ void GridSolver::configure_tiles(const Shape &shape)
 	CGP_HIP(hipMemcpyAsync(mAllTiles.get(), all.data(), all.size() * sizeof(int32_t), hipMemcpyHostToDevice, mStream));
 	build_colours(mPhase, mConn == 6 ? 2 : 8, mConn != 6);
 	build_colours(mSweep, 2, false);
+	// beyond one workgroup's compaction, ordered discharge lists are laid out in class regions of
+	// the largest colour class each
+	mClassStride = mPhase.max_size;
+	mList.allocate(mTiles > kCompactBlockTiles ? std::max<int64_t>(mTiles, (int64_t)kWorkClasses * mClassStride)
+	                                           : (int64_t)mTiles,
+	               mArena);
 	// workgroups of the voxel-parallel init (init_blocks per grid)
 	{
 		std::vector<int32_t> first(mInst.size());
@@ -620,35 +628,38 @@ void GridSolver::launch_compact(const ColourSet &set, uint8_t *flags, int colour
 			                   set.colour.get(), colour, clear ? 1 : 0, mList.get(), mCount.get(),
 			                   (const uint8_t *)nullptr, (int32_t *)nullptr, work, ctl, gate);
 		});
+		mListCount = mCount.get();
+		mDischargeStride = 0;
 		return;
 	}
 	const int blocks = (mTiles + kCompactTilesPerGroup - 1) / kCompactTilesPerGroup;
 	if (work) {
-		// ordered by work class over many workgroups (the lists of discharge phases never clear flags)
-		launch(kFill, 0, 0.0, [&] {
-			hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, mStream, mOrderWs.get(), (int64_t)kOrderWorkspace, 0,
-			                   (const SolveControl *)nullptr, 0u);
-		});
+		// ordered by work class into class regions (the lists of discharge phases never clear flags);
+		// the cursors alternate between two sets, each compaction clearing the other
+		int32_t *cursors = mCursors.get() + kWorkClasses * mOrderParity;
+		int32_t *next = mCursors.get() + kWorkClasses * (mOrderParity ^ 1);
 		launch(kCompact, 0, 0.0, [&] {
-			hipLaunchKernelGGL(k_order_count, dim3(blocks), dim3(kCompactThreads), 0, mStream, (const uint8_t *)flags,
-			                   mTiles, (const uint8_t *)set.colour.get(), colour, work, mOrderWs.get(), ctl, gate);
-			hipLaunchKernelGGL(k_order_place, dim3(blocks), dim3(kCompactThreads), 0, mStream, (const uint8_t *)flags,
-			                   mTiles, (const uint8_t *)set.colour.get(), colour, work, mOrderWs.get(), mList.get(),
-			                   mCount.get(), ctl, gate);
+			hipLaunchKernelGGL(k_order_regions, dim3(blocks), dim3(kCompactThreads), 0, mStream, (const uint8_t *)flags,
+			                   mTiles, (const uint8_t *)set.colour.get(), colour, work, mList.get(), mClassStride,
+			                   cursors, next, ctl, gate);
 		});
+		mListCount = cursors;
+		mDischargeStride = mClassStride;
+		mOrderParity ^= 1;
 		return;
 	}
-	launch(kFill, 0, 0.0, [&] {
-		hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, mStream, mCount.get(), (int64_t)1, 0,
-		                   (const SolveControl *)nullptr, 0u);
-	});
+	// the count alternates between two words, each compaction clearing the other
+	int32_t *count = mCursors.get() + 2 * kWorkClasses + mListParity;
+	int32_t *next = mCursors.get() + 2 * kWorkClasses + (mListParity ^ 1);
 	launch(kCompact, 0, 0.0, [&] {
 		hipLaunchKernelGGL(k_compact_tiles, dim3(blocks), dim3(kCompactThreads), 0, mStream, flags, mTiles,
-		                   set.colour.get(), colour, clear ? 1 : 0, mList.get(), mCount.get(), ctl, gate);
+		                   set.colour.get(), colour, clear ? 1 : 0, mList.get(), count, next, ctl, gate);
 	});
+	mListCount = count;
+	mDischargeStride = 0;
+	mListParity ^= 1;
 }
 
-// Input errors and the residual width found by the init kernel (status bits 0-2).
 void GridSolver::apply_status(int32_t status)
 {
 	mStatusPending = false;
@@ -663,11 +674,16 @@ void GridSolver::apply_status(int32_t status)
 }
 
 // Counts the set flags of two arrays with one host read (which also brings a pending init status).
+// The host copy holds the count words at int32 [1] [2] (and [3], the closure's reached flag); during
+// a closure (mPrefetchTotals) the read also brings the visit-counter and flow totals, since the
+// solve ends at one of the closure's checks (mTotalsRead then says the copy is current).
 void GridSolver::count_two(uint8_t *a, uint8_t *b, int &na, int &nb)
 {
 	int32_t *pinned = reinterpret_cast<int32_t *>(mHostScratch.get());
+	const size_t n = mInst.size();
+	int32_t *status_copy = pinned + 2 * (kReadbackHead + n);
 	if (mStatusPending) {
-		CGP_HIP(hipMemcpyAsync(pinned + 4, mStatus.get(), sizeof(int32_t), hipMemcpyDeviceToHost, mStream));
+		CGP_HIP(hipMemcpyAsync(status_copy, mStatus.get(), sizeof(int32_t), hipMemcpyDeviceToHost, mStream));
 	}
 	if (mTiles <= kCompactBlockTiles) {
 		launch(kCompact, 0, 0.0, [&] {
@@ -683,22 +699,47 @@ void GridSolver::count_two(uint8_t *a, uint8_t *b, int &na, int &nb)
 		launch(kCompact, 0, 0.0, [&] {
 			const int blocks = (mTiles + kCompactTilesPerGroup - 1) / kCompactTilesPerGroup;
 			hipLaunchKernelGGL(k_compact_tiles, dim3(blocks), dim3(kCompactThreads), 0, mStream, a, mTiles,
-			                   mSweep.colour.get(), -1, 0, (int32_t *)nullptr, mCount.get() + 1,
+			                   mSweep.colour.get(), -1, 0, (int32_t *)nullptr, mCount.get() + 1, (int32_t *)nullptr,
 			                   (const SolveControl *)nullptr, 0u);
 			if (b) {
 				hipLaunchKernelGGL(k_compact_tiles, dim3(blocks), dim3(kCompactThreads), 0, mStream, b, mTiles,
 				                   mSweep.colour.get(), -1, 0, (int32_t *)nullptr, mCount.get() + 2,
-				                   (const SolveControl *)nullptr, 0u);
+				                   (int32_t *)nullptr, (const SolveControl *)nullptr, 0u);
 			}
 		});
 	}
-	CGP_HIP(hipMemcpyAsync(pinned, mCount.get() + 1, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, mStream));
-	CGP_HIP(hipStreamSynchronize(mStream));
-	if (mStatusPending) {
-		apply_status(pinned[4]);
+	size_t bytes = 4 * sizeof(int32_t);
+	if (mPrefetchTotals) {
+		hipLaunchKernelGGL(k_sum_slots, dim3(1), dim3(64), 0, mStream, mCounters.get(), (int64_t)3, mCounterTotal.get());
+		hipLaunchKernelGGL(k_sum_slots, dim3(blocks_for((int64_t)n, 256)), dim3(256), 0, mStream, mFlow.get(),
+		                   (int64_t)n, mFlowTotal.get());
+		CGP_LAUNCH_CHECK();
+		bytes = (kReadbackHead + n) * sizeof(unsigned long long);
 	}
-	na = pinned[0];
-	nb = b ? pinned[1] : 0;
+	CGP_HIP(hipMemcpyAsync(pinned, mCount.get(), bytes, hipMemcpyDeviceToHost, mStream));
+	CGP_HIP(hipStreamSynchronize(mStream));
+	mTotalsRead = mPrefetchTotals;
+	if (mStatusPending) {
+		apply_status(*status_copy);
+	}
+	na = pinned[1];
+	nb = b ? pinned[2] : 0;
+}
+
+// Flow of every grid (into mFlows) and the three visit counters from the totals the last closure
+// check read back.
+int64_t GridSolver::totals_from_readback(unsigned long long *visits)
+{
+	const unsigned long long *words = reinterpret_cast<const unsigned long long *>(mHostScratch.get());
+	for (int k = 0; k < 3; ++k) {
+		visits[k] = words[2 + k];
+	}
+	int64_t total = 0;
+	for (size_t i = 0; i < mInst.size(); ++i) {
+		mFlows[i] = (int64_t)words[kReadbackHead + i];
+		total += mFlows[i];
+	}
+	return total;
 }
 
 // Visits every tile once, colour by colour, then keeps visiting the dirty tiles colour by colour
@@ -741,7 +782,7 @@ int GridSolver::colour_sweeps(uint8_t *dirty, uint8_t *also_count, int &also, Pa
 			if (stop) {
 				// the sweeps' own flag (the closure reaching t), read with the counts
 				const int32_t *pinned = reinterpret_cast<const int32_t *>(mHostScratch.get());
-				if (pinned[2]) {
+				if (pinned[3]) {
 					*stop = true;
 					return launches;
 				}
@@ -753,7 +794,7 @@ int GridSolver::colour_sweeps(uint8_t *dirty, uint8_t *also_count, int &also, Pa
 		for (int cycle = 0; cycle < kCyclesPerCheck; ++cycle) {
 			for (int c = 0; c < cs.colours; ++c) {
 				launch_compact(cs, dirty, c, true);
-				pass(mList.get(), mCount.get(), cs.max_size, false);
+				pass(mList.get(), mListCount, cs.max_size, false);
 				++launches;
 			}
 		}
@@ -805,12 +846,12 @@ void GridSolver::launch_discharge(int max_iters, bool wide)
 			using S = decltype(tag);
 			if (wide) {
 				hipLaunchKernelGGL((k_grid_discharge<S::conn, S::tx, S::ty, S::tz, int32_t>), dim3(blocks),
-				                   dim3(S::tx * S::ty), 0, mStream, g, mList.get(), mCount.get(), max_iters,
-				                   mActive.get());
+				                   dim3(S::tx * S::ty), 0, mStream, g, mList.get(), mListCount, mDischargeStride,
+				                   max_iters, mActive.get());
 			} else {
 				hipLaunchKernelGGL((k_grid_discharge<S::conn, S::tx, S::ty, S::tz, int16_t>), dim3(blocks),
-				                   dim3(S::tx * S::ty), 0, mStream, g, mList.get(), mCount.get(), max_iters,
-				                   mActive.get());
+				                   dim3(S::tx * S::ty), 0, mStream, g, mList.get(), mListCount, mDischargeStride,
+				                   max_iters, mActive.get());
 			}
 		});
 	});
@@ -860,12 +901,20 @@ bool GridSolver::run_source_closure(cgp_stats &stats, bool check)
 	});
 	int unused = 0;
 	bool hit = false;
-	stats.cut_passes += colour_sweeps(
-	    mDirtyB.get(), nullptr, unused,
-	    [&](const int32_t *list, const int32_t *count, int grid, bool first) {
-		    launch_closure_pass(list, count, grid, first, reached, nullptr, first ? grid * tile_voxels : -1);
-	    },
-	    check ? &hit : nullptr);
+	mTotalsRead = false;
+	mPrefetchTotals = true;
+	try {
+		stats.cut_passes += colour_sweeps(
+		    mDirtyB.get(), nullptr, unused,
+		    [&](const int32_t *list, const int32_t *count, int grid, bool first) {
+			    launch_closure_pass(list, count, grid, first, reached, nullptr, first ? grid * tile_voxels : -1);
+		    },
+		    check ? &hit : nullptr);
+	} catch (...) {
+		mPrefetchTotals = false;
+		throw;
+	}
+	mPrefetchTotals = false;
 	// (the last count of the sweeps read the flag after their last pass)
 	return hit;
 }
@@ -976,7 +1025,7 @@ void GridSolver::capture_round_graph(int phases, int max_iters, bool wide)
 		for (int cycle = 0; cycle < kGraphSweepCycles; ++cycle) {
 			for (int c = 0; c < 2; ++c) {
 				launch_compact(mSweep, mDirtyA.get(), c, true, nullptr, ctl, kGateRelabel);
-				launch_bfs_pass(mList.get(), mCount.get(), mSweep.max_size, 0, -1);
+				launch_bfs_pass(mList.get(), mListCount, mSweep.max_size, 0, -1);
 			}
 		}
 		hipLaunchKernelGGL(k_ctl_relabel_end, dim3(1), dim3(kCtlThreads), 0, mStream, ctl,
@@ -1002,7 +1051,7 @@ void GridSolver::capture_round_graph(int phases, int max_iters, bool wide)
 		for (int cycle = 0; cycle < kGraphSweepCycles; ++cycle) {
 			for (int c = 0; c < 2; ++c) {
 				launch_compact(mSweep, mDirtyB.get(), c, true, nullptr, ctl, kGateClosure | kGateUnlessReached);
-				launch_closure_pass(mList.get(), mCount.get(), mSweep.max_size, false, &ctl->reached, ctl, -1);
+				launch_closure_pass(mList.get(), mListCount, mSweep.max_size, false, &ctl->reached, ctl, -1);
 			}
 		}
 		hipLaunchKernelGGL(k_ctl_closure_end, dim3(1), dim3(kCtlThreads), 0, mStream, ctl,
@@ -1131,6 +1180,10 @@ int64_t GridSolver::solve(const cgp_policy &policy, cgp_stats *stats_out, const 
 		rec = KernelRecord();
 	}
 	configure_tiles(pick_shape(policy));
+	// the alternating counts and class cursors of the multi-workgroup compactions start at zero
+	hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, mStream, mCursors.get(), (int64_t)kCursorWords, 0,
+	                   (const SolveControl *)nullptr, 0u);
+	mOrderParity = mListParity = 0;
 	const int ncolours = mConn == 6 ? 2 : 8;
 	const TunedPolicy tp = tuned_policy(mConn, mInst.size() > 1, mVox / (int64_t)mInst.size());
 	const int phases = policy.discharges_per_relabel > 0 ? policy.discharges_per_relabel : tp.phases;
@@ -1259,7 +1312,8 @@ int64_t GridSolver::solve(const cgp_policy &policy, cgp_stats *stats_out, const 
 		run_source_closure(stats);
 	}
 	unsigned long long visits[3];
-	const int64_t flow = read_flow(visits);
+	// the closure's last check normally brought the totals already
+	const int64_t flow = mTotalsRead ? totals_from_readback(visits) : read_flow(visits);
 	stats.cut_ms += ms_since(t0);
 	stats.discharge_tile_visits = (int64_t)visits[0];
 	stats.bfs_tile_visits = (int64_t)visits[1];

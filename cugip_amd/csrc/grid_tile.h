@@ -98,6 +98,9 @@ struct GridView {
 // The flow of an instance and the visit counters are sums over kSlots slots, each on its own
 // 128-byte line, picked by tile id: thousands of workgroups adding into one address serialise in
 // the L2 and hold up every other request queued behind them.
+// Work classes of the ordered discharge lists (heaviest first).
+constexpr int kWorkClasses = 8;
+
 constexpr int kSlots = 32;
 constexpr int kSlotStride = 16;  // unsigned long longs per slot (128 bytes)
 constexpr int kSlotWords = kSlots * kSlotStride;  // per instance (flow) or per counter kind

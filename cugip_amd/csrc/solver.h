@@ -203,7 +203,19 @@ private:
 	DeviceBuffer<int32_t> mStatus, mCount, mList, mAllTiles;
 	DeviceBuffer<uint8_t> mActive, mDirtyA, mDirtyB;
 	DeviceBuffer<uint8_t> mTileWork;  // per tile: discharge work estimate from the last relabel
-	DeviceBuffer<int32_t> mOrderWs;   // ordered multi-workgroup compaction: class counts and cursors
+	// multi-workgroup compactions: two sets of class cursors of the ordered lists, then two list
+	// counts; each compaction uses one and clears the other (zeroed at the start of a solve)
+	DeviceBuffer<int32_t> mCursors;
+	static constexpr int kCursorWords = 2 * kWorkClasses + 2;
+	int mOrderParity = 0, mListParity = 0;
+	int mClassStride = 0;                   // class region size of the ordered lists
+	const int32_t *mListCount = nullptr;    // count (or class cursors) of the last compacted list
+	int mDischargeStride = 0;               // 0: the last list is contiguous; else its class stride
+	DeviceBuffer<unsigned long long> mReadback;  // counts, visit-counter and flow totals (one read)
+	static constexpr int kReadbackHead = 5;      // 2 words of counts, 3 of visit counters
+	bool mPrefetchTotals = false;  // closure checks also read the flow and visit-counter totals
+	bool mTotalsRead = false;      // the last check did (and nothing ran since)
+	int64_t totals_from_readback(unsigned long long *visits);
 	// device-driven solve: control block (kCtlTotals words), then 3 visit-counter totals and one flow
 	// total per grid
 	DeviceBuffer<unsigned long long> mCtl;
