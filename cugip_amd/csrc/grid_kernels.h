@@ -356,10 +356,14 @@ __device__ inline int leaving_slot(int lx, int ly, int lz, int d)
 // halo labels; a tile whose boundary label fell below what a neighbouring voxel could use marks
 // that neighbour's tile in dirty_next, so a pass that marks nothing ends the relabel. Every
 // processed tile refreshes its active flag (some voxel with excess and a finite label).
+// first_mode replaces the fill on 6-connected grids, whose halo holds only face neighbours, all of
+// the other sweep colour: 1 = first pass of the first colour (own and halo labels read as
+// kLabelInf, nothing is loaded), 2 = first pass of the second colour (own labels read as kLabelInf,
+// the halo is the first colour's fresh labels); both store every own label.
 template <int kConn, int kTX, int kTY, int kTZ>
 __global__ __launch_bounds__(kTX *kTY) void k_grid_bfs(GridView gv, const int32_t *__restrict__ list,
                                                        const int32_t *__restrict__ count, int32_t threshold,
-                                                       uint8_t *__restrict__ dirty_next,
+                                                       int first_mode, uint8_t *__restrict__ dirty_next,
                                                        uint8_t *__restrict__ tile_active,
                                                        uint8_t *__restrict__ tile_work)
 {
@@ -389,7 +393,24 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_bfs(GridView gv, const int32_
 	}
 	// every global load of the visit first (halo labels, state words), then the LDS stores
 	HaloLabels<T> halo;
-	fetch_label_tile<T>(g, halo, ox, oy, oz);
+	if (first_mode == 1) {
+#pragma unroll
+		for (int pz = 0; pz < T::PZ; ++pz) {
+			halo.col[pz] = kLabelInf;
+		}
+#pragma unroll
+		for (int it = 0; it < HaloLabels<T>::kRingPer; ++it) {
+			halo.ring[it] = kLabelInf;
+		}
+	} else {
+		fetch_label_tile<T>(g, halo, ox, oy, oz);
+		if (first_mode == 2) {
+#pragma unroll
+			for (int k = 0; k < T::TZ; ++k) {
+				halo.col[k + 1] = kLabelInf;  // own labels are not written yet
+			}
+		}
+	}
 	const uint32_t xy = slice_xy(g, gx, gy), sxy = (uint32_t)(g.nx * g.ny);
 	uint32_t valid = 0;
 	uint32_t state[T::TZ];
@@ -449,6 +470,9 @@ __global__ __launch_bounds__(kTX *kTY) void k_grid_bfs(GridView gv, const int32_
 			top = nl;
 		}
 		bottom = nl < bottom ? nl : bottom;
+		if (first_mode && nl == before[k]) {
+			tile_at(g.h, g, oz, k, xy, sxy) = nl;  // kLabelInf over whatever the last solve left
+		}
 		if (nl != before[k]) {
 			tile_at(g.h, g, oz, k, xy, sxy) = nl;
 #pragma unroll
@@ -1578,17 +1602,6 @@ __global__ __launch_bounds__(kCompactThreads) void k_order_regions(const uint8_t
 // ---------------------------------------------------------------------------------------------
 // Controllers of the device-driven solve (single workgroups; SolveControl in grid_control.h).
 
-// Start of a device-driven solve (one thread).
-__global__ void k_ctl_reset(SolveControl *ctl, int32_t max_rounds)
-{
-	if (threadIdx.x == 0 && blockIdx.x == 0) {
-		SolveControl c = {};
-		c.mode = kModeInit;
-		c.max_rounds = max_rounds;
-		*ctl = c;
-	}
-}
-
 // Sets the stage (one thread): the host continues a solve that stopped for kModeNeedWide.
 __global__ void k_ctl_set_mode(SolveControl *ctl, int32_t mode)
 {
@@ -1883,6 +1896,114 @@ __global__ __launch_bounds__(256) void k_fill_i32(int32_t *__restrict__ p, int64
 	const int64_t stride = (int64_t)gridDim.x * blockDim.x;
 	for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
 		p[i] = value;
+	}
+}
+
+// Start of a solve, one launch: the init status, the slotted visit counters and flows (one
+// region), the alternating counts of the multi-workgroup compactions and, for the device-driven
+// solve, the control block (cursor_words must not exceed the threads of the launch).
+__global__ __launch_bounds__(256) void k_solve_reset(int32_t *__restrict__ status, unsigned long long *__restrict__ slots,
+                                                     int64_t slot_words, int32_t *__restrict__ cursors, int cursor_words,
+                                                     SolveControl *ctl, int32_t max_rounds)
+{
+	const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+	for (int64_t i = i0; i < slot_words; i += stride) {
+		slots[i] = 0;
+	}
+	if (i0 < cursor_words) {
+		cursors[i0] = 0;
+	}
+	if (i0 == 0) {
+		*status = 0;
+		if (ctl) {
+			SolveControl c = {};
+			c.mode = kModeInit;
+			c.max_rounds = max_rounds;
+			*ctl = c;
+		}
+	}
+}
+
+// Start of an S_min closure, one launch: s[0, n) cleared with 16-byte stores (s is 16-byte
+// aligned), and, when given, the reached flag and the closure's tile flags dirty[0, ntiles).
+__global__ __launch_bounds__(256) void k_closure_reset(uint8_t *__restrict__ s, int64_t n, int32_t *__restrict__ reached,
+                                                       uint8_t *__restrict__ dirty, int64_t ntiles,
+                                                       const SolveControl *ctl, unsigned gate)
+{
+	if (!gate_open(ctl, gate)) {
+		return;
+	}
+	const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+	const int64_t words = n / 16;
+	uint4 *s4 = reinterpret_cast<uint4 *>(s);
+	for (int64_t i = i0; i < words; i += stride) {
+		s4[i] = make_uint4(0u, 0u, 0u, 0u);
+	}
+	for (int64_t i = words * 16 + i0; i < n; i += stride) {
+		s[i] = 0;
+	}
+	if (dirty) {
+		for (int64_t i = i0; i < ntiles; i += stride) {
+			dirty[i] = 0;
+		}
+	}
+	if (reached && i0 == 0) {
+		*reached = 0;
+	}
+}
+
+// Set flags of a[0, n) and b[0, n) (b may be null) in one launch over many workgroups (4096 flags
+// each, 4 a thread by one 4-byte load; the arrays are padded to whole 32-byte pieces): every
+// workgroup adds its counts into acc[0], acc[1] and takes a ticket in acc[2]; the last one stores
+// the totals in out[0], out[1] and clears acc for the next launch (acc starts at zero).
+__global__ __launch_bounds__(1024) void k_count_flags2(const uint8_t *__restrict__ a, const uint8_t *__restrict__ b,
+                                                       int n, int32_t *__restrict__ out, int32_t *__restrict__ acc)
+{
+	__shared__ int sums[2][1024 / 64];
+	const int i = ((int)blockIdx.x * 1024 + (int)threadIdx.x) * 4;
+	int ca = 0, cb = 0;
+	if (i < n) {
+		const uint32_t wa = *reinterpret_cast<const uint32_t *>(a + i);
+		const uint32_t wb = b ? *reinterpret_cast<const uint32_t *>(b + i) : 0u;
+#pragma unroll
+		for (int j = 0; j < 4; ++j) {
+			const bool in = i + j < n;
+			ca += (in && ((wa >> (8 * j)) & 0xffu)) ? 1 : 0;
+			cb += (in && ((wb >> (8 * j)) & 0xffu)) ? 1 : 0;
+		}
+	}
+#pragma unroll
+	for (int off = 32; off > 0; off >>= 1) {
+		ca += __shfl_xor(ca, off, 64);
+		cb += __shfl_xor(cb, off, 64);
+	}
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	if (lane == 0) {
+		sums[0][wave] = ca;
+		sums[1][wave] = cb;
+	}
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		int ta = 0, tb = 0;
+		for (int w = 0; w < 1024 / 64; ++w) {
+			ta += sums[0][w];
+			tb += sums[1][w];
+		}
+		if (ta) {
+			atomicAdd(&acc[0], ta);
+		}
+		if (tb) {
+			atomicAdd(&acc[1], tb);
+		}
+		__threadfence();
+		if (atomicAdd(&acc[2], 1) == (int)gridDim.x - 1) {
+			__threadfence();
+			out[0] = atomicExch(&acc[0], 0);
+			out[1] = atomicExch(&acc[1], 0);
+			atomicExch(&acc[2], 0);
+		}
 	}
 }
 

@@ -248,8 +248,11 @@ void GridSolver::allocate_grids(int conn, const std::vector<InstanceDesc> &inst,
 	mR.allocate((size_t)total * conn, mArena);
 	mS.allocate(total, mArena);
 	mOmask.allocate(total, mArena);
-	mFlow.allocate(count * kSlotWords, mArena);
-	mCounters.allocate(3 * kSlotWords, mArena);
+	// the slotted visit counters (3 kinds) and then the per-grid flows, one region: one kernel sums
+	// both, in the order of the totals of the read-back and of the control block
+	mSlots.allocate((3 + count) * kSlotWords, mArena);
+	mCounters.borrow(mSlots.get(), 3 * kSlotWords);
+	mFlow.borrow(mSlots.get() + 3 * kSlotWords, count * kSlotWords);
 	mStatus.allocate(1, mArena);
 	// one region read back whole by the closure's checks: the counts (int32 [0] list count, [1] [2]
 	// count_two, [3] closure reached t), then the 3 visit-counter totals and the per-grid flow totals
@@ -498,6 +501,9 @@ void GridSolver::configure_tiles(const Shape &shape)
 	mDirtyA.allocate(padded, mArena);
 	mDirtyB.allocate(padded, mArena);
 	mCursors.allocate(kCursorWords, mArena);
+	hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, mStream, mCursors.get(), (int64_t)kCursorWords, 0,
+	                   (const SolveControl *)nullptr, 0u);
+	mDirtyAClean = mDirtyBClean = false;  // new flag arrays
 	mAllTiles.allocate(mTiles + 1, mArena);
 	std::vector<int32_t> all(mTiles + 1);
 	for (int t = 0; t < mTiles; ++t) {
@@ -692,27 +698,16 @@ void GridSolver::count_two(uint8_t *a, uint8_t *b, int &na, int &nb)
 			                   mCount.get() + 2, (const uint8_t *)nullptr, (const SolveControl *)nullptr, 0u);
 		});
 	} else {
-		launch(kFill, 0, 0.0, [&] {
-			hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, mStream, mCount.get() + 1, (int64_t)2, 0,
-			                   (const SolveControl *)nullptr, 0u);
-		});
 		launch(kCompact, 0, 0.0, [&] {
 			const int blocks = (mTiles + kCompactTilesPerGroup - 1) / kCompactTilesPerGroup;
-			hipLaunchKernelGGL(k_compact_tiles, dim3(blocks), dim3(kCompactThreads), 0, mStream, a, mTiles,
-			                   mSweep.colour.get(), -1, 0, (int32_t *)nullptr, mCount.get() + 1, (int32_t *)nullptr,
-			                   (const SolveControl *)nullptr, 0u);
-			if (b) {
-				hipLaunchKernelGGL(k_compact_tiles, dim3(blocks), dim3(kCompactThreads), 0, mStream, b, mTiles,
-				                   mSweep.colour.get(), -1, 0, (int32_t *)nullptr, mCount.get() + 2,
-				                   (int32_t *)nullptr, (const SolveControl *)nullptr, 0u);
-			}
+			hipLaunchKernelGGL(k_count_flags2, dim3(blocks), dim3(1024), 0, mStream, (const uint8_t *)a,
+			                   (const uint8_t *)b, mTiles, mCount.get() + 1, mCursors.get() + kCountAcc);
 		});
 	}
 	size_t bytes = 4 * sizeof(int32_t);
 	if (mPrefetchTotals) {
-		hipLaunchKernelGGL(k_sum_slots, dim3(1), dim3(64), 0, mStream, mCounters.get(), (int64_t)3, mCounterTotal.get());
-		hipLaunchKernelGGL(k_sum_slots, dim3(blocks_for((int64_t)n, 256)), dim3(256), 0, mStream, mFlow.get(),
-		                   (int64_t)n, mFlowTotal.get());
+		hipLaunchKernelGGL(k_sum_slots, dim3(blocks_for((int64_t)(3 + n), 256)), dim3(256), 0, mStream, mSlots.get(),
+		                   (int64_t)(3 + n), mCounterTotal.get());
 		CGP_LAUNCH_CHECK();
 		bytes = (kReadbackHead + n) * sizeof(unsigned long long);
 	}
@@ -759,7 +754,7 @@ int GridSolver::colour_sweeps(uint8_t *dirty, uint8_t *also_count, int &also, Pa
 		if (cs.size[c] == 0) {
 			continue;  // e.g. a grid one tile thick has no tile of the odd colours
 		}
-		pass(cs.tiles.get() + cs.start[c], cs.tiles.get() + mTiles + c, cs.size[c], true);
+		pass(cs.tiles.get() + cs.start[c], cs.tiles.get() + mTiles + c, cs.size[c], c + 1);
 		++launches;
 		if (stop && mConn != 6 && c + 1 < cs.colours) {
 			// a test that fails usually fails in its first pass; 26C solves take several rounds, so
@@ -794,7 +789,7 @@ int GridSolver::colour_sweeps(uint8_t *dirty, uint8_t *also_count, int &also, Pa
 		for (int cycle = 0; cycle < kCyclesPerCheck; ++cycle) {
 			for (int c = 0; c < cs.colours; ++c) {
 				launch_compact(cs, dirty, c, true);
-				pass(mList.get(), mListCount, cs.max_size, false);
+				pass(mList.get(), mListCount, cs.max_size, 0);
 				++launches;
 			}
 		}
@@ -803,7 +798,7 @@ int GridSolver::colour_sweeps(uint8_t *dirty, uint8_t *also_count, int &also, Pa
 
 // One global-relabel pass over `grid` blocks taking tiles from list (count tiles).
 void GridSolver::launch_bfs_pass(const int32_t *list, const int32_t *count, int grid, int32_t threshold,
-                                 int64_t voxels)
+                                 int64_t voxels, int first_mode)
 {
 	GridView g = CGP_VIEW();
 	// exact passes read the state word and h and write h; threshold passes read r[K], rt, e, h
@@ -812,7 +807,8 @@ void GridSolver::launch_bfs_pass(const int32_t *list, const int32_t *count, int 
 		dispatch_shape(mConn, mShape.tx, mShape.ty, mShape.tz, [&](auto tag) {
 			using S = decltype(tag);
 			hipLaunchKernelGGL((k_grid_bfs<S::conn, S::tx, S::ty, S::tz>), dim3(grid), dim3(S::tx * S::ty), 0,
-			                   mStream, g, list, count, threshold, mDirtyA.get(), mActive.get(), mTileWork.get());
+			                   mStream, g, list, count, threshold, first_mode, mDirtyA.get(), mActive.get(),
+			                   mTileWork.get());
 		});
 	});
 }
@@ -860,19 +856,29 @@ void GridSolver::launch_discharge(int max_iters, bool wide)
 int GridSolver::run_global_relabel(int32_t threshold, cgp_stats &stats)
 {
 	const int64_t tile_voxels = (int64_t)mShape.tx * mShape.ty * mShape.tz;
-	launch(kFill, mVox, 4.0, [&] {
-		hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(mVox, 256, 8192)), dim3(256), 0, mStream, mH.get(), mVox,
-		                   kLabelInf, (const SolveControl *)nullptr, 0u);
-	});
-	launch(kFill, 0, 0.0, [&] {
-		hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(mTiles, 256)), dim3(256), 0, mStream, mDirtyA.get(),
-		                   (int64_t)mTiles, (uint8_t)0, (const SolveControl *)nullptr, 0u);
-	});
+	// 6-connected first sweeps start from kLabelInf themselves (first_mode); 26-connected tiles read
+	// diagonal neighbours of their own sweep colour, so the labels are filled first
+	if (mConn != 6) {
+		launch(kFill, mVox, 4.0, [&] {
+			hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(mVox, 256, 8192)), dim3(256), 0, mStream, mH.get(), mVox,
+			                   kLabelInf, (const SolveControl *)nullptr, 0u);
+		});
+	}
+	// a relabel that ran to its end left no dirty flag
+	if (!mDirtyAClean) {
+		launch(kFill, 0, 0.0, [&] {
+			hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(mTiles, 256)), dim3(256), 0, mStream, mDirtyA.get(),
+			                   (int64_t)mTiles, (uint8_t)0, (const SolveControl *)nullptr, 0u);
+		});
+	}
+	mDirtyAClean = false;
 	int active = 0;
 	stats.bfs_passes += colour_sweeps(mDirtyA.get(), mActive.get(), active,
-	                                  [&](const int32_t *list, const int32_t *count, int grid, bool first) {
-		                                  launch_bfs_pass(list, count, grid, threshold, first ? grid * tile_voxels : -1);
+	                                  [&](const int32_t *list, const int32_t *count, int grid, int first) {
+		                                  launch_bfs_pass(list, count, grid, threshold, first ? grid * tile_voxels : -1,
+		                                                  mConn == 6 ? first : 0);
 	                                  });
+	mDirtyAClean = true;
 	return active;
 }
 
@@ -887,18 +893,13 @@ bool GridSolver::run_source_closure(cgp_stats &stats, bool check)
 {
 	const int64_t tile_voxels = (int64_t)mShape.tx * mShape.ty * mShape.tz;
 	int32_t *reached = check ? mCount.get() + 3 : nullptr;
-	launch(kFill, 0, 0.0, [&] {
-		hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, mStream, mCount.get() + 3, (int64_t)1, 0,
+	// s, the reached flag and (unless the last closure consumed them all) the closure's tile flags
+	launch(kFill, mVox, 1.0, [&] {
+		hipLaunchKernelGGL(k_closure_reset, dim3(blocks_for(mVox / 16 + 1, 256, 4096)), dim3(256), 0, mStream, mS.get(),
+		                   mVox, mCount.get() + 3, mDirtyBClean ? (uint8_t *)nullptr : mDirtyB.get(), (int64_t)mTiles,
 		                   (const SolveControl *)nullptr, 0u);
 	});
-	launch(kFill, mVox, 1.0, [&] {
-		hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(mVox, 256, 8192)), dim3(256), 0, mStream, mS.get(), mVox,
-		                   (uint8_t)0, (const SolveControl *)nullptr, 0u);
-	});
-	launch(kFill, 0, 0.0, [&] {
-		hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(mTiles, 256)), dim3(256), 0, mStream, mDirtyB.get(),
-		                   (int64_t)mTiles, (uint8_t)0, (const SolveControl *)nullptr, 0u);
-	});
+	mDirtyBClean = false;
 	int unused = 0;
 	bool hit = false;
 	mTotalsRead = false;
@@ -906,8 +907,8 @@ bool GridSolver::run_source_closure(cgp_stats &stats, bool check)
 	try {
 		stats.cut_passes += colour_sweeps(
 		    mDirtyB.get(), nullptr, unused,
-		    [&](const int32_t *list, const int32_t *count, int grid, bool first) {
-			    launch_closure_pass(list, count, grid, first, reached, nullptr, first ? grid * tile_voxels : -1);
+		    [&](const int32_t *list, const int32_t *count, int grid, int first) {
+			    launch_closure_pass(list, count, grid, first != 0, reached, nullptr, first ? grid * tile_voxels : -1);
 		    },
 		    check ? &hit : nullptr);
 	} catch (...) {
@@ -915,6 +916,7 @@ bool GridSolver::run_source_closure(cgp_stats &stats, bool check)
 		throw;
 	}
 	mPrefetchTotals = false;
+	mDirtyBClean = !hit;  // the sweeps ran until no flag was left
 	// (the last count of the sweeps read the flag after their last pass)
 	return hit;
 }
@@ -937,25 +939,20 @@ int64_t GridSolver::read_flow(unsigned long long *visits)
 {
 	const size_t n = mInst.size();
 	unsigned long long *pinned = reinterpret_cast<unsigned long long *>(mHostScratch.get());
-	hipLaunchKernelGGL(k_sum_slots, dim3(blocks_for((int64_t)n, 256)), dim3(256), 0, mStream, mFlow.get(), (int64_t)n,
-	                   mFlowTotal.get());
+	hipLaunchKernelGGL(k_sum_slots, dim3(blocks_for((int64_t)(3 + n), 256)), dim3(256), 0, mStream, mSlots.get(),
+	                   (int64_t)(3 + n), mCounterTotal.get());
 	CGP_LAUNCH_CHECK();
-	CGP_HIP(hipMemcpyAsync(pinned, mFlowTotal.get(), n * sizeof(unsigned long long), hipMemcpyDeviceToHost, mStream));
-	if (visits) {
-		hipLaunchKernelGGL(k_sum_slots, dim3(1), dim3(64), 0, mStream, mCounters.get(), (int64_t)3, mCounterTotal.get());
-		CGP_LAUNCH_CHECK();
-		CGP_HIP(hipMemcpyAsync(pinned + n, mCounterTotal.get(), 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-		                       mStream));
-	}
+	CGP_HIP(hipMemcpyAsync(pinned, mCounterTotal.get(), (3 + n) * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+	                       mStream));
 	CGP_HIP(hipStreamSynchronize(mStream));
 	if (visits) {
-		visits[0] = pinned[n];
-		visits[1] = pinned[n + 1];
-		visits[2] = pinned[n + 2];
+		visits[0] = pinned[0];
+		visits[1] = pinned[1];
+		visits[2] = pinned[2];
 	}
 	int64_t total = 0;
 	for (size_t i = 0; i < n; ++i) {
-		mFlows[i] = (int64_t)pinned[i];
+		mFlows[i] = (int64_t)pinned[3 + i];
 		total += mFlows[i];
 	}
 	return total;
@@ -997,35 +994,29 @@ void GridSolver::capture_round_graph(int phases, int max_iters, bool wide)
 	CGP_HIP(hipStreamBeginCapture(mStream, hipStreamCaptureModeThreadLocal));
 	try {
 		mCapturing = true;
-		// init (gated: it runs once per solve)
-		hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, mStream, mStatus.get(), (int64_t)1, 0,
-		                   (const SolveControl *)ctl, kGateInit);
-		const int64_t flow_bytes = 8 * (int64_t)n * kSlotWords;
-		hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(flow_bytes, 256)), dim3(256), 0, mStream,
-		                   reinterpret_cast<uint8_t *>(mFlow.get()), flow_bytes, (uint8_t)0, (const SolveControl *)ctl,
-		                   kGateInit);
-		hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(8 * 3 * kSlotWords, 256)), dim3(256), 0, mStream,
-		                   reinterpret_cast<uint8_t *>(mCounters.get()), (int64_t)(8 * 3 * kSlotWords), (uint8_t)0,
-		                   (const SolveControl *)ctl, kGateInit);
+		// init (gated: it runs once per solve; the status, flows and counters were cleared before the
+		// first replay)
 		launch_init(ctl);
 		hipLaunchKernelGGL(k_ctl_after_init, dim3(1), dim3(64), 0, mStream, ctl, (const int32_t *)mStatus.get(),
 		                   wide ? 1 : 0);
 		// global relabel: from scratch, or continued when the last replay left dirty tiles
 		hipLaunchKernelGGL(k_ctl_stage_begin, dim3(1), dim3(64), 0, mStream, ctl, kGateRelabelNew, sweep_sizes);
-		hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(mVox, 256, 8192)), dim3(256), 0, mStream, mH.get(), mVox,
-		                   kLabelInf, (const SolveControl *)ctl, kGateRelabelNew);
+		if (mConn != 6) {
+			hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(mVox, 256, 8192)), dim3(256), 0, mStream, mH.get(), mVox,
+			                   kLabelInf, (const SolveControl *)ctl, kGateRelabelNew);
+		}
 		hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(mTiles, 256)), dim3(256), 0, mStream, mDirtyA.get(),
 		                   (int64_t)mTiles, (uint8_t)0, (const SolveControl *)ctl, kGateRelabelNew);
 		for (int c = 0; c < 2; ++c) {
 			if (mSweep.size[c] > 0) {
 				launch_bfs_pass(mSweep.tiles.get() + mSweep.start[c], &ctl->first[c], mSweep.size[c], 0,
-				                mSweep.size[c] * tile_voxels);
+				                mSweep.size[c] * tile_voxels, mConn == 6 ? c + 1 : 0);
 			}
 		}
 		for (int cycle = 0; cycle < kGraphSweepCycles; ++cycle) {
 			for (int c = 0; c < 2; ++c) {
 				launch_compact(mSweep, mDirtyA.get(), c, true, nullptr, ctl, kGateRelabel);
-				launch_bfs_pass(mList.get(), mListCount, mSweep.max_size, 0, -1);
+				launch_bfs_pass(mList.get(), mListCount, mSweep.max_size, 0, -1, 0);
 			}
 		}
 		hipLaunchKernelGGL(k_ctl_relabel_end, dim3(1), dim3(kCtlThreads), 0, mStream, ctl,
@@ -1038,10 +1029,9 @@ void GridSolver::capture_round_graph(int phases, int max_iters, bool wide)
 		hipLaunchKernelGGL(k_ctl_discharge_end, dim3(1), dim3(64), 0, mStream, ctl, phases);
 		// termination test (or the final S_min closure after a relabel found nothing active)
 		hipLaunchKernelGGL(k_ctl_stage_begin, dim3(1), dim3(64), 0, mStream, ctl, kGateClosureNew, sweep_sizes);
-		hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(mVox, 256, 8192)), dim3(256), 0, mStream, mS.get(), mVox,
-		                   (uint8_t)0, (const SolveControl *)ctl, kGateClosureNew);
-		hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(mTiles, 256)), dim3(256), 0, mStream, mDirtyB.get(),
-		                   (int64_t)mTiles, (uint8_t)0, (const SolveControl *)ctl, kGateClosureNew);
+		hipLaunchKernelGGL(k_closure_reset, dim3(blocks_for(mVox / 16 + 1, 256, 4096)), dim3(256), 0, mStream, mS.get(),
+		                   mVox, (int32_t *)nullptr, mDirtyB.get(), (int64_t)mTiles, (const SolveControl *)ctl,
+		                   kGateClosureNew);
 		for (int c = 0; c < 2; ++c) {
 			if (mSweep.size[c] > 0) {
 				launch_closure_pass(mSweep.tiles.get() + mSweep.start[c], &ctl->first[c], mSweep.size[c], true,
@@ -1057,9 +1047,8 @@ void GridSolver::capture_round_graph(int phases, int max_iters, bool wide)
 		hipLaunchKernelGGL(k_ctl_closure_end, dim3(1), dim3(kCtlThreads), 0, mStream, ctl,
 		                   (const uint8_t *)mDirtyB.get(), mTiles);
 		// visit counters and per-grid flows, read back with the control block
-		hipLaunchKernelGGL(k_sum_slots, dim3(1), dim3(64), 0, mStream, mCounters.get(), (int64_t)3, totals);
-		hipLaunchKernelGGL(k_sum_slots, dim3(blocks_for((int64_t)n, 256)), dim3(256), 0, mStream, mFlow.get(),
-		                   (int64_t)n, totals + 3);
+		hipLaunchKernelGGL(k_sum_slots, dim3(blocks_for((int64_t)(3 + n), 256)), dim3(256), 0, mStream, mSlots.get(),
+		                   (int64_t)(3 + n), totals);
 		CGP_LAUNCH_CHECK();
 		mCapturing = false;
 		CGP_HIP(hipStreamEndCapture(mStream, &graph));
@@ -1095,8 +1084,6 @@ int64_t GridSolver::solve_graph(int phases, int max_iters, int max_rounds, cgp_s
 	    mGraph.iterations != max_iters || mGraph.wide != wide) {
 		capture_round_graph(phases, max_iters, wide);
 	}
-	hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(64), 0, mStream, control(), (int32_t)max_rounds);
-	CGP_LAUNCH_CHECK();
 	unsigned long long *pinned = reinterpret_cast<unsigned long long *>(mHostScratch.get());
 	const SolveControl *host_ctl = reinterpret_cast<const SolveControl *>(pinned);
 	int round_seen = -1, replays_in_round = 0;
@@ -1180,10 +1167,6 @@ int64_t GridSolver::solve(const cgp_policy &policy, cgp_stats *stats_out, const 
 		rec = KernelRecord();
 	}
 	configure_tiles(pick_shape(policy));
-	// the alternating counts and class cursors of the multi-workgroup compactions start at zero
-	hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, mStream, mCursors.get(), (int64_t)kCursorWords, 0,
-	                   (const SolveControl *)nullptr, 0u);
-	mOrderParity = mListParity = 0;
 	const int ncolours = mConn == 6 ? 2 : 8;
 	const TunedPolicy tp = tuned_policy(mConn, mInst.size() > 1, mVox / (int64_t)mInst.size());
 	const int phases = policy.discharges_per_relabel > 0 ? policy.discharges_per_relabel : tp.phases;
@@ -1208,8 +1191,19 @@ int64_t GridSolver::solve(const cgp_policy &policy, cgp_stats *stats_out, const 
 		fail(CGP_ERR_INVALID, "the graph engine serves 6-connected grids; 26-connected grids run on the tiles engine");
 	}
 	const bool use_graph = policy.engine == CGP_ENGINE_GRAPH;
+	// one launch clears the init status, the flow and visit-counter slots, the alternating counts of
+	// the multi-workgroup compactions and (device-driven solve) the control block
+	{
+		const int64_t words = (int64_t)mSlots.size();
+		hipLaunchKernelGGL(k_solve_reset, dim3(blocks_for(words, 256)), dim3(256), 0, mStream, mStatus.get(),
+		                   mSlots.get(), words, mCursors.get(), kCursorWords, use_graph ? control() : nullptr,
+		                   (int32_t)max_rounds);
+		CGP_LAUNCH_CHECK();
+		mOrderParity = mListParity = 0;
+	}
 	if (use_graph) {
 		mStarted = true;
+		mDirtyAClean = mDirtyBClean = false;  // the graph clears its flags itself
 		const int64_t flow = solve_graph(phases, max_iters, max_rounds, stats);
 		stats.flow = flow;
 		stats.total_ms = ms_since(t_start);
@@ -1226,17 +1220,6 @@ int64_t GridSolver::solve(const cgp_policy &policy, cgp_stats *stats_out, const 
 		return flow;
 	}
 
-	hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, mStream, mStatus.get(), (int64_t)1, 0,
-	                   (const SolveControl *)nullptr, 0u);
-	{
-		const int64_t flow_bytes = 8 * (int64_t)mInst.size() * kSlotWords;
-		hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(flow_bytes, 256)), dim3(256), 0, mStream,
-		                   reinterpret_cast<uint8_t *>(mFlow.get()), flow_bytes, (uint8_t)0,
-		                   (const SolveControl *)nullptr, 0u);
-		hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(8 * 3 * kSlotWords, 256)), dim3(256), 0, mStream,
-		                   reinterpret_cast<uint8_t *>(mCounters.get()), (int64_t)(8 * 3 * kSlotWords), (uint8_t)0,
-		                   (const SolveControl *)nullptr, 0u);
-	}
 	launch_init(nullptr);
 	mStarted = true;
 	if (trace.fn) {

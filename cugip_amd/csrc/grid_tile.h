@@ -95,12 +95,12 @@ struct GridView {
 	int32_t tbase;             // global id of this view's first tile
 };
 
-// The flow of an instance and the visit counters are sums over kSlots slots, each on its own
-// 128-byte line, picked by tile id: thousands of workgroups adding into one address serialise in
-// the L2 and hold up every other request queued behind them.
 // Work classes of the ordered discharge lists (heaviest first).
 constexpr int kWorkClasses = 8;
 
+// The flow of an instance and the visit counters are sums over kSlots slots, each on its own
+// 128-byte line, picked by tile id: thousands of workgroups adding into one address serialise in
+// the L2 and hold up every other request queued behind them.
 constexpr int kSlots = 32;
 constexpr int kSlotStride = 16;  // unsigned long longs per slot (128 bytes)
 constexpr int kSlotWords = kSlots * kSlotStride;  // per instance (flow) or per counter kind

@@ -120,7 +120,8 @@ private:
 	};
 	void launch_compact(const ColourSet &set, uint8_t *flags, int colour, bool clear = false,
 	                    const uint8_t *work = nullptr, const SolveControl *ctl = nullptr, unsigned gate = 0);
-	void launch_bfs_pass(const int32_t *list, const int32_t *count, int grid, int32_t threshold, int64_t voxels);
+	void launch_bfs_pass(const int32_t *list, const int32_t *count, int grid, int32_t threshold, int64_t voxels,
+	                     int first_mode);
 	void launch_closure_pass(const int32_t *list, const int32_t *count, int grid, bool first, int32_t *reached,
 	                         const SolveControl *ctl, int64_t voxels);
 	void launch_discharge(int max_iters, bool wide);
@@ -197,7 +198,8 @@ private:
 	DeviceBuffer<int32_t> mE, mRt, mH, mR;
 	DeviceBuffer<uint8_t> mS;
 	DeviceBuffer<uint32_t> mOmask;
-	DeviceBuffer<unsigned long long> mFlow, mCounters;  // slotted (kSlots per instance / per counter kind)
+	// slotted (kSlots per counter kind / per instance): views of mSlots, the 3 counters then the flows
+	DeviceBuffer<unsigned long long> mSlots, mFlow, mCounters;
 	DeviceBuffer<unsigned long long> mFlowTotal, mCounterTotal;
 	DeviceBuffer<unsigned long long> mCutSlots;  // cut capacity: kSlots slots and their sum
 	DeviceBuffer<int32_t> mStatus, mCount, mList, mAllTiles;
@@ -206,7 +208,9 @@ private:
 	// multi-workgroup compactions: two sets of class cursors of the ordered lists, then two list
 	// counts; each compaction uses one and clears the other (zeroed at the start of a solve)
 	DeviceBuffer<int32_t> mCursors;
-	static constexpr int kCursorWords = 2 * kWorkClasses + 2;
+	static constexpr int kCountAcc = 2 * kWorkClasses + 2;  // then 3 words of k_count_flags2's accumulators
+	static constexpr int kCursorWords = kCountAcc + 3;
+	bool mDirtyAClean = false, mDirtyBClean = false;  // every relabel / closure tile flag is known to be 0
 	int mOrderParity = 0, mListParity = 0;
 	int mClassStride = 0;                   // class region size of the ordered lists
 	const int32_t *mListCount = nullptr;    // count (or class cursors) of the last compacted list
