@@ -76,11 +76,8 @@ double ms_since(std::chrono::steady_clock::time_point t)
 
 GridSolver::~GridSolver()
 {
-	if (mEvStart) {
-		(void)hipEventDestroy(mEvStart);
-	}
-	if (mEvStop) {
-		(void)hipEventDestroy(mEvStop);
+	for (hipEvent_t ev : mEvents) {
+		(void)hipEventDestroy(ev);
 	}
 }
 
@@ -92,31 +89,40 @@ void GridSolver::launch(Kern kind, int64_t voxels, double bytes_per_voxel, F &&f
 		CGP_LAUNCH_CHECK();
 		return;
 	}
-	if (!mEvStart) {
-		CGP_HIP(hipEventCreate(&mEvStart));
-		CGP_HIP(hipEventCreate(&mEvStop));
+	const size_t ev = 2 * mPending.size();
+	while (mEvents.size() < ev + 2) {
+		hipEvent_t e = nullptr;
+		CGP_HIP(hipEventCreate(&e));
+		mEvents.push_back(e);
 	}
-	unsigned long long before = 0;
-	// tile-visit counter of the kernel kind: [0] discharge, [1] relabel, [2] closure
-	const int counter = kind == kBfs ? 1 : (kind == kClosure ? 2 : 0);
-	if (voxels < 0) {
-		before = read_counter(counter);
-	}
-	CGP_HIP(hipEventRecord(mEvStart, mStream));
+	CGP_HIP(hipEventRecord(mEvents[ev], mStream));
 	fn();
 	CGP_LAUNCH_CHECK();
-	CGP_HIP(hipEventRecord(mEvStop, mStream));
-	CGP_HIP(hipEventSynchronize(mEvStop));
-	float ms = 0.0f;
-	CGP_HIP(hipEventElapsedTime(&ms, mEvStart, mEvStop));
-	if (voxels < 0) {
-		voxels = (int64_t)(read_counter(counter) - before) * mShape.tx * mShape.ty * mShape.tz;
+	CGP_HIP(hipEventRecord(mEvents[ev + 1], mStream));
+	mPending.push_back(PendingTiming{kind, voxels, bytes_per_voxel, ev});
+}
+
+// Per-kernel times of a profiled solve, once its stream is idle; the tile passes' voxel visits
+// come from the visit counters.
+void GridSolver::finish_profile(const cgp_stats &stats)
+{
+	CGP_HIP(hipStreamSynchronize(mStream));
+	for (const PendingTiming &t : mPending) {
+		float ms = 0.0f;
+		CGP_HIP(hipEventElapsedTime(&ms, mEvents[t.event], mEvents[t.event + 1]));
+		KernelRecord &rec = mProf[t.kind];
+		rec.launches += 1;
+		rec.ms += ms;
+		rec.bytes_per_voxel = t.bytes_per_voxel;
+		if (t.voxels > 0) {
+			rec.voxels += t.voxels;
+		}
 	}
-	KernelRecord &rec = mProf[kind];
-	rec.launches += 1;
-	rec.voxels += voxels;
-	rec.ms += ms;
-	rec.bytes_per_voxel = bytes_per_voxel;
+	mPending.clear();
+	const int64_t tile_voxels = (int64_t)mShape.tx * mShape.ty * mShape.tz;
+	mProf[kDischarge].voxels = stats.discharge_tile_visits * tile_voxels;
+	mProf[kBfs].voxels = stats.bfs_tile_visits * tile_voxels;
+	mProf[kClosure].voxels = stats.cut_tile_visits * tile_voxels;
 }
 
 int GridSolver::profile(cgp_kernel_profile *out, int capacity) const
@@ -921,18 +927,6 @@ bool GridSolver::run_source_closure(cgp_stats &stats, bool check)
 	return hit;
 }
 
-// Sum of the slots of visit counter `kind` (0 discharge, 1 relabel, 2 closure).
-unsigned long long GridSolver::read_counter(int kind)
-{
-	unsigned long long *pinned = reinterpret_cast<unsigned long long *>(mHostScratch.get());
-	hipLaunchKernelGGL(k_sum_slots, dim3(1), dim3(64), 0, mStream, mCounters.get() + (int64_t)kind * kSlotWords,
-	                   (int64_t)1, mCounterTotal.get());
-	CGP_LAUNCH_CHECK();
-	CGP_HIP(hipMemcpyAsync(pinned, mCounterTotal.get(), sizeof(unsigned long long), hipMemcpyDeviceToHost, mStream));
-	CGP_HIP(hipStreamSynchronize(mStream));
-	return pinned[0];
-}
-
 // Flow of every grid (summed into mFlows) and, with visits set, the three visit counters, with one
 // host wait.
 int64_t GridSolver::read_flow(unsigned long long *visits)
@@ -1166,6 +1160,7 @@ int64_t GridSolver::solve(const cgp_policy &policy, cgp_stats *stats_out, const 
 	for (KernelRecord &rec : mProf) {
 		rec = KernelRecord();
 	}
+	mPending.clear();
 	configure_tiles(pick_shape(policy));
 	const int ncolours = mConn == 6 ? 2 : 8;
 	const TunedPolicy tp = tuned_policy(mConn, mInst.size() > 1, mVox / (int64_t)mInst.size());
@@ -1308,6 +1303,9 @@ int64_t GridSolver::solve(const cgp_policy &policy, cgp_stats *stats_out, const 
 	stats.tile_y = mShape.ty;
 	stats.tile_z = mShape.tz;
 	stats.conn = mConn;
+	if (mProfile) {
+		finish_profile(stats);
+	}
 	mSolved = true;
 	// labels are those of the last relabel when the solve stopped at the closure test; T_min and
 	// label readbacks relabel once on request

@@ -152,7 +152,6 @@ private:
 	void capture_round_graph(int phases, int max_iters, bool wide);
 	int64_t solve_graph(int phases, int max_iters, int max_rounds, cgp_stats &stats);
 	SolveControl *control() const { return reinterpret_cast<SolveControl *>(mCtl.get()); }
-	unsigned long long read_counter(int kind);
 	const InstanceDesc &instance(int index) const;
 	std::vector<InstanceDesc> describe(int conn, const GridSpec *specs, int count);
 	// borrow (single grid): read the caller's device capacity arrays in place instead of copies
@@ -229,7 +228,17 @@ private:
 	PinnedBuffer<int64_t> mHostScratch;
 	bool mProfile = false;
 	KernelRecord mProf[kKernCount];
-	hipEvent_t mEvStart = nullptr, mEvStop = nullptr;
+	// profiling: an event pair around every launch, read once the solve has ended (no host wait
+	// between launches, so the timings are those of the unprofiled solve's kernels)
+	struct PendingTiming {
+		Kern kind;
+		int64_t voxels;  // < 0: from the visit counters at the end
+		double bytes_per_voxel;
+		size_t event;
+	};
+	std::vector<hipEvent_t> mEvents;
+	std::vector<PendingTiming> mPending;
+	void finish_profile(const cgp_stats &stats);
 };
 
 // Scratch of the no-alloc grid solve, per device (CGP_NOALLOC_SCRATCH_BYTES in the C ABI).
