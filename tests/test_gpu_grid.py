@@ -166,6 +166,30 @@ def test_golden_fixtures(golden):
         assert digest(smin) == rec["smin_sha256"] and digest(tmin) == rec["tmin_sha256"]
 
 
+@pytest.mark.parametrize("conn,policy", [
+    (26, dict(tile_x=16, tile_y=4, tile_z=4)),
+    (26, dict(tile_x=16, tile_y=4, tile_z=4, discharges_per_relabel=5)),
+    (6, dict(engine=cugip_amd.ENGINE_GRAPH)),
+    (6, dict(tile_x=32, tile_y=2, tile_z=4, discharges_per_relabel=3, max_tile_iterations=4)),
+])
+def test_golden_multi_workgroup_lists(golden, conn, policy):
+    """The large golden instances (256^3 6C, 256x256x192 26C) with tile shapes of more than 32768
+    tiles, so every list goes through the multi-workgroup compactions (class regions for the
+    discharge, alternating counts, one-launch flag counts), also with an odd number of discharge
+    phases per round (the cursor sets then alternate across rounds) and on the device-driven
+    engine: flow, S_min and T_min equal the fixtures bit for bit."""
+    rec = [r for r in golden["instances"] if r["conn"] == conn and r["dims"][0] >= 256][0]
+    nx, ny, nz = rec["dims"]
+    say(f"golden multi-workgroup {rec['dims']} conn {conn} policy {policy}")
+    vol, cs, ct, cn = cugip_amd.make_instance(nx, ny, nz, rec["seed"], rec["instance"], conn)
+    flow, smin, tmin, stats = gpu_solve(cs, ct, cn, conn=conn, **policy)
+    tiles = -(-nx // stats["tile_x"]) * -(-ny // stats["tile_y"]) * -(-nz // stats["tile_z"])
+    say(f"  {tiles} tiles, {stats['rounds']} rounds, {stats['discharge_phases']} phases")
+    assert tiles > 32768
+    assert flow == rec["flow"]
+    assert digest(smin) == rec["smin_sha256"] and digest(tmin) == rec["tmin_sha256"]
+
+
 def test_full_size_properties():
     """256^3: the S_min and T_min cuts both have capacity equal to the flow, S_min and T_min are
     disjoint, and repeated solves on one handle agree bit for bit."""
