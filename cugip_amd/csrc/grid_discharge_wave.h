@@ -1,0 +1,354 @@
+// Discharge of a one-wave 6-connected tile (TX * TY == 64 lanes, lane = ly * TX + lx, each lane
+// the z-column of TZ voxels) with the whole tile state in registers: excess, residual sink link,
+// the 6 residuals and the label of every voxel live in the lane that owns it, x neighbours are
+// read across lanes with DPP wave shifts, y neighbours with a lane permute by TX, z neighbours are
+// the lane's own registers, and the halo labels (fixed while the tile runs: tiles of one colour
+// never touch) are loaded once into the lanes on the tile faces. No LDS, no barriers: a step of
+// the push sweep is a few dozen vector instructions instead of a chain of LDS round trips.
+//
+// The algorithm is the one of k_grid_discharge (the reference's relabel/push loop restricted to a
+// tile, include/cugip/advanced_operations/detail/graph_cut_push.hpp:96-186 for the push,
+// graph_cut_relabeling.hpp for the distance labels): repeat {region relabel = exact distances to
+// t inside the tile given the halo labels; top-down push over the levels that hold excess, each
+// voxel of level L pushing first into t, then along its admissible directions in order to level
+// L - 1} up to max_iters times, then write back what changed and send the flow that left the tile
+// to the neighbouring voxels. Included at the end of grid_kernels.h.
+#pragma once
+
+namespace cgp {
+
+// Value of lane - 1 (the -x neighbour) or, on the tile's -x face, `edge`.
+template <int kTX>
+__device__ inline int32_t lane_xm(int32_t v, int32_t edge, int lx)
+{
+	const int32_t s = __builtin_amdgcn_update_dpp(edge, v, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
+	return lx == 0 ? edge : s;
+}
+
+// Value of lane + 1 (the +x neighbour) or, on the +x face, `edge`.
+template <int kTX>
+__device__ inline int32_t lane_xp(int32_t v, int32_t edge, int lx)
+{
+	const int32_t s = __builtin_amdgcn_update_dpp(edge, v, 0x130 /* wave_shl:1 */, 0xf, 0xf, false);
+	return lx == kTX - 1 ? edge : s;
+}
+
+// Value of lane - TX (the -y neighbour) or, on the -y face, `edge`.
+template <int kTX>
+__device__ inline int32_t lane_ym(int32_t v, int32_t edge, int ly)
+{
+	const int32_t s = __shfl_up(v, (unsigned)kTX, 64);
+	return ly == 0 ? edge : s;
+}
+
+// Value of lane + TX (the +y neighbour) or, on the +y face, `edge`.
+template <int kTX, int kTY>
+__device__ inline int32_t lane_yp(int32_t v, int32_t edge, int ly)
+{
+	const int32_t s = __shfl_down(v, (unsigned)kTX, 64);
+	return ly == kTY - 1 ? edge : s;
+}
+
+template <int kTX, int kTY, int kTZ>
+__global__ __launch_bounds__(64, 4) void k_grid_discharge_wave(GridView gv, const int32_t *__restrict__ list,
+                                                            const int32_t *__restrict__ count, int class_stride,
+                                                            int max_iters, uint8_t *__restrict__ tile_active)
+{
+	static_assert(kTX * kTY == 64, "one wave per tile");
+	static_assert(kTZ <= 4, "the dirty bits of a column fit one 32-bit word");
+	using D = Dirs<6>;
+	constexpr int TZ = kTZ;
+	// direction bits: 0 -x, 1 -y, 2 -z, 3 +x, 4 +y, 5 +z
+	const int tile = discharge_tile(list, count, class_stride, (int)blockIdx.x);
+	if (tile < 0) {
+		return;
+	}
+	int local;
+	const GridView g = instance_view(gv, tile, local);
+	int tx, ty, tz;
+	tile_coords(g, local, tx, ty, tz);
+	const int ox = tx * kTX, oy = ty * kTY, oz = tz * kTZ;
+	const int lane = threadIdx.x;
+	const int lx = lane % kTX, ly = lane / kTX;
+	const int gx = ox + lx, gy = oy + ly;
+	const bool in_xy = gx < g.nx && gy < g.ny;
+	const uint32_t xy = slice_xy(g, gx, gy), sxy = (uint32_t)(g.nx * g.ny);
+
+	// ---- loads: own voxels, then the halo labels of the faces this lane lies on
+	uint32_t valid = 0;
+	int32_t ex[TZ], rt[TZ], r[6][TZ], hx[TZ], hy[TZ];
+	const int hxx = lx == 0 ? gx - 1 : gx + 1;  // the x face of this lane (only lanes on one use it)
+	const int hyy = ly == 0 ? gy - 1 : gy + 1;
+	const bool x_face = (lx == 0 || lx == kTX - 1) && hxx >= 0 && hxx < g.nx && gy < g.ny;
+	const bool y_face = (ly == 0 || ly == kTY - 1) && hyy >= 0 && hyy < g.ny && gx < g.nx;
+	const uint32_t hxxy = x_face ? slice_xy(g, hxx, gy) : 0u, hyxy = y_face ? slice_xy(g, gx, hyy) : 0u;
+#pragma unroll
+	for (int k = 0; k < TZ; ++k) {
+		const bool v = in_xy && oz + k < g.nz;
+		valid |= v ? (1u << k) : 0u;
+		ex[k] = v ? tile_at(g.e, g, oz, k, xy, sxy) : 0;
+		rt[k] = v ? tile_at(g.rt, g, oz, k, xy, sxy) : 0;
+#pragma unroll
+		for (int d = 0; d < 6; ++d) {
+			r[d][k] = v ? tile_at(g.r + (int64_t)d * g.nvox, g, oz, k, xy, sxy) : 0;
+		}
+		hx[k] = (x_face && oz + k < g.nz) ? tile_at(g.h, g, oz, k, hxxy, sxy) : kLabelInf;
+		hy[k] = (y_face && oz + k < g.nz) ? tile_at(g.h, g, oz, k, hyxy, sxy) : kLabelInf;
+	}
+	const int32_t hzlo = (in_xy && oz > 0) ? tile_at(g.h, g, oz, -1, xy, sxy) : kLabelInf;
+	const int32_t hzhi = (in_xy && oz + TZ < g.nz) ? tile_at(g.h, g, oz, TZ, xy, sxy) : kLabelInf;
+
+	// 6C: flow pushed across the tile faces during the visit (a lane touches at most one x face and
+	// one y face, and the two z faces at its column's ends)
+	int32_t out_x[TZ], out_y[TZ], out_zlo = 0, out_zhi = 0;
+	uint32_t dirty = 0;  // bits 8k + d: residual d of voxel k changed; bit 8k + 6: excess or rt changed
+#pragma unroll
+	for (int k = 0; k < TZ; ++k) {
+		out_x[k] = 0;
+		out_y[k] = 0;
+	}
+	long long sink = 0;
+	int32_t lab[TZ];  // the loaded labels, stored back as they are if no iteration runs
+#pragma unroll
+	for (int k = 0; k < TZ; ++k) {
+		lab[k] = ((valid >> k) & 1u) ? tile_at(g.h, g, oz, k, xy, sxy) : kLabelInf;
+	}
+
+	for (int it = 0; it < max_iters; ++it) {
+		// ---- region relabel: exact distances to t inside the tile, halo labels fixed
+		uint32_t open[TZ];
+#pragma unroll
+		for (int k = 0; k < TZ; ++k) {
+			uint32_t o = 0;
+#pragma unroll
+			for (int d = 0; d < 6; ++d) {
+				o |= r[d][k] > 0 ? (1u << d) : 0u;
+			}
+			open[k] = o;
+			lab[k] = ((valid >> k) & 1u) && rt[k] > 0 ? 1 : kLabelInf;
+		}
+		for (;;) {
+			bool changed = false;
+			// x and y neighbours from the labels before this round (every lane takes part in the
+			// cross-lane reads), then the column up and down in registers
+#pragma unroll
+			for (int k = 0; k < TZ; ++k) {
+				const int32_t xm = lane_xm<kTX>(lab[k], hx[k], lx), xp = lane_xp<kTX>(lab[k], hx[k], lx);
+				const int32_t ym = lane_ym<kTX>(lab[k], hy[k], ly), yp = lane_yp<kTX, kTY>(lab[k], hy[k], ly);
+				int32_t best = lab[k];
+				best = (open[k] & 1u) && xm + 1 < best ? xm + 1 : best;
+				best = (open[k] & 8u) && xp + 1 < best ? xp + 1 : best;
+				best = (open[k] & 2u) && ym + 1 < best ? ym + 1 : best;
+				best = (open[k] & 16u) && yp + 1 < best ? yp + 1 : best;
+				if (((valid >> k) & 1u) && best < lab[k]) {
+					lab[k] = best;
+					changed = true;
+				}
+			}
+#pragma unroll
+			for (int k = 0; k < TZ; ++k) {
+				const int32_t below = (k == 0 ? hzlo : lab[k > 0 ? k - 1 : 0]) + 1;
+				if (((valid >> k) & 1u) && (open[k] & 4u) && below < lab[k]) {
+					lab[k] = below;
+					changed = true;
+				}
+			}
+#pragma unroll
+			for (int k = TZ - 1; k >= 0; --k) {
+				const int32_t above = (k == TZ - 1 ? hzhi : lab[k < TZ - 1 ? k + 1 : 0]) + 1;
+				if (((valid >> k) & 1u) && (open[k] & 32u) && above < lab[k]) {
+					lab[k] = above;
+					changed = true;
+				}
+			}
+			if (!__any(changed)) {
+				break;
+			}
+		}
+
+		// ---- admissible directions (open, neighbour one level lower), the voxels holding excess
+		// and the top level holding excess
+		uint32_t adm[TZ];
+		uint32_t held = 0;
+		int top = 0;
+#pragma unroll
+		for (int k = 0; k < TZ; ++k) {
+			const int32_t l = lab[k];
+			const int32_t xm = lane_xm<kTX>(l, hx[k], lx), xp = lane_xp<kTX>(l, hx[k], lx);
+			const int32_t ym = lane_ym<kTX>(l, hy[k], ly), yp = lane_yp<kTX, kTY>(l, hy[k], ly);
+			const int32_t zm = k == 0 ? hzlo : lab[k > 0 ? k - 1 : 0];
+			const int32_t zp = k == TZ - 1 ? hzhi : lab[k < TZ - 1 ? k + 1 : 0];
+			uint32_t a = 0;
+			a |= xm == l - 1 ? 1u : 0u;
+			a |= ym == l - 1 ? 2u : 0u;
+			a |= zm == l - 1 ? 4u : 0u;
+			a |= xp == l - 1 ? 8u : 0u;
+			a |= yp == l - 1 ? 16u : 0u;
+			a |= zp == l - 1 ? 32u : 0u;
+			adm[k] = a & open[k];
+			const bool h = ((valid >> k) & 1u) && ex[k] > 0 && l < kLabelInf;
+			held |= h ? (1u << k) : 0u;
+			top = h && l > top ? l : top;
+		}
+		int level = wave_max_i32(top);
+		if (level == 0) {
+			break;
+		}
+		// ---- top-down push sweep over the levels that hold excess: a voxel of level L pushes only
+		// to voxels of level L - 1 (exact labels), so the senders of a step never receive in it;
+		// the next level is L - 1 if this step pushed inside the tile, else the highest lower level
+		// whose voxels held excess when the sweep began
+		while (level >= 1) {
+			int next = 0;
+			bool inside = false;
+#pragma unroll
+			for (int k = 0; k < TZ; ++k) {
+				const bool send = ((valid >> k) & 1u) && lab[k] == level && ex[k] > 0;
+				if (!send && lab[k] < level && lab[k] > next && ((held >> k) & 1u)) {
+					next = lab[k];
+				}
+				if (!__any(send)) {
+					continue;  // no voxel of this z-level sends: nothing to deliver either
+				}
+				int32_t e = ex[k];
+				const int32_t fs = send ? (e < rt[k] ? e : rt[k]) : 0;
+				rt[k] -= fs;
+				e -= fs;
+				sink += fs;
+				int32_t f[6];
+#pragma unroll
+				for (int d = 0; d < 6; ++d) {
+					const int32_t fd = (send && ((adm[k] >> d) & 1u)) ? (e < r[d][k] ? e : r[d][k]) : 0;
+					r[d][k] -= fd;
+					e -= fd;
+					f[d] = fd;
+					dirty |= fd > 0 ? (1u << (8 * k + d)) : 0u;
+				}
+				dirty |= e != ex[k] ? (1u << (8 * k + 6)) : 0u;
+				ex[k] = e;
+				// deliveries: the receiver's excess and its residual back towards the sender
+				const int32_t from_xm = lane_xm<kTX>(f[3], 0, lx);  // sent +x by lane - 1
+				const int32_t from_xp = lane_xp<kTX>(f[0], 0, lx);  // sent -x by lane + 1
+				const int32_t from_ym = lane_ym<kTX>(f[4], 0, ly);
+				const int32_t from_yp = lane_yp<kTX, kTY>(f[1], 0, ly);
+				const int32_t got = from_xm + from_xp + from_ym + from_yp;
+				ex[k] += got;
+				r[0][k] += from_xm;
+				r[3][k] += from_xp;
+				r[1][k] += from_ym;
+				r[4][k] += from_yp;
+				dirty |= (from_xm > 0 ? (1u << (8 * k + 0)) : 0u) | (from_xp > 0 ? (1u << (8 * k + 3)) : 0u) |
+				         (from_ym > 0 ? (1u << (8 * k + 1)) : 0u) | (from_yp > 0 ? (1u << (8 * k + 4)) : 0u) |
+				         (got > 0 ? (1u << (8 * k + 6)) : 0u);
+				// flow across the tile faces stays in registers until the write-back
+				out_x[k] += (lx == 0 ? f[0] : 0) + (lx == kTX - 1 ? f[3] : 0);
+				out_y[k] += (ly == 0 ? f[1] : 0) + (ly == kTY - 1 ? f[4] : 0);
+				inside |= (lx > 0 && f[0] > 0) || (lx < kTX - 1 && f[3] > 0) || (ly > 0 && f[1] > 0) ||
+				          (ly < kTY - 1 && f[4] > 0);
+				// z: the column's own next voxels, or the z faces
+				if (k > 0) {
+					ex[k > 0 ? k - 1 : 0] += f[2];
+					r[5][k > 0 ? k - 1 : 0] += f[2];
+					dirty |= f[2] > 0 ? (1u << (8 * (k - 1) + 5)) | (1u << (8 * (k - 1) + 6)) : 0u;
+					inside |= f[2] > 0;
+				} else {
+					out_zlo += f[2];
+				}
+				if (k < TZ - 1) {
+					ex[k < TZ - 1 ? k + 1 : 0] += f[5];
+					r[2][k < TZ - 1 ? k + 1 : 0] += f[5];
+					dirty |= f[5] > 0 ? (1u << (8 * (k + 1) + 2)) | (1u << (8 * (k + 1) + 6)) : 0u;
+					inside |= f[5] > 0;
+				} else {
+					out_zhi += f[5];
+				}
+			}
+			if (inside) {
+				next = level - 1 > next ? level - 1 : next;
+			}
+			level = wave_max_i32(next);
+		}
+		// when the sweep left no excess in the tile, another region relabel would only find that
+		// out; the labels stay valid (pushes along admissible edges keep them valid)
+		bool left = false;
+#pragma unroll
+		for (int k = 0; k < TZ; ++k) {
+			left |= ((valid >> k) & 1u) && ex[k] > 0;
+		}
+		if (!__any(left)) {
+			break;
+		}
+	}
+
+	// ---- write back what changed; pushes that left the tile go to the neighbouring voxels
+	bool active = false;
+	uint32_t marks = 0;  // bit s: neighbour tile slot s (dz, dy, dx in {-1, 0, 1}) received flow
+#pragma unroll
+	for (int k = 0; k < TZ; ++k) {
+		if (!((valid >> k) & 1u)) {
+			continue;
+		}
+		active |= ex[k] > 0 && lab[k] < kLabelInf;
+		tile_at(g.h, g, oz, k, xy, sxy) = lab[k];
+		const uint32_t dk = (dirty >> (8 * k)) & 0x7fu;
+		if (!dk) {
+			continue;
+		}
+		tile_at(g.e, g, oz, k, xy, sxy) = ex[k];
+		tile_at(g.rt, g, oz, k, xy, sxy) = rt[k];
+		uint32_t st = (ex[k] > 0 ? kStateExcess : 0u) | (rt[k] > 0 ? kStateSink : 0u);
+#pragma unroll
+		for (int d = 0; d < 6; ++d) {
+			st |= r[d][k] > 0 ? (1u << d) : 0u;
+		}
+		tile_at(g.omask, g, oz, k, xy, sxy) = st;
+#pragma unroll
+		for (int d = 0; d < 6; ++d) {
+			if ((dk >> d) & 1u) {
+				tile_at(g.r + (int64_t)d * g.nvox, g, oz, k, xy, sxy) = r[d][k];
+			}
+		}
+	}
+	// flow that crossed the faces goes to the neighbouring voxels (atomics without return)
+	auto send_out = [&](int k, int d, int32_t f) {
+		if (f <= 0) {
+			return;
+		}
+		const int uk = k + D::dz(d);
+		const uint32_t uxy = slice_xy(g, gx + D::dx(d), gy + D::dy(d));
+		atomicAdd(&tile_at(g.e, g, oz, uk, uxy, sxy), f);
+		atomicAdd(&tile_at(g.r + (int64_t)opposite<6>(d) * g.nvox, g, oz, uk, uxy, sxy), f);
+		atomicOr(&tile_at(g.omask, g, oz, uk, uxy, sxy), kStateExcess | (1u << opposite<6>(d)));
+		const int sx = D::dx(d) + 1, sy = D::dy(d) + 1, sz = (k + D::dz(d) < 0 ? 0 : (k + D::dz(d) >= TZ ? 2 : 1));
+		marks |= 1u << ((sz * 3 + sy) * 3 + sx);
+	};
+#pragma unroll
+	for (int k = 0; k < TZ; ++k) {
+		send_out(k, lx == 0 ? 0 : 3, out_x[k]);
+		send_out(k, ly == 0 ? 1 : 4, out_y[k]);
+	}
+	send_out(0, 2, out_zlo);
+	send_out(TZ - 1, 5, out_zhi);
+	// (a lane on both x faces or both y faces would need two accumulators; TX, TY >= 2 here)
+	static_assert(kTX >= 2 && kTY >= 2, "a lane lies on at most one x face and one y face");
+
+	// wave reductions: the flow into t, the activity, the neighbour tiles to mark
+#pragma unroll
+	for (int off = 32; off > 0; off >>= 1) {
+		sink += __shfl_xor(sink, off, 64);
+		marks |= (uint32_t)__shfl_xor((int)marks, off, 64);
+	}
+	const bool any_active = __any(active);
+	if (lane == 0) {
+		if (sink) {
+			atomicAdd(flow_slot(g, tile), (unsigned long long)sink);
+		}
+		tile_active[tile] = any_active ? 1 : 0;
+		atomicAdd(counter_slot(g, 0, tile), 1ull);
+	}
+	if (lane < 27 && ((marks >> lane) & 1u)) {
+		mark_neighbour_tile(g, tile_active, tx, ty, tz, lane);
+	}
+}
+
+}  // namespace cgp

@@ -2017,3 +2017,5 @@ __global__ __launch_bounds__(256) void k_sink_side_mask(const int32_t *__restric
 }
 
 }  // namespace cgp
+
+#include "grid_discharge_wave.h"

@@ -841,11 +841,18 @@ void GridSolver::launch_discharge(int max_iters, bool wide)
 {
 	GridView g = CGP_VIEW();
 	const int blocks = mPhase.max_size;
-	// reads and writes e, rt, h and r[K], writes the state word; residuals staged in LDS as int16
-	// unless some edge's capacity pair exceeds 32767
+	// reads and writes e, rt, h and r[K], writes the state word; one-wave 6C tiles keep the tile in
+	// registers, the others stage it in LDS with int16 residuals unless some edge's capacity pair
+	// exceeds 32767
 	launch(kDischarge, -1, 8.0 * (mConn + 3) + 4.0, [&] {
 		dispatch_shape(mConn, mShape.tx, mShape.ty, mShape.tz, [&](auto tag) {
 			using S = decltype(tag);
+			if constexpr (S::conn == 6 && S::tx * S::ty == 64 && S::tz <= 4) {
+				// one-wave 6C tiles: the whole tile state in registers (int32 residuals, any width)
+				hipLaunchKernelGGL((k_grid_discharge_wave<S::tx, S::ty, S::tz>), dim3(blocks), dim3(64), 0, mStream, g,
+				                   mList.get(), mListCount, mDischargeStride, max_iters, mActive.get());
+				return;
+			}
 			if (wide) {
 				hipLaunchKernelGGL((k_grid_discharge<S::conn, S::tx, S::ty, S::tz, int32_t>), dim3(blocks),
 				                   dim3(S::tx * S::ty), 0, mStream, g, mList.get(), mListCount, mDischargeStride,
