@@ -20,8 +20,8 @@ import cugip_amd
 
 SHAPES = {6: [(32, 8, 4), (32, 8, 8), (64, 4, 4), (32, 4, 4), (64, 4, 8), (16, 8, 4), (16, 4, 4), (32, 2, 4)],
           26: [(32, 4, 4), (16, 8, 4), (16, 4, 4)]}
-PHASES = {6: [2, 4, 8, 12, 16], 26: [8, 16, 24]}
-ITERATIONS = [1, 2, 4, 16]
+PHASES = {6: [2, 4, 6, 8, 12, 16], 26: [8, 16, 24]}
+ITERATIONS = [1, 2, 4, 8, 16, 32]
 
 # name: (conn, batch, max_voxels of the table row, builder of the sample grids). Single-grid
 # workloads are timed as the sum over their sample grids (one solver each), so that the winner is
@@ -102,7 +102,22 @@ def tune_workload(name, repeats=3, log=print):
                 best=best, candidates=results)
 
 
+def _row_class(line):
+    """(conn, batch, bound) of a table row of the header."""
+    f = [x.strip() for x in line.strip().lstrip("{").split("}")[0].split(",")]
+    return int(f[0]), int(f[1]), f[2]
+
+
 def header_text(tuned):
+    """The header with one row per tuned workload; rows of the classes (connectivity, batch flag,
+    size bound) not tuned in this run are kept as they are."""
+    src = open(os.path.join(os.path.dirname(__file__), "csrc", "tuned_policy.h")).read()
+    start = src.index("static const TunedPolicy kTunedPolicies[] = {")
+    end = src.index("};", start)
+    classes = {(e["conn"], 1 if e["batch"] else 0, "INT64_MAX" if e["max_voxels"] is None else "%dLL" % e["max_voxels"])
+               for e in tuned}
+    kept = [line for line in src[start:end].splitlines()
+            if line.strip().startswith("{") and _row_class(line) not in classes]
     rows = []
     # bounded rows first, so that they match before the unbounded row of the same class
     for entry in sorted(tuned, key=lambda e: (e["conn"], e["batch"], e["max_voxels"] is None,
@@ -112,9 +127,11 @@ def header_text(tuned):
         rows.append("\t{%d, %d, %s, %d, %d, %d, %d, %d},  // %s: %.2f ms" % (
             entry["conn"], 1 if entry["batch"] else 0, mv, b["tile"][0], b["tile"][1], b["tile"][2], b["phases"],
             b["iterations"], entry["workload"], b["ms"]))
-    src = open(os.path.join(os.path.dirname(__file__), "csrc", "tuned_policy.h")).read()
-    start = src.index("static const TunedPolicy kTunedPolicies[] = {")
-    end = src.index("};", start)
+    # keep the table ordered by connectivity, batch flag and bound, whichever run wrote a row
+    def key(line):
+        conn, batch, bound = _row_class(line)
+        return conn, batch, bound == "INT64_MAX", 0 if bound == "INT64_MAX" else int(bound.rstrip("L"))
+    rows = sorted(rows + kept, key=key)
     head = src[:start] + "static const TunedPolicy kTunedPolicies[] = {\n" \
         "\t// conn batch max_voxels  tile        phases iters\n"
     return head + "\n".join(rows) + "\n" + src[end:]
