@@ -2018,4 +2018,4 @@ __global__ __launch_bounds__(256) void k_sink_side_mask(const int32_t *__restric
 
 }  // namespace cgp
 
-#include "grid_discharge_wave.h"
+#include "grid_wave_tiles.h"

@@ -812,6 +812,12 @@ void GridSolver::launch_bfs_pass(const int32_t *list, const int32_t *count, int 
 	launch(kBfs, voxels, bytes, [&] {
 		dispatch_shape(mConn, mShape.tx, mShape.ty, mShape.tz, [&](auto tag) {
 			using S = decltype(tag);
+			if constexpr (S::conn == 6 && S::tx * S::ty == 64) {
+				// one-wave 6C tiles: the tile in registers
+				hipLaunchKernelGGL((k_grid_bfs_wave<S::tx, S::ty, S::tz>), dim3(grid), dim3(64), 0, mStream, g, list,
+				                   count, threshold, first_mode, mDirtyA.get(), mActive.get(), mTileWork.get());
+				return;
+			}
 			hipLaunchKernelGGL((k_grid_bfs<S::conn, S::tx, S::ty, S::tz>), dim3(grid), dim3(S::tx * S::ty), 0,
 			                   mStream, g, list, count, threshold, first_mode, mDirtyA.get(), mActive.get(),
 			                   mTileWork.get());

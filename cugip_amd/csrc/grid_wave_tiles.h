@@ -1,5 +1,7 @@
-// Discharge of a one-wave 6-connected tile (TX * TY == 64 lanes, lane = ly * TX + lx, each lane
-// the z-column of TZ voxels) with the whole tile state in registers: excess, residual sink link,
+// Tile kernels for one-wave 6-connected tiles (TX * TY == 64 lanes, lane = ly * TX + lx, each lane
+// the z-column of TZ voxels) with the tile in registers: the discharge and the global-relabel pass.
+//
+// Discharge with the whole tile state in registers: excess, residual sink link,
 // the 6 residuals and the label of every voxel live in the lane that owns it, x neighbours are
 // read across lanes with DPP wave shifts, y neighbours with a lane permute by TX, z neighbours are
 // the lane's own registers, and the halo labels (fixed while the tile runs: tiles of one colour
@@ -47,6 +49,52 @@ __device__ inline int32_t lane_yp(int32_t v, int32_t edge, int ly)
 {
 	const int32_t s = __shfl_down(v, (unsigned)kTX, 64);
 	return ly == kTY - 1 ? edge : s;
+}
+
+// Region fixpoint h = min(h, 1 + h(nbr)) over the open directions of the lane's column, x and y
+// neighbours across lanes (Jacobi rounds), the column swept up and down in registers; halo labels
+// fixed. Shared by the relabel pass and the discharge's region relabel.
+template <int kTX, int kTY, int kTZ>
+__device__ inline void wave_relax(int32_t (&lab)[kTZ], const uint32_t (&open)[kTZ], uint32_t valid,
+                                  const int32_t (&hx)[kTZ], const int32_t (&hy)[kTZ], int32_t hzlo, int32_t hzhi,
+                                  int lx, int ly)
+{
+	for (;;) {
+		bool changed = false;
+#pragma unroll
+		for (int k = 0; k < kTZ; ++k) {
+			const int32_t xm = lane_xm<kTX>(lab[k], hx[k], lx), xp = lane_xp<kTX>(lab[k], hx[k], lx);
+			const int32_t ym = lane_ym<kTX>(lab[k], hy[k], ly), yp = lane_yp<kTX, kTY>(lab[k], hy[k], ly);
+			int32_t best = lab[k];
+			best = (open[k] & 1u) && xm + 1 < best ? xm + 1 : best;
+			best = (open[k] & 8u) && xp + 1 < best ? xp + 1 : best;
+			best = (open[k] & 2u) && ym + 1 < best ? ym + 1 : best;
+			best = (open[k] & 16u) && yp + 1 < best ? yp + 1 : best;
+			if (((valid >> k) & 1u) && best < lab[k]) {
+				lab[k] = best;
+				changed = true;
+			}
+		}
+#pragma unroll
+		for (int k = 0; k < kTZ; ++k) {
+			const int32_t below = (k == 0 ? hzlo : lab[k > 0 ? k - 1 : 0]) + 1;
+			if (((valid >> k) & 1u) && (open[k] & 4u) && below < lab[k]) {
+				lab[k] = below;
+				changed = true;
+			}
+		}
+#pragma unroll
+		for (int k = kTZ - 1; k >= 0; --k) {
+			const int32_t above = (k == kTZ - 1 ? hzhi : lab[k < kTZ - 1 ? k + 1 : 0]) + 1;
+			if (((valid >> k) & 1u) && (open[k] & 32u) && above < lab[k]) {
+				lab[k] = above;
+				changed = true;
+			}
+		}
+		if (!__any(changed)) {
+			return;
+		}
+	}
 }
 
 template <int kTX, int kTY, int kTZ>
@@ -127,44 +175,7 @@ __global__ __launch_bounds__(64, 4) void k_grid_discharge_wave(GridView gv, cons
 			open[k] = o;
 			lab[k] = ((valid >> k) & 1u) && rt[k] > 0 ? 1 : kLabelInf;
 		}
-		for (;;) {
-			bool changed = false;
-			// x and y neighbours from the labels before this round (every lane takes part in the
-			// cross-lane reads), then the column up and down in registers
-#pragma unroll
-			for (int k = 0; k < TZ; ++k) {
-				const int32_t xm = lane_xm<kTX>(lab[k], hx[k], lx), xp = lane_xp<kTX>(lab[k], hx[k], lx);
-				const int32_t ym = lane_ym<kTX>(lab[k], hy[k], ly), yp = lane_yp<kTX, kTY>(lab[k], hy[k], ly);
-				int32_t best = lab[k];
-				best = (open[k] & 1u) && xm + 1 < best ? xm + 1 : best;
-				best = (open[k] & 8u) && xp + 1 < best ? xp + 1 : best;
-				best = (open[k] & 2u) && ym + 1 < best ? ym + 1 : best;
-				best = (open[k] & 16u) && yp + 1 < best ? yp + 1 : best;
-				if (((valid >> k) & 1u) && best < lab[k]) {
-					lab[k] = best;
-					changed = true;
-				}
-			}
-#pragma unroll
-			for (int k = 0; k < TZ; ++k) {
-				const int32_t below = (k == 0 ? hzlo : lab[k > 0 ? k - 1 : 0]) + 1;
-				if (((valid >> k) & 1u) && (open[k] & 4u) && below < lab[k]) {
-					lab[k] = below;
-					changed = true;
-				}
-			}
-#pragma unroll
-			for (int k = TZ - 1; k >= 0; --k) {
-				const int32_t above = (k == TZ - 1 ? hzhi : lab[k < TZ - 1 ? k + 1 : 0]) + 1;
-				if (((valid >> k) & 1u) && (open[k] & 32u) && above < lab[k]) {
-					lab[k] = above;
-					changed = true;
-				}
-			}
-			if (!__any(changed)) {
-				break;
-			}
-		}
+		wave_relax<kTX, kTY, kTZ>(lab, open, valid, hx, hy, hzlo, hzhi, lx, ly);
 
 		// ---- admissible directions (open, neighbour one level lower), the voxels holding excess
 		// and the top level holding excess
@@ -348,6 +359,128 @@ __global__ __launch_bounds__(64, 4) void k_grid_discharge_wave(GridView gv, cons
 	}
 	if (lane < 27 && ((marks >> lane) & 1u)) {
 		mark_neighbour_tile(g, tile_active, tx, ty, tz, lane);
+	}
+}
+
+// Global-relabel pass over a list of one-wave 6C tiles, the semantics of k_grid_bfs (labels only
+// fall; a tile whose boundary label fell below what a neighbouring voxel could use marks that
+// neighbour's tile in dirty_next; the tile's active flag and discharge work estimate are
+// refreshed; first_mode 1 / 2 as there) with the tile in registers: own labels and open directions
+// in the lanes, the face lanes holding the halo labels.
+template <int kTX, int kTY, int kTZ>
+__global__ __launch_bounds__(64) void k_grid_bfs_wave(GridView gv, const int32_t *__restrict__ list,
+                                                      const int32_t *__restrict__ count, int32_t threshold,
+                                                      int first_mode, uint8_t *__restrict__ dirty_next,
+                                                      uint8_t *__restrict__ tile_active,
+                                                      uint8_t *__restrict__ tile_work)
+{
+	static_assert(kTX * kTY == 64 && kTX >= 2 && kTY >= 2, "one wave per tile");
+	constexpr int TZ = kTZ;
+	const int tile = list[blockIdx.x];  // in bounds: the grid never exceeds the list's capacity
+	if ((int)blockIdx.x >= *count) {
+		return;
+	}
+	int local;
+	const GridView g = instance_view(gv, tile, local);
+	int tx, ty, tz;
+	tile_coords(g, local, tx, ty, tz);
+	const int ox = tx * kTX, oy = ty * kTY, oz = tz * kTZ;
+	const int lane = threadIdx.x;
+	const int lx = lane % kTX, ly = lane / kTX;
+	const int gx = ox + lx, gy = oy + ly;
+	const bool in_xy = gx < g.nx && gy < g.ny;
+	const uint32_t xy = slice_xy(g, gx, gy), sxy = (uint32_t)(g.nx * g.ny);
+	const int hxx = lx == 0 ? gx - 1 : gx + 1;
+	const int hyy = ly == 0 ? gy - 1 : gy + 1;
+	// halo labels are read unless this is the first colour's first pass (first_mode 1)
+	const bool x_face = first_mode != 1 && (lx == 0 || lx == kTX - 1) && hxx >= 0 && hxx < g.nx && gy < g.ny;
+	const bool y_face = first_mode != 1 && (ly == 0 || ly == kTY - 1) && hyy >= 0 && hyy < g.ny && gx < g.nx;
+	const uint32_t hxxy = x_face ? slice_xy(g, hxx, gy) : 0u, hyxy = y_face ? slice_xy(g, gx, hyy) : 0u;
+	uint32_t valid = 0, excess = 0;
+	uint32_t open[TZ];
+	int32_t lab[TZ], before[TZ], hx[TZ], hy[TZ];
+#pragma unroll
+	for (int k = 0; k < TZ; ++k) {
+		const bool v = in_xy && oz + k < g.nz;
+		valid |= v ? (1u << k) : 0u;
+		uint32_t o = 0;
+		bool sink_open = false, ex = false;
+		if (v) {
+			if (threshold == 0) {
+				const uint32_t st = tile_at(g.omask, g, oz, k, xy, sxy);
+				o = st & 63u;
+				ex = (st & kStateExcess) != 0;
+				sink_open = (st & kStateSink) != 0;
+			} else {
+#pragma unroll
+				for (int d = 0; d < 6; ++d) {
+					o |= tile_at(g.r + (int64_t)d * g.nvox, g, oz, k, xy, sxy) > threshold ? (1u << d) : 0u;
+				}
+				ex = tile_at(g.e, g, oz, k, xy, sxy) > 0;
+				sink_open = tile_at(g.rt, g, oz, k, xy, sxy) > threshold;
+			}
+		}
+		open[k] = o;
+		excess |= ex ? (1u << k) : 0u;
+		before[k] = (v && first_mode == 0) ? tile_at(g.h, g, oz, k, xy, sxy) : kLabelInf;
+		lab[k] = sink_open && before[k] > 1 ? 1 : before[k];
+		hx[k] = (x_face && oz + k < g.nz) ? tile_at(g.h, g, oz, k, hxxy, sxy) : kLabelInf;
+		hy[k] = (y_face && oz + k < g.nz) ? tile_at(g.h, g, oz, k, hyxy, sxy) : kLabelInf;
+	}
+	const int32_t hzlo = (first_mode != 1 && in_xy && oz > 0) ? tile_at(g.h, g, oz, -1, xy, sxy) : kLabelInf;
+	const int32_t hzhi = (first_mode != 1 && in_xy && oz + TZ < g.nz) ? tile_at(g.h, g, oz, TZ, xy, sxy) : kLabelInf;
+
+	wave_relax<kTX, kTY, kTZ>(lab, open, valid, hx, hy, hzlo, hzhi, lx, ly);
+
+	bool active = false;
+	int top = 0, bottom = kLabelInf;
+	uint32_t marks = 0;  // bit s: neighbour tile slot s whose voxel could improve
+#pragma unroll
+	for (int k = 0; k < TZ; ++k) {
+		if (!((valid >> k) & 1u)) {
+			continue;
+		}
+		const int32_t nl = lab[k];
+		const bool h = ((excess >> k) & 1u) && nl < kLabelInf;
+		active |= h;
+		top = h && nl > top ? nl : top;
+		bottom = nl < bottom ? nl : bottom;
+		if (first_mode || nl != before[k]) {
+			tile_at(g.h, g, oz, k, xy, sxy) = nl;  // first passes store every label
+		}
+		if (nl != before[k]) {
+			// a voxel beyond the tile pulls this label; it must be relaxed again if it could improve
+			if (x_face && hx[k] > nl + 1) {
+				marks |= 1u << ((1 * 3 + 1) * 3 + (lx == 0 ? 0 : 2));
+			}
+			if (y_face && hy[k] > nl + 1) {
+				marks |= 1u << ((1 * 3 + (ly == 0 ? 0 : 2)) * 3 + 1);
+			}
+			if (k == 0 && hzlo > nl + 1 && oz > 0) {
+				marks |= 1u << ((0 * 3 + 1) * 3 + 1);
+			}
+			if (k == TZ - 1 && hzhi > nl + 1 && oz + TZ < g.nz) {
+				marks |= 1u << ((2 * 3 + 1) * 3 + 1);
+			}
+		}
+	}
+#pragma unroll
+	for (int off = 32; off > 0; off >>= 1) {
+		marks |= (uint32_t)__shfl_xor((int)marks, off, 64);
+	}
+	top = wave_max_i32(top);
+	bottom = -wave_max_i32(-bottom);
+	const bool any_active = __any(active);
+	if (lane < 27 && ((marks >> lane) & 1u)) {
+		mark_neighbour_tile(g, dirty_next, tx, ty, tz, lane);
+	}
+	if (lane == 0) {
+		tile_active[tile] = any_active ? 1 : 0;
+		// discharge work estimate: the levels between the tile's lowest label and its highest
+		// label holding excess (the discharge runs the heaviest tiles first)
+		const int levels = any_active ? top - bottom + 1 : 0;
+		tile_work[tile] = (uint8_t)(levels < 255 ? levels : 255);
+		atomicAdd(counter_slot(g, 1, tile), 1ull);
 	}
 }
 
