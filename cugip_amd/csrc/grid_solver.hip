@@ -835,6 +835,12 @@ void GridSolver::launch_closure_pass(const int32_t *list, const int32_t *count, 
 	launch(kClosure, voxels, 6.0, [&] {
 		dispatch_shape(mConn, mShape.tx, mShape.ty, mShape.tz, [&](auto tag) {
 			using S = decltype(tag);
+			if constexpr (S::conn == 6 && S::tx * S::ty == 64) {
+				// one-wave 6C tiles: the tile in registers
+				hipLaunchKernelGGL((k_grid_closure_wave<S::tx, S::ty, S::tz>), dim3(grid), dim3(64), 0, mStream, g, list,
+				                   count, first ? 1 : 0, mDirtyB.get(), reached, ctl);
+				return;
+			}
 			hipLaunchKernelGGL((k_grid_source_closure<S::conn, S::tx, S::ty, S::tz>), dim3(grid), dim3(S::tx * S::ty),
 			                   0, mStream, g, list, count, first ? 1 : 0, mDirtyB.get(), reached, ctl);
 		});

@@ -484,4 +484,127 @@ __global__ __launch_bounds__(64) void k_grid_bfs_wave(GridView gv, const int32_t
 	}
 }
 
+// S_min closure pass over a list of one-wave 6C tiles, the semantics of k_grid_source_closure
+// (s(u) = 1 if u holds excess or some v with s(v) = 1 has residual v -> u > 0; bits only rise;
+// first_pass seeds from the excess; the closure is pushed across the tile faces into the
+// neighbouring voxels, whose tiles are marked; `reached` is set when a voxel with a residual sink
+// link joins) with the tile in registers: a lane's s bits and open directions travel together to
+// its x and y neighbour lanes, the column is swept up and down in registers.
+template <int kTX, int kTY, int kTZ>
+__global__ __launch_bounds__(64) void k_grid_closure_wave(GridView gv, const int32_t *__restrict__ list,
+                                                          const int32_t *__restrict__ count, int first_pass,
+                                                          uint8_t *__restrict__ dirty_next,
+                                                          int32_t *__restrict__ reached, const SolveControl *ctl)
+{
+	static_assert(kTX * kTY == 64 && kTX >= 2 && kTY >= 2, "one wave per tile");
+	constexpr int TZ = kTZ;
+	using D = Dirs<6>;
+	const int tile = list[blockIdx.x];  // in bounds: the grid never exceeds the list's capacity
+	if ((int)blockIdx.x >= *count || !gate_open(ctl, kGateClosure | kGateUnlessReached)) {
+		return;
+	}
+	int local;
+	const GridView g = instance_view(gv, tile, local);
+	int tx, ty, tz;
+	tile_coords(g, local, tx, ty, tz);
+	const int ox = tx * kTX, oy = ty * kTY, oz = tz * kTZ;
+	const int lane = threadIdx.x;
+	const int lx = lane % kTX, ly = lane / kTX;
+	const int gx = ox + lx, gy = oy + ly;
+	const bool in_xy = gx < g.nx && gy < g.ny;
+	const uint32_t xy = slice_xy(g, gx, gy), sxy = (uint32_t)(g.nx * g.ny);
+	uint32_t valid = 0, sbits = 0, before = 0, sink = 0;
+	uint32_t open[TZ];
+#pragma unroll
+	for (int k = 0; k < TZ; ++k) {
+		const bool v = in_xy && oz + k < g.nz;
+		valid |= v ? (1u << k) : 0u;
+		const uint32_t st = v ? tile_at(g.omask, g, oz, k, xy, sxy) : 0u;
+		const bool b = v && tile_at(g.s, g, oz, k, xy, sxy) != 0;
+		open[k] = st & 63u;
+		before |= b ? (1u << k) : 0u;
+		sink |= (st & kStateSink) ? (1u << k) : 0u;
+		sbits |= (b || (first_pass && (st & kStateExcess))) ? (1u << k) : 0u;
+	}
+	// fixpoint inside the tile: a lane's s bit with its open directions (bit 6 and bits 0-5) go to
+	// the x and y neighbour lanes of the same z-level
+	for (;;) {
+		const uint32_t old = sbits;
+#pragma unroll
+		for (int k = 0; k < TZ; ++k) {
+			const int32_t pk = (int32_t)(open[k] | (((sbits >> k) & 1u) << 6));
+			const int32_t xm = lane_xm<kTX>(pk, 0, lx), xp = lane_xp<kTX>(pk, 0, lx);
+			const int32_t ym = lane_ym<kTX>(pk, 0, ly), yp = lane_yp<kTX, kTY>(pk, 0, ly);
+			const bool in = ((xm & 64) && (xm & 8)) || ((xp & 64) && (xp & 1)) || ((ym & 64) && (ym & 16)) ||
+			                ((yp & 64) && (yp & 2));
+			sbits |= (in && ((valid >> k) & 1u)) ? (1u << k) : 0u;
+		}
+#pragma unroll
+		for (int k = 1; k < TZ; ++k) {
+			const bool in = ((sbits >> (k - 1)) & 1u) && (open[k - 1] & 32u);
+			sbits |= (in && ((valid >> k) & 1u)) ? (1u << k) : 0u;
+		}
+#pragma unroll
+		for (int k = TZ - 2; k >= 0; --k) {
+			const bool in = ((sbits >> (k + 1)) & 1u) && (open[k + 1] & 4u);
+			sbits |= (in && ((valid >> k) & 1u)) ? (1u << k) : 0u;
+		}
+		if (!__any(sbits != old)) {
+			break;
+		}
+	}
+	// write back own voxels; push the closure across the tile faces
+	uint32_t marks = 0;
+	auto push_out = [&](int k, int d) {
+		uint8_t &su = tile_at(g.s, g, oz, k + D::dz(d), slice_xy(g, gx + D::dx(d), gy + D::dy(d)), sxy);
+		if (!su) {
+			su = 1;
+			const int sz = k + D::dz(d) < 0 ? 0 : (k + D::dz(d) >= TZ ? 2 : 1);
+			marks |= 1u << ((sz * 3 + D::dy(d) + 1) * 3 + D::dx(d) + 1);
+		}
+	};
+#pragma unroll
+	for (int k = 0; k < TZ; ++k) {
+		if (!((sbits >> k) & 1u)) {
+			continue;
+		}
+		if (!((before >> k) & 1u)) {
+			tile_at(g.s, g, oz, k, xy, sxy) = 1;
+		}
+		// (an open direction always leads to a voxel inside the volume)
+		if (lx == 0 && (open[k] & 1u)) {
+			push_out(k, 0);
+		}
+		if (lx == kTX - 1 && (open[k] & 8u)) {
+			push_out(k, 3);
+		}
+		if (ly == 0 && (open[k] & 2u)) {
+			push_out(k, 1);
+		}
+		if (ly == kTY - 1 && (open[k] & 16u)) {
+			push_out(k, 4);
+		}
+		if (k == 0 && (open[k] & 4u)) {
+			push_out(k, 2);
+		}
+		if (k == TZ - 1 && (open[k] & 32u)) {
+			push_out(k, 5);
+		}
+	}
+	const bool hit = (sbits & sink) != 0;
+	if (reached && __any(hit) && lane == 0) {
+		*reached = 1;  // every writer stores the same value
+	}
+#pragma unroll
+	for (int off = 32; off > 0; off >>= 1) {
+		marks |= (uint32_t)__shfl_xor((int)marks, off, 64);
+	}
+	if (lane < 27 && ((marks >> lane) & 1u)) {
+		mark_neighbour_tile(g, dirty_next, tx, ty, tz, lane);
+	}
+	if (lane == 0) {
+		atomicAdd(counter_slot(g, 2, tile), 1ull);
+	}
+}
+
 }  // namespace cgp
