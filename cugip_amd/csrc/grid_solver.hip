@@ -689,7 +689,7 @@ void GridSolver::apply_status(int32_t status)
 // The host copy holds the count words at int32 [1] [2] (and [3], the closure's reached flag); during
 // a closure (mPrefetchTotals) the read also brings the visit-counter and flow totals, since the
 // solve ends at one of the closure's checks (mTotalsRead then says the copy is current).
-void GridSolver::count_two(uint8_t *a, uint8_t *b, int &na, int &nb)
+void GridSolver::count_two(uint8_t *a, uint8_t *b, int &na, int &nb, bool defer)
 {
 	int32_t *pinned = reinterpret_cast<int32_t *>(mHostScratch.get());
 	const size_t n = mInst.size();
@@ -717,14 +717,36 @@ void GridSolver::count_two(uint8_t *a, uint8_t *b, int &na, int &nb)
 		CGP_LAUNCH_CHECK();
 		bytes = (kReadbackHead + n) * sizeof(unsigned long long);
 	}
+	if (defer) {
+		// the counts go to their own slot: the next host wait reads them (deferred_counts)
+		CGP_HIP(hipMemcpyAsync(const_cast<int32_t *>(deferred_counts()), mCount.get(), 4 * sizeof(int32_t),
+		                       hipMemcpyDeviceToHost, mStream));
+		mCheckDeferred = true;
+		return;
+	}
 	CGP_HIP(hipMemcpyAsync(pinned, mCount.get(), bytes, hipMemcpyDeviceToHost, mStream));
+	count_two_wait(na, nb, b != nullptr);
+}
+
+// Host copy of the counts of a deferred count_two (valid after the next host wait): [1] the first
+// array's flags, [2] the second's.
+const int32_t *GridSolver::deferred_counts() const
+{
+	return reinterpret_cast<const int32_t *>(mHostScratch.get()) + 2 * (kReadbackHead + mInst.size()) + 4;
+}
+
+// The host wait of a count_two: the counts (and a pending init status) once the stream got there.
+void GridSolver::count_two_wait(int &na, int &nb, bool has_b)
+{
+	const int32_t *pinned = reinterpret_cast<const int32_t *>(mHostScratch.get());
 	CGP_HIP(hipStreamSynchronize(mStream));
+	mCheckDeferred = false;
 	mTotalsRead = mPrefetchTotals;
 	if (mStatusPending) {
-		apply_status(*status_copy);
+		apply_status(pinned[2 * (kReadbackHead + mInst.size())]);
 	}
 	na = pinned[1];
-	nb = b ? pinned[2] : 0;
+	nb = has_b ? pinned[2] : 0;
 }
 
 // Flow of every grid (into mFlows) and the three visit counters from the totals the last closure
@@ -751,7 +773,7 @@ int64_t GridSolver::totals_from_readback(unsigned long long *visits)
 // Returns the number of pass launches; `also` receives the flag count of also_count (if given)
 // at the final check.
 template <typename Pass>
-int GridSolver::colour_sweeps(uint8_t *dirty, uint8_t *also_count, int &also, Pass &&pass, bool *stop)
+int GridSolver::colour_sweeps(uint8_t *dirty, uint8_t *also_count, int &also, Pass &&pass, bool *stop, bool defer)
 {
 	constexpr int kCyclesPerCheck = 2;
 	int launches = 0;
@@ -779,7 +801,11 @@ int GridSolver::colour_sweeps(uint8_t *dirty, uint8_t *also_count, int &also, Pa
 	for (bool check = false;; check = true) {
 		if (check) {
 			int pending = 0;
-			count_two(dirty, also_count, pending, also);
+			++mSweepChecks;
+			count_two(dirty, also_count, pending, also, defer);
+			if (defer) {
+				return launches;  // the caller queues more work, then waits for the counts
+			}
 			if (stop) {
 				// the sweeps' own flag (the closure reaching t), read with the counts
 				const int32_t *pinned = reinterpret_cast<const int32_t *>(mHostScratch.get());
@@ -878,7 +904,7 @@ void GridSolver::launch_discharge(int max_iters, bool wide)
 	});
 }
 
-int GridSolver::run_global_relabel(int32_t threshold, cgp_stats &stats)
+int GridSolver::run_global_relabel(int32_t threshold, cgp_stats &stats, bool defer)
 {
 	const int64_t tile_voxels = (int64_t)mShape.tx * mShape.ty * mShape.tz;
 	// 6-connected first sweeps start from kLabelInf themselves (first_mode); 26-connected tiles read
@@ -898,12 +924,13 @@ int GridSolver::run_global_relabel(int32_t threshold, cgp_stats &stats)
 	}
 	mDirtyAClean = false;
 	int active = 0;
-	stats.bfs_passes += colour_sweeps(mDirtyA.get(), mActive.get(), active,
-	                                  [&](const int32_t *list, const int32_t *count, int grid, int first) {
-		                                  launch_bfs_pass(list, count, grid, threshold, first ? grid * tile_voxels : -1,
-		                                                  mConn == 6 ? first : 0);
-	                                  });
-	mDirtyAClean = true;
+	stats.bfs_passes += colour_sweeps(
+	    mDirtyA.get(), mActive.get(), active,
+	    [&](const int32_t *list, const int32_t *count, int grid, int first) {
+		    launch_bfs_pass(list, count, grid, threshold, first ? grid * tile_voxels : -1, mConn == 6 ? first : 0);
+	    },
+	    nullptr, defer);
+	mDirtyAClean = !mCheckDeferred;  // a deferred check may still find dirty tiles
 	return active;
 }
 
@@ -1251,10 +1278,46 @@ int64_t GridSolver::solve(const cgp_policy &policy, cgp_stats *stats_out, const 
 
 	int round = 0;
 	int64_t phase = 0;  // colours rotate across rounds, so every colour is discharged whatever `phases` is
+	// One-wave 6C tiles: when the last relabel ended at its first check, the next one queues its
+	// discharge phases behind that check instead of waiting for it (one host wait fewer per round).
+	// If the check then finds dirty tiles left, the phases ran on labels not yet exact; that only
+	// costs work, since pushes along any labels keep a valid preflow and the closure test decides
+	// termination; such a round goes to the closure test, and the next relabel waits again.
+	const bool may_defer = mConn == 6 && mShape.tx * mShape.ty == 64 && !trace.fn && !mProfile && mCertificate == 0;
+	bool exact_at_close = false;
 	for (;;) {
 		trace.emit(CGP_TRACE_BEGIN_ITERATION, round, 0, 0);
 		auto t0 = std::chrono::steady_clock::now();
-		const int active = run_global_relabel(threshold, stats);
+		const bool defer = may_defer && mRelabelOneCheck && threshold == 0 && round < max_rounds;
+		mSweepChecks = 0;
+		int active = run_global_relabel(threshold, stats, defer);
+		if (defer) {
+			// the phases and the termination closure follow in the same queue; the closure's first host
+			// wait also brings the relabel's counts
+			for (int p = 0; p < phases; ++p, ++phase) {
+				run_discharge_phase((int)(phase % ncolours), max_iters);
+			}
+			stats.relabel_ms += ms_since(t0);
+			++stats.rounds;
+			stats.discharge_phases += phases;
+			++round;
+			t0 = std::chrono::steady_clock::now();
+			const bool reached = run_source_closure(stats, true);
+			stats.cut_ms += ms_since(t0);
+			const int32_t *counts = deferred_counts();
+			const int pending = counts[1];
+			active = counts[2];
+			mDirtyAClean = pending == 0;
+			mRelabelOneCheck = pending == 0;
+			if (!reached) {
+				// (with nothing active after an exact relabel, the labels are exact too)
+				mClosedByCheck = true;
+				exact_at_close = pending == 0 && active == 0;
+				break;
+			}
+			continue;
+		}
+		mRelabelOneCheck = mSweepChecks == 1;
 		stats.relabel_ms += ms_since(t0);
 		++stats.rounds;
 		trace.emit(CGP_TRACE_AFTER_RELABEL, round, 0, 0);
@@ -1328,7 +1391,7 @@ int64_t GridSolver::solve(const cgp_policy &policy, cgp_stats *stats_out, const 
 	mSolved = true;
 	// labels are those of the last relabel when the solve stopped at the closure test; T_min and
 	// label readbacks relabel once on request
-	mLabelsExact = !mClosedByCheck;
+	mLabelsExact = !mClosedByCheck || exact_at_close;
 	trace.emit(CGP_TRACE_AFTER_PUSH, round, 1, flow);
 	trace.emit(CGP_TRACE_FINISHED, round, 1, flow);
 	if (stats_out) {

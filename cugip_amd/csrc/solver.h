@@ -109,7 +109,7 @@ private:
 	Shape pick_shape(const cgp_policy &policy);
 	void configure_tiles(const Shape &shape);
 	void launch_init(const SolveControl *ctl);
-	int run_global_relabel(int32_t threshold, cgp_stats &stats);
+	int run_global_relabel(int32_t threshold, cgp_stats &stats, bool defer = false);
 	// A colouring of the tiles: per-tile colour, the tile ids grouped by colour (then one count per
 	// colour), the groups' starts and sizes, and the largest group.
 	struct ColourSet {
@@ -125,9 +125,15 @@ private:
 	void launch_closure_pass(const int32_t *list, const int32_t *count, int grid, bool first, int32_t *reached,
 	                         const SolveControl *ctl, int64_t voxels);
 	void launch_discharge(int max_iters, bool wide);
-	void count_two(uint8_t *a, uint8_t *b, int &na, int &nb);
+	void count_two(uint8_t *a, uint8_t *b, int &na, int &nb, bool defer = false);
+	void count_two_wait(int &na, int &nb, bool has_b);
+	const int32_t *deferred_counts() const;
+	bool mCheckDeferred = false;      // a count_two was queued without its host wait
+	bool mRelabelOneCheck = true;     // the last relabel ended at its first check (the deferral pays)
+	int mSweepChecks = 0;             // host checks of the last colour sweeps
 	template <typename Pass>
-	int colour_sweeps(uint8_t *dirty, uint8_t *also_count, int &also, Pass &&pass, bool *stop = nullptr);
+	int colour_sweeps(uint8_t *dirty, uint8_t *also_count, int &also, Pass &&pass, bool *stop = nullptr,
+	                  bool defer = false);
 	void run_discharge_phase(int colour, int max_iters);
 	// check: also report whether the closure reached a voxel with a residual sink link (the
 	// preflow is then not maximum and the closure is not S_min); it stops early when it does
