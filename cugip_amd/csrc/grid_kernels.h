@@ -1267,9 +1267,10 @@ struct PieceWork {
 };
 
 // Exclusive scan of (lo, hi) pairs of packed counts over the workgroup; totals in tlo, thi.
-__device__ inline void block_exclusive_scan_pair_1024(unsigned long long &lo, unsigned long long &hi,
-                                                      unsigned long long (*wave_sums)[2], unsigned long long &tlo,
-                                                      unsigned long long &thi)
+template <int kThreads = kCompactThreads>
+__device__ inline void block_exclusive_scan_pair(unsigned long long &lo, unsigned long long &hi,
+                                                 unsigned long long (*wave_sums)[2], unsigned long long &tlo,
+                                                 unsigned long long &thi)
 {
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 	unsigned long long a = lo, b = hi;
@@ -1289,7 +1290,7 @@ __device__ inline void block_exclusive_scan_pair_1024(unsigned long long &lo, un
 	unsigned long long before_a = 0, before_b = 0;
 	tlo = thi = 0;
 #pragma unroll
-	for (int w = 0; w < kCompactThreads / 64; ++w) {
+	for (int w = 0; w < kThreads / 64; ++w) {
 		const unsigned long long sa = wave_sums[w][0], sb = wave_sums[w][1];
 		before_a += w < wave ? sa : 0;
 		before_b += w < wave ? sb : 0;
@@ -1471,7 +1472,7 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact_block(uint8_t *__re
 		}
 	}
 	unsigned long long tlo = 0, thi = 0;
-	block_exclusive_scan_pair_1024(lo, hi, class_sums, tlo, thi);
+	block_exclusive_scan_pair(lo, hi, class_sums, tlo, thi);
 	// class c starts after the totals of the classes before it
 	int offset[kWorkClasses];
 	int start = 0;
@@ -1515,7 +1516,7 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact_block(uint8_t *__re
 // compacts quickly: the list is laid out in class regions (class c's tiles at list + c * stride,
 // stride = the largest colour class, their number in cursors[c]), which the discharge walks in
 // class order (discharge_tile), so no pass needs the totals of the classes before it. Workgroups
-// take 4096 tiles (4 a thread, one 4-byte load per array), scan their per-class counts, and
+// take 1024 tiles (4 a thread, one 4-byte load per array), scan their per-class counts, and
 // reserve their run in each class's region with one atomic per class (the eight issued together by
 // eight lanes). The cursors must be zero beforehand; zero_next, the other set of a solver that
 // alternates between two, is cleared. Within a class the order between workgroups follows their
@@ -1551,7 +1552,9 @@ __device__ inline uint32_t order_select(const uint8_t *flags, int ntiles, const 
 	return sel;
 }
 
-__global__ __launch_bounds__(kCompactThreads) void k_order_regions(const uint8_t *__restrict__ flags, int ntiles,
+constexpr int kOrderThreads = 256;  // 1024 tiles a workgroup: short scans, many workgroups at once
+
+__global__ __launch_bounds__(kOrderThreads) void k_order_regions(const uint8_t *__restrict__ flags, int ntiles,
                                                                    const uint8_t *__restrict__ colours, int colour,
                                                                    const uint8_t *__restrict__ work,
                                                                    int32_t *__restrict__ list, int stride,
@@ -1559,7 +1562,7 @@ __global__ __launch_bounds__(kCompactThreads) void k_order_regions(const uint8_t
                                                                    int32_t *__restrict__ zero_next,
                                                                    const SolveControl *ctl, unsigned gate)
 {
-	__shared__ unsigned long long sums[kCompactThreads / 64][2];
+	__shared__ unsigned long long sums[kOrderThreads / 64][2];
 	__shared__ int base[kWorkClasses];
 	if (blockIdx.x == 0 && threadIdx.x < kWorkClasses) {
 		zero_next[threadIdx.x] = 0;
@@ -1567,12 +1570,12 @@ __global__ __launch_bounds__(kCompactThreads) void k_order_regions(const uint8_t
 	if (!gate_open(ctl, gate)) {
 		return;  // the cursors stay at zero: an empty list
 	}
-	const int t0 = ((int)blockIdx.x * kCompactThreads + (int)threadIdx.x) * kCompactTilesPerThread;
+	const int t0 = ((int)blockIdx.x * kOrderThreads + (int)threadIdx.x) * kCompactTilesPerThread;
 	uint32_t cls4;
 	unsigned long long lo, hi;
 	const uint32_t sel = order_select(flags, ntiles, colours, colour, work, t0, cls4, lo, hi);
 	unsigned long long tlo = 0, thi = 0;
-	block_exclusive_scan_pair_1024(lo, hi, sums, tlo, thi);  // lo, hi: this thread's offsets per class
+	block_exclusive_scan_pair<kOrderThreads>(lo, hi, sums, tlo, thi);  // lo, hi: this thread's offsets per class
 	if (threadIdx.x < kWorkClasses) {
 		const int c = (int)threadIdx.x;
 		const int n = (int)(((c < 4 ? tlo : thi) >> (16 * (c & 3))) & 0xffffull);

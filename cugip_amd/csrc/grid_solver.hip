@@ -651,7 +651,8 @@ void GridSolver::launch_compact(const ColourSet &set, uint8_t *flags, int colour
 		int32_t *cursors = mCursors.get() + kWorkClasses * mOrderParity;
 		int32_t *next = mCursors.get() + kWorkClasses * (mOrderParity ^ 1);
 		launch(kCompact, 0, 0.0, [&] {
-			hipLaunchKernelGGL(k_order_regions, dim3(blocks), dim3(kCompactThreads), 0, mStream, (const uint8_t *)flags,
+			const int oblocks = (mTiles + kOrderThreads * kCompactTilesPerThread - 1) / (kOrderThreads * kCompactTilesPerThread);
+			hipLaunchKernelGGL(k_order_regions, dim3(oblocks), dim3(kOrderThreads), 0, mStream, (const uint8_t *)flags,
 			                   mTiles, (const uint8_t *)set.colour.get(), colour, work, mList.get(), mClassStride,
 			                   cursors, next, ctl, gate);
 		});
