@@ -366,7 +366,8 @@ def traffic_from_counters(fetch_csv, write_csv, out_path, copy_bytes):
     wcal = known_kib / sum(wa[i][1] for i in ids[:3])
     last_init = [i for i in ids if "k_grid_init" in fa[i][0]][-1]
     names = {"k_grid_discharge": "grid_discharge", "k_grid_discharge_wave": "grid_discharge", "k_grid_bfs": "grid_bfs",
-             "k_grid_init": "grid_init", "k_grid_source_closure": "grid_source_closure"}
+             "k_grid_bfs_wave": "grid_bfs", "k_grid_init": "grid_init", "k_grid_source_closure": "grid_source_closure",
+             "k_grid_closure_wave": "grid_source_closure"}
     out = dict(workload="last solve of `bench.py --calibration-copies --steps 1 --warmup 1 --no-cpu --batch 0 --generic 0 --batch-workload 0`",
                calibration=dict(fetch_factor=round(fcal, 4), write_factor=round(wcal, 4),
                                 how="bytes of the three device-to-device copies / their FETCH_SIZE and WRITE_SIZE"))
