@@ -205,6 +205,34 @@ __device__ inline int wave_max_i32(int v)
 	return v;
 }
 
+// Wave maximum and minimum of non-negative values by DPP row shifts and row broadcasts, the result
+// collecting in lane 63 and returned wave-uniform; lanes with no source lane take the operation's
+// identity (0 for the maximum, all ones for the minimum), so each step is one DPP-combined max/min.
+// No LDS round trips, unlike the shuffle version above.
+template <bool kMax>
+__device__ inline uint32_t wave_reduce_dpp(uint32_t v)
+{
+	constexpr uint32_t id = kMax ? 0u : 0xffffffffu;
+	auto op = [](uint32_t a, uint32_t b) { return kMax ? (a > b ? a : b) : (a < b ? a : b); };
+	v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x111, 0xf, 0xf, false));  // row_shr:1
+	v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x112, 0xf, 0xf, false));  // row_shr:2
+	v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x114, 0xf, 0xf, false));  // row_shr:4
+	v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x118, 0xf, 0xf, false));  // row_shr:8
+	v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x142, 0xa, 0xf, false));  // row_bcast:15 into rows 1, 3
+	v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x143, 0xc, 0xf, false));  // row_bcast:31 into rows 2, 3
+	return __builtin_amdgcn_readlane(v, 63);
+}
+
+__device__ inline int wave_max_nonneg(int v)
+{
+	return (int)wave_reduce_dpp<true>((uint32_t)v);
+}
+
+__device__ inline int wave_min_nonneg(int v)
+{
+	return (int)wave_reduce_dpp<false>((uint32_t)v);
+}
+
 // Block-wide int64 sum, result valid in thread 0. `scratch` holds one slot per wave.
 template <int kThreads>
 __device__ inline long long block_sum_i64(long long v, long long *scratch)

@@ -3,7 +3,8 @@
 //
 // Discharge with the whole tile state in registers: excess, residual sink link,
 // the 6 residuals and the label of every voxel live in the lane that owns it, x neighbours are
-// read across lanes with DPP wave shifts, y neighbours with a lane permute by TX, z neighbours are
+// read across lanes with DPP shifts (row shifts when a tile row is a 16-lane DPP row), y
+// neighbours with a lane permute by TX, z neighbours are
 // the lane's own registers, and the halo labels (fixed while the tile runs: tiles of one colour
 // never touch) are loaded once into the lanes on the tile faces. No LDS, no barriers: a step of
 // the push sweep is a few dozen vector instructions instead of a chain of LDS round trips.
@@ -14,7 +15,10 @@
 // t inside the tile given the halo labels; top-down push over the levels that hold excess, each
 // voxel of level L pushing first into t, then along its admissible directions in order to level
 // L - 1} up to max_iters times, then write back what changed and send the flow that left the tile
-// to the neighbouring voxels. Included at the end of grid_kernels.h.
+// to the neighbouring voxels. What changed is tracked per voxel and residual, generously: a
+// sender's admissible residuals and a receiver's four x and y residuals are all written (a store
+// of an unchanged value is cheaper than the vector instructions that would tell them apart).
+// Included at the end of grid_kernels.h.
 #pragma once
 
 namespace cgp {
@@ -23,6 +27,10 @@ namespace cgp {
 template <int kTX>
 __device__ inline int32_t lane_xm(int32_t v, int32_t edge, int lx)
 {
+	if constexpr (kTX == 16) {
+		// a tile row is a DPP row: lanes with lx == 0 have no source lane and keep `edge`
+		return __builtin_amdgcn_update_dpp(edge, v, 0x111 /* row_shr:1 */, 0xf, 0xf, false);
+	}
 	const int32_t s = __builtin_amdgcn_update_dpp(edge, v, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
 	return lx == 0 ? edge : s;
 }
@@ -31,6 +39,9 @@ __device__ inline int32_t lane_xm(int32_t v, int32_t edge, int lx)
 template <int kTX>
 __device__ inline int32_t lane_xp(int32_t v, int32_t edge, int lx)
 {
+	if constexpr (kTX == 16) {
+		return __builtin_amdgcn_update_dpp(edge, v, 0x101 /* row_shl:1 */, 0xf, 0xf, false);
+	}
 	const int32_t s = __builtin_amdgcn_update_dpp(edge, v, 0x130 /* wave_shl:1 */, 0xf, 0xf, false);
 	return lx == kTX - 1 ? edge : s;
 }
@@ -147,13 +158,18 @@ __global__ __launch_bounds__(64, 4) void k_grid_discharge_wave(GridView gv, cons
 	const int32_t hzhi = (in_xy && oz + TZ < g.nz) ? tile_at(g.h, g, oz, TZ, xy, sxy) : kLabelInf;
 
 	// 6C: flow pushed across the tile faces during the visit (a lane touches at most one x face and
-	// one y face, and the two z faces at its column's ends)
-	int32_t out_x[TZ], out_y[TZ], out_zlo = 0, out_zhi = 0;
+	// one y face, and the two z faces at its column's ends) is the drop of the residual across that
+	// face: no flow enters the tile from outside while it runs (the neighbouring tiles are of the
+	// other colour), so these residuals only fall. The residuals at load time are kept here.
+	const int dfx = lx == 0 ? 0 : 3, dfy = ly == 0 ? 1 : 4;
+	const bool on_x = lx == 0 || lx == kTX - 1, on_y = ly == 0 || ly == kTY - 1;
+	int32_t out_x[TZ], out_y[TZ];
+	const int32_t out_zlo = r[2][0], out_zhi = r[5][TZ - 1];
 	uint32_t dirty = 0;  // bits 8k + d: residual d of voxel k changed; bit 8k + 6: excess or rt changed
 #pragma unroll
 	for (int k = 0; k < TZ; ++k) {
-		out_x[k] = 0;
-		out_y[k] = 0;
+		out_x[k] = on_x ? (dfx == 0 ? r[0][k] : r[3][k]) : 0;
+		out_y[k] = on_y ? (dfy == 1 ? r[1][k] : r[4][k]) : 0;
 	}
 	long long sink = 0;
 	int32_t lab[TZ];  // the loaded labels, stored back as they are if no iteration runs
@@ -196,20 +212,22 @@ __global__ __launch_bounds__(64, 4) void k_grid_discharge_wave(GridView gv, cons
 			a |= xp == l - 1 ? 8u : 0u;
 			a |= yp == l - 1 ? 16u : 0u;
 			a |= zp == l - 1 ? 32u : 0u;
-			adm[k] = a & open[k];
+			adm[k] = (a & open[k]) | 0x40u;  // bit 6: the excess, marked with the directions below
 			const bool h = ((valid >> k) & 1u) && ex[k] > 0 && l < kLabelInf;
 			held |= h ? (1u << k) : 0u;
 			top = h && l > top ? l : top;
 		}
-		int level = wave_max_i32(top);
+		int level = wave_max_nonneg(top);
 		if (level == 0) {
 			break;
 		}
 		// ---- top-down push sweep over the levels that hold excess: a voxel of level L pushes only
 		// to voxels of level L - 1 (exact labels), so the senders of a step never receive in it;
 		// the next level is L - 1 if this step pushed inside the tile, else the highest lower level
-		// whose voxels held excess when the sweep began
-		while (level >= 1) {
+		// whose voxels held excess when the sweep began. Only level 1 has open sink links (a label
+		// is 1 exactly where rt > 0, and rt only falls during the visit) and no admissible
+		// direction (no label is 0), so levels >= 2 push along directions and level 1 into t.
+		while (level >= 2) {
 			int next = 0;
 			bool inside = false;
 #pragma unroll
@@ -221,63 +239,62 @@ __global__ __launch_bounds__(64, 4) void k_grid_discharge_wave(GridView gv, cons
 				if (!__any(send)) {
 					continue;  // no voxel of this z-level sends: nothing to deliver either
 				}
-				int32_t e = ex[k];
-				const int32_t fs = send ? (e < rt[k] ? e : rt[k]) : 0;
-				rt[k] -= fs;
-				e -= fs;
-				sink += fs;
+				const int32_t e0 = send ? ex[k] : 0;  // non-senders push nothing
+				int32_t e = e0;
 				int32_t f[6];
 #pragma unroll
 				for (int d = 0; d < 6; ++d) {
-					const int32_t fd = (send && ((adm[k] >> d) & 1u)) ? (e < r[d][k] ? e : r[d][k]) : 0;
+					const int32_t fd = ((adm[k] >> d) & 1u) ? (e < r[d][k] ? e : r[d][k]) : 0;
 					r[d][k] -= fd;
 					e -= fd;
 					f[d] = fd;
-					dirty |= fd > 0 ? (1u << (8 * k + d)) : 0u;
 				}
-				dirty |= e != ex[k] ? (1u << (8 * k + 6)) : 0u;
-				ex[k] = e;
+				// a sender's excess and every admissible residual are written back (the directions
+				// after the one that used up its excess unchanged)
+				dirty |= (send ? adm[k] : 0u) << (8 * k);
 				// deliveries: the receiver's excess and its residual back towards the sender
 				const int32_t from_xm = lane_xm<kTX>(f[3], 0, lx);  // sent +x by lane - 1
 				const int32_t from_xp = lane_xp<kTX>(f[0], 0, lx);  // sent -x by lane + 1
 				const int32_t from_ym = lane_ym<kTX>(f[4], 0, ly);
 				const int32_t from_yp = lane_yp<kTX, kTY>(f[1], 0, ly);
 				const int32_t got = from_xm + from_xp + from_ym + from_yp;
-				ex[k] += got;
+				ex[k] += got - (e0 - e);
 				r[0][k] += from_xm;
 				r[3][k] += from_xp;
 				r[1][k] += from_ym;
 				r[4][k] += from_yp;
-				dirty |= (from_xm > 0 ? (1u << (8 * k + 0)) : 0u) | (from_xp > 0 ? (1u << (8 * k + 3)) : 0u) |
-				         (from_ym > 0 ? (1u << (8 * k + 1)) : 0u) | (from_yp > 0 ? (1u << (8 * k + 4)) : 0u) |
-				         (got > 0 ? (1u << (8 * k + 6)) : 0u);
-				// flow across the tile faces stays in registers until the write-back
-				out_x[k] += (lx == 0 ? f[0] : 0) + (lx == kTX - 1 ? f[3] : 0);
-				out_y[k] += (ly == 0 ? f[1] : 0) + (ly == kTY - 1 ? f[4] : 0);
-				inside |= (lx > 0 && f[0] > 0) || (lx < kTX - 1 && f[3] > 0) || (ly > 0 && f[1] > 0) ||
-				          (ly < kTY - 1 && f[4] > 0);
-				// z: the column's own next voxels, or the z faces
+				// a receiver's excess and its four x and y residuals
+				dirty |= got > 0 ? (0x5bu << (8 * k)) : 0u;
+				inside |= got > 0;  // every push inside the tile is received by a lane or the column
+				// z: the column's own next voxels (pushes across the z faces are counted at the end)
 				if (k > 0) {
 					ex[k > 0 ? k - 1 : 0] += f[2];
 					r[5][k > 0 ? k - 1 : 0] += f[2];
 					dirty |= f[2] > 0 ? (1u << (8 * (k - 1) + 5)) | (1u << (8 * (k - 1) + 6)) : 0u;
 					inside |= f[2] > 0;
-				} else {
-					out_zlo += f[2];
 				}
 				if (k < TZ - 1) {
 					ex[k < TZ - 1 ? k + 1 : 0] += f[5];
 					r[2][k < TZ - 1 ? k + 1 : 0] += f[5];
 					dirty |= f[5] > 0 ? (1u << (8 * (k + 1) + 2)) | (1u << (8 * (k + 1) + 6)) : 0u;
 					inside |= f[5] > 0;
-				} else {
-					out_zhi += f[5];
 				}
 			}
-			if (inside) {
+			if (__any(inside)) {
 				next = level - 1 > next ? level - 1 : next;
 			}
-			level = wave_max_i32(next);
+			level = wave_max_nonneg(next);
+		}
+		if (level == 1) {
+#pragma unroll
+			for (int k = 0; k < TZ; ++k) {
+				const bool send = ((valid >> k) & 1u) && lab[k] == 1 && ex[k] > 0;
+				const int32_t fs = send ? (ex[k] < rt[k] ? ex[k] : rt[k]) : 0;
+				rt[k] -= fs;
+				ex[k] -= fs;
+				sink += fs;
+				dirty |= fs > 0 ? (1u << (8 * k + 6)) : 0u;
+			}
 		}
 		// when the sweep left no excess in the tile, another region relabel would only find that
 		// out; the labels stay valid (pushes along admissible edges keep them valid)
@@ -335,11 +352,11 @@ __global__ __launch_bounds__(64, 4) void k_grid_discharge_wave(GridView gv, cons
 	};
 #pragma unroll
 	for (int k = 0; k < TZ; ++k) {
-		send_out(k, lx == 0 ? 0 : 3, out_x[k]);
-		send_out(k, ly == 0 ? 1 : 4, out_y[k]);
+		send_out(k, dfx, on_x ? out_x[k] - (dfx == 0 ? r[0][k] : r[3][k]) : 0);
+		send_out(k, dfy, on_y ? out_y[k] - (dfy == 1 ? r[1][k] : r[4][k]) : 0);
 	}
-	send_out(0, 2, out_zlo);
-	send_out(TZ - 1, 5, out_zhi);
+	send_out(0, 2, out_zlo - r[2][0]);
+	send_out(TZ - 1, 5, out_zhi - r[5][TZ - 1]);
 	// (a lane on both x faces or both y faces would need two accumulators; TX, TY >= 2 here)
 	static_assert(kTX >= 2 && kTY >= 2, "a lane lies on at most one x face and one y face");
 
@@ -468,8 +485,8 @@ __global__ __launch_bounds__(64) void k_grid_bfs_wave(GridView gv, const int32_t
 	for (int off = 32; off > 0; off >>= 1) {
 		marks |= (uint32_t)__shfl_xor((int)marks, off, 64);
 	}
-	top = wave_max_i32(top);
-	bottom = -wave_max_i32(-bottom);
+	top = wave_max_nonneg(top);
+	bottom = wave_min_nonneg(bottom);
 	const bool any_active = __any(active);
 	if (lane < 27 && ((marks >> lane) & 1u)) {
 		mark_neighbour_tile(g, dirty_next, tx, ty, tz, lane);
