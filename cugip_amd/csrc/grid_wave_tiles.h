@@ -62,46 +62,86 @@ __device__ inline int32_t lane_yp(int32_t v, int32_t edge, int ly)
 	return ly == kTY - 1 ? edge : s;
 }
 
+// Value of lane - 1 (the -x neighbour), or the lane's own value on the tile's -x face.
+template <int kTX>
+__device__ inline int32_t lane_xm_or_own(int32_t v, int lx)
+{
+	if constexpr (kTX == 16) {
+		return __builtin_amdgcn_update_dpp(v, v, 0x111 /* row_shr:1 */, 0xf, 0xf, false);
+	}
+	const int32_t s = __builtin_amdgcn_update_dpp(v, v, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
+	return lx == 0 ? v : s;
+}
+
+// Value of lane + 1 (the +x neighbour), or the lane's own value on the +x face.
+template <int kTX>
+__device__ inline int32_t lane_xp_or_own(int32_t v, int lx)
+{
+	if constexpr (kTX == 16) {
+		return __builtin_amdgcn_update_dpp(v, v, 0x101 /* row_shl:1 */, 0xf, 0xf, false);
+	}
+	const int32_t s = __builtin_amdgcn_update_dpp(v, v, 0x130 /* wave_shl:1 */, 0xf, 0xf, false);
+	return lx == kTX - 1 ? v : s;
+}
+
 // Region fixpoint h = min(h, 1 + h(nbr)) over the open directions of the lane's column, x and y
 // neighbours across lanes (Jacobi rounds), the column swept up and down in registers; halo labels
-// fixed. Shared by the relabel pass and the discharge's region relabel.
+// fixed, so their terms are applied once before the rounds, and a round reads in-tile neighbours
+// only (a lane with no neighbour in a direction reads its own label, which changes nothing).
+// Voxels outside the volume have no open direction and keep their label. Labels only fall, so a
+// round changed something exactly when the lane's label sum changed. Shared by the relabel pass
+// and the discharge's region relabel.
 template <int kTX, int kTY, int kTZ>
-__device__ inline void wave_relax(int32_t (&lab)[kTZ], const uint32_t (&open)[kTZ], uint32_t valid,
-                                  const int32_t (&hx)[kTZ], const int32_t (&hy)[kTZ], int32_t hzlo, int32_t hzhi,
-                                  int lx, int ly)
+__device__ inline void wave_relax(int32_t (&lab)[kTZ], const uint32_t (&open)[kTZ], const int32_t (&hx)[kTZ],
+                                  const int32_t (&hy)[kTZ], int32_t hzlo, int32_t hzhi, int lx, int ly)
 {
+	const uint32_t xbit = lx == 0 ? 1u : (lx == kTX - 1 ? 8u : 0u);  // the x face of this lane
+	const uint32_t ybit = ly == 0 ? 2u : (ly == kTY - 1 ? 16u : 0u);
+	uint32_t sum = 0;
+#pragma unroll
+	for (int k = 0; k < kTZ; ++k) {
+		int32_t b = lab[k];
+		b = (open[k] & xbit) && hx[k] + 1 < b ? hx[k] + 1 : b;
+		b = (open[k] & ybit) && hy[k] + 1 < b ? hy[k] + 1 : b;
+		if (k == 0) {
+			b = (open[k] & 4u) && hzlo + 1 < b ? hzlo + 1 : b;
+		}
+		if (k == kTZ - 1) {
+			b = (open[k] & 32u) && hzhi + 1 < b ? hzhi + 1 : b;
+		}
+		lab[k] = b;
+		sum += (uint32_t)b;
+	}
 	for (;;) {
-		bool changed = false;
 #pragma unroll
 		for (int k = 0; k < kTZ; ++k) {
-			const int32_t xm = lane_xm<kTX>(lab[k], hx[k], lx), xp = lane_xp<kTX>(lab[k], hx[k], lx);
-			const int32_t ym = lane_ym<kTX>(lab[k], hy[k], ly), yp = lane_yp<kTX, kTY>(lab[k], hy[k], ly);
+			const int32_t lp = lab[k] + 1;
+			const int32_t xm = lane_xm_or_own<kTX>(lp, lx), xp = lane_xp_or_own<kTX>(lp, lx);
+			const int32_t ym = __shfl_up(lp, (unsigned)kTX, 64), yp = __shfl_down(lp, (unsigned)kTX, 64);
 			int32_t best = lab[k];
-			best = (open[k] & 1u) && xm + 1 < best ? xm + 1 : best;
-			best = (open[k] & 8u) && xp + 1 < best ? xp + 1 : best;
-			best = (open[k] & 2u) && ym + 1 < best ? ym + 1 : best;
-			best = (open[k] & 16u) && yp + 1 < best ? yp + 1 : best;
-			if (((valid >> k) & 1u) && best < lab[k]) {
-				lab[k] = best;
-				changed = true;
-			}
+			best = (open[k] & 1u) && xm < best ? xm : best;
+			best = (open[k] & 8u) && xp < best ? xp : best;
+			best = (open[k] & 2u) && ym < best ? ym : best;
+			best = (open[k] & 16u) && yp < best ? yp : best;
+			lab[k] = best;
 		}
+#pragma unroll
+		for (int k = 1; k < kTZ; ++k) {
+			const int32_t below = lab[k - 1] + 1;
+			lab[k] = (open[k] & 4u) && below < lab[k] ? below : lab[k];
+		}
+#pragma unroll
+		for (int k = kTZ - 2; k >= 0; --k) {
+			const int32_t above = lab[k + 1] + 1;
+			lab[k] = (open[k] & 32u) && above < lab[k] ? above : lab[k];
+		}
+		uint32_t now = 0;
 #pragma unroll
 		for (int k = 0; k < kTZ; ++k) {
-			const int32_t below = (k == 0 ? hzlo : lab[k > 0 ? k - 1 : 0]) + 1;
-			if (((valid >> k) & 1u) && (open[k] & 4u) && below < lab[k]) {
-				lab[k] = below;
-				changed = true;
-			}
+			now += (uint32_t)lab[k];
 		}
-#pragma unroll
-		for (int k = kTZ - 1; k >= 0; --k) {
-			const int32_t above = (k == kTZ - 1 ? hzhi : lab[k < kTZ - 1 ? k + 1 : 0]) + 1;
-			if (((valid >> k) & 1u) && (open[k] & 32u) && above < lab[k]) {
-				lab[k] = above;
-				changed = true;
-			}
-		}
+		const bool changed = now != sum;
+		sum = now;
 		if (!__any(changed)) {
 			return;
 		}
@@ -191,7 +231,7 @@ __global__ __launch_bounds__(64, 4) void k_grid_discharge_wave(GridView gv, cons
 			open[k] = o;
 			lab[k] = ((valid >> k) & 1u) && rt[k] > 0 ? 1 : kLabelInf;
 		}
-		wave_relax<kTX, kTY, kTZ>(lab, open, valid, hx, hy, hzlo, hzhi, lx, ly);
+		wave_relax<kTX, kTY, kTZ>(lab, open, hx, hy, hzlo, hzhi, lx, ly);
 
 		// ---- admissible directions (open, neighbour one level lower), the voxels holding excess
 		// and the top level holding excess
@@ -447,7 +487,7 @@ __global__ __launch_bounds__(64) void k_grid_bfs_wave(GridView gv, const int32_t
 	const int32_t hzlo = (first_mode != 1 && in_xy && oz > 0) ? tile_at(g.h, g, oz, -1, xy, sxy) : kLabelInf;
 	const int32_t hzhi = (first_mode != 1 && in_xy && oz + TZ < g.nz) ? tile_at(g.h, g, oz, TZ, xy, sxy) : kLabelInf;
 
-	wave_relax<kTX, kTY, kTZ>(lab, open, valid, hx, hy, hzlo, hzhi, lx, ly);
+	wave_relax<kTX, kTY, kTZ>(lab, open, hx, hy, hzlo, hzhi, lx, ly);
 
 	bool active = false;
 	int top = 0, bottom = kLabelInf;
