@@ -841,7 +841,9 @@ void GridSolver::launch_bfs_pass(const int32_t *list, const int32_t *count, int 
 			using S = decltype(tag);
 			if constexpr (S::conn == 6 && S::tx * S::ty == 64) {
 				// one-wave 6C tiles: the tile in registers
-				hipLaunchKernelGGL((k_grid_bfs_wave<S::tx, S::ty, S::tz>), dim3(grid), dim3(64), 0, mStream, g, list,
+				hipLaunchKernelGGL((k_grid_bfs_wave<S::tx, S::ty, S::tz>),
+				                   dim3((grid + kPassTilesPerBlock - 1) / kPassTilesPerBlock), dim3(64 * kPassTilesPerBlock),
+				                   0, mStream, g, list,
 				                   count, threshold, first_mode, mDirtyA.get(), mActive.get(), mTileWork.get());
 				return;
 			}
@@ -864,7 +866,9 @@ void GridSolver::launch_closure_pass(const int32_t *list, const int32_t *count, 
 			using S = decltype(tag);
 			if constexpr (S::conn == 6 && S::tx * S::ty == 64) {
 				// one-wave 6C tiles: the tile in registers
-				hipLaunchKernelGGL((k_grid_closure_wave<S::tx, S::ty, S::tz>), dim3(grid), dim3(64), 0, mStream, g, list,
+				hipLaunchKernelGGL((k_grid_closure_wave<S::tx, S::ty, S::tz>),
+				                   dim3((grid + kPassTilesPerBlock - 1) / kPassTilesPerBlock), dim3(64 * kPassTilesPerBlock),
+				                   0, mStream, g, list,
 				                   count, first ? 1 : 0, mDirtyB.get(), reached, ctl);
 				return;
 			}
@@ -888,7 +892,10 @@ void GridSolver::launch_discharge(int max_iters, bool wide)
 			using S = decltype(tag);
 			if constexpr (S::conn == 6 && S::tx * S::ty == 64 && S::tz <= 4) {
 				// one-wave 6C tiles: the whole tile state in registers (int32 residuals, any width)
-				hipLaunchKernelGGL((k_grid_discharge_wave<S::tx, S::ty, S::tz>), dim3(blocks), dim3(64), 0, mStream, g,
+				hipLaunchKernelGGL((k_grid_discharge_wave<S::tx, S::ty, S::tz>),
+				                   dim3((blocks + kDischargeTilesPerBlock - 1) / kDischargeTilesPerBlock),
+				                   dim3(64 * kDischargeTilesPerBlock),
+				                   0, mStream, g,
 				                   mList.get(), mListCount, mDischargeStride, max_iters, mActive.get());
 				return;
 			}

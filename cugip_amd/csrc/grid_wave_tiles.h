@@ -148,8 +148,21 @@ __device__ inline void wave_relax(int32_t (&lab)[kTZ], const uint32_t (&open)[kT
 	}
 }
 
+// Tiles per workgroup of the one-wave kernels: each wave of a workgroup runs its own tile (no LDS, no
+// barriers), the workgroup only packages them. Launches cover every tile of a colour (the list
+// lengths stay on the device), and fewer, larger workgroups dispatch faster than many single waves;
+// the discharge, whose visits differ most in length, keeps its workgroups small.
+constexpr int kDischargeTilesPerBlock = 2, kPassTilesPerBlock = 4;
+
+// Index of this wave's tile in the launch (wave-uniform, so tile addressing stays scalar).
+template <int kPerBlock>
+__device__ inline int wave_tile_index()
+{
+	return (int)blockIdx.x * kPerBlock + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+}
+
 template <int kTX, int kTY, int kTZ>
-__global__ __launch_bounds__(64, 4) void k_grid_discharge_wave(GridView gv, const int32_t *__restrict__ list,
+__global__ __launch_bounds__(64 * kDischargeTilesPerBlock, 4) void k_grid_discharge_wave(GridView gv, const int32_t *__restrict__ list,
                                                             const int32_t *__restrict__ count, int class_stride,
                                                             int max_iters, uint8_t *__restrict__ tile_active)
 {
@@ -158,7 +171,7 @@ __global__ __launch_bounds__(64, 4) void k_grid_discharge_wave(GridView gv, cons
 	using D = Dirs<6>;
 	constexpr int TZ = kTZ;
 	// direction bits: 0 -x, 1 -y, 2 -z, 3 +x, 4 +y, 5 +z
-	const int tile = discharge_tile(list, count, class_stride, (int)blockIdx.x);
+	const int tile = discharge_tile(list, count, class_stride, wave_tile_index<kDischargeTilesPerBlock>());
 	if (tile < 0) {
 		return;
 	}
@@ -167,7 +180,7 @@ __global__ __launch_bounds__(64, 4) void k_grid_discharge_wave(GridView gv, cons
 	int tx, ty, tz;
 	tile_coords(g, local, tx, ty, tz);
 	const int ox = tx * kTX, oy = ty * kTY, oz = tz * kTZ;
-	const int lane = threadIdx.x;
+	const int lane = threadIdx.x & 63;
 	const int lx = lane % kTX, ly = lane / kTX;
 	const int gx = ox + lx, gy = oy + ly;
 	const bool in_xy = gx < g.nx && gy < g.ny;
@@ -425,7 +438,7 @@ __global__ __launch_bounds__(64, 4) void k_grid_discharge_wave(GridView gv, cons
 // refreshed; first_mode 1 / 2 as there) with the tile in registers: own labels and open directions
 // in the lanes, the face lanes holding the halo labels.
 template <int kTX, int kTY, int kTZ>
-__global__ __launch_bounds__(64) void k_grid_bfs_wave(GridView gv, const int32_t *__restrict__ list,
+__global__ __launch_bounds__(64 * kPassTilesPerBlock) void k_grid_bfs_wave(GridView gv, const int32_t *__restrict__ list,
                                                       const int32_t *__restrict__ count, int32_t threshold,
                                                       int first_mode, uint8_t *__restrict__ dirty_next,
                                                       uint8_t *__restrict__ tile_active,
@@ -433,16 +446,17 @@ __global__ __launch_bounds__(64) void k_grid_bfs_wave(GridView gv, const int32_t
 {
 	static_assert(kTX * kTY == 64 && kTX >= 2 && kTY >= 2, "one wave per tile");
 	constexpr int TZ = kTZ;
-	const int tile = list[blockIdx.x];  // in bounds: the grid never exceeds the list's capacity
-	if ((int)blockIdx.x >= *count) {
+	const int b = wave_tile_index<kPassTilesPerBlock>();
+	if (b >= *count) {
 		return;
 	}
+	const int tile = list[b];
 	int local;
 	const GridView g = instance_view(gv, tile, local);
 	int tx, ty, tz;
 	tile_coords(g, local, tx, ty, tz);
 	const int ox = tx * kTX, oy = ty * kTY, oz = tz * kTZ;
-	const int lane = threadIdx.x;
+	const int lane = threadIdx.x & 63;
 	const int lx = lane % kTX, ly = lane / kTX;
 	const int gx = ox + lx, gy = oy + ly;
 	const bool in_xy = gx < g.nx && gy < g.ny;
@@ -548,7 +562,7 @@ __global__ __launch_bounds__(64) void k_grid_bfs_wave(GridView gv, const int32_t
 // link joins) with the tile in registers: a lane's s bits and open directions travel together to
 // its x and y neighbour lanes, the column is swept up and down in registers.
 template <int kTX, int kTY, int kTZ>
-__global__ __launch_bounds__(64) void k_grid_closure_wave(GridView gv, const int32_t *__restrict__ list,
+__global__ __launch_bounds__(64 * kPassTilesPerBlock) void k_grid_closure_wave(GridView gv, const int32_t *__restrict__ list,
                                                           const int32_t *__restrict__ count, int first_pass,
                                                           uint8_t *__restrict__ dirty_next,
                                                           int32_t *__restrict__ reached, const SolveControl *ctl)
@@ -556,16 +570,17 @@ __global__ __launch_bounds__(64) void k_grid_closure_wave(GridView gv, const int
 	static_assert(kTX * kTY == 64 && kTX >= 2 && kTY >= 2, "one wave per tile");
 	constexpr int TZ = kTZ;
 	using D = Dirs<6>;
-	const int tile = list[blockIdx.x];  // in bounds: the grid never exceeds the list's capacity
-	if ((int)blockIdx.x >= *count || !gate_open(ctl, kGateClosure | kGateUnlessReached)) {
+	const int b = wave_tile_index<kPassTilesPerBlock>();
+	if (b >= *count || !gate_open(ctl, kGateClosure | kGateUnlessReached)) {
 		return;
 	}
+	const int tile = list[b];
 	int local;
 	const GridView g = instance_view(gv, tile, local);
 	int tx, ty, tz;
 	tile_coords(g, local, tx, ty, tz);
 	const int ox = tx * kTX, oy = ty * kTY, oz = tz * kTZ;
-	const int lane = threadIdx.x;
+	const int lane = threadIdx.x & 63;
 	const int lx = lane % kTX, ly = lane / kTX;
 	const int gx = ox + lx, gy = oy + ly;
 	const bool in_xy = gx < g.nx && gy < g.ny;
