@@ -223,6 +223,18 @@ __device__ inline uint32_t wave_reduce_dpp(uint32_t v)
 	return __builtin_amdgcn_readlane(v, 63);
 }
 
+// Bitwise OR over the wave (same DPP pattern), wave-uniform.
+__device__ inline uint32_t wave_or_dpp(uint32_t v)
+{
+	v |= __builtin_amdgcn_update_dpp(0u, v, 0x111, 0xf, 0xf, false);
+	v |= __builtin_amdgcn_update_dpp(0u, v, 0x112, 0xf, 0xf, false);
+	v |= __builtin_amdgcn_update_dpp(0u, v, 0x114, 0xf, 0xf, false);
+	v |= __builtin_amdgcn_update_dpp(0u, v, 0x118, 0xf, 0xf, false);
+	v |= __builtin_amdgcn_update_dpp(0u, v, 0x142, 0xa, 0xf, false);
+	v |= __builtin_amdgcn_update_dpp(0u, v, 0x143, 0xc, 0xf, false);
+	return __builtin_amdgcn_readlane(v, 63);
+}
+
 __device__ inline int wave_max_nonneg(int v)
 {
 	return (int)wave_reduce_dpp<true>((uint32_t)v);

@@ -265,8 +265,19 @@ __global__ __launch_bounds__(64 * kDischargeTilesPerBlock, 4) void k_grid_discha
 			a |= xp == l - 1 ? 8u : 0u;
 			a |= yp == l - 1 ? 16u : 0u;
 			a |= zp == l - 1 ? 32u : 0u;
-			adm[k] = (a & open[k]) | 0x40u;  // bit 6: the excess, marked with the directions below
-			const bool h = ((valid >> k) & 1u) && ex[k] > 0 && l < kLabelInf;
+			// the dirty bits a push sweep step of this voxel sets when it sends: its admissible
+			// residuals and its excess (bits 8k + d, 8k + 6), and for the -z / +z directions the
+			// receiving voxel's residual back and excess; bits 8k .. 8k + 5 are the admissible set
+			a &= open[k];
+			uint32_t m = (a | 0x40u) << (8 * k);
+			if (k > 0) {
+				m |= (a & 4u) ? (0x60u << (8 * (k > 0 ? k - 1 : 0))) : 0u;
+			}
+			if (k < TZ - 1) {
+				m |= (a & 32u) ? (0x44u << (8 * (k < TZ - 1 ? k + 1 : 0))) : 0u;
+			}
+			adm[k] = m;
+			const bool h = ex[k] > 0 && l < kLabelInf;  // (voxels outside the volume hold nothing)
 			held |= h ? (1u << k) : 0u;
 			top = h && l > top ? l : top;
 		}
@@ -280,15 +291,27 @@ __global__ __launch_bounds__(64 * kDischargeTilesPerBlock, 4) void k_grid_discha
 		// whose voxels held excess when the sweep began. Only level 1 has open sink links (a label
 		// is 1 exactly where rt > 0, and rt only falls during the visit) and no admissible
 		// direction (no label is 0), so levels >= 2 push along directions and level 1 into t.
+		// The levels that held excess, as a wave-uniform mask: bit j for level top - j (j < 64);
+		// `deep` when some voxel held excess further below top.
+		const int top_level = level;
+		uint32_t held_lo = 0, held_hi = 0;
+		bool deep_lane = false;
+#pragma unroll
+		for (int k = 0; k < TZ; ++k) {
+			const uint32_t j = (uint32_t)(top_level - lab[k]);
+			const bool h = (held >> k) & 1u;
+			held_lo |= (h && j < 32u) ? (1u << (j & 31u)) : 0u;
+			held_hi |= (h && j >= 32u && j < 64u) ? (1u << (j & 31u)) : 0u;
+			deep_lane |= h && j >= 64u;
+		}
+		const uint64_t held_levels =
+		    ((uint64_t)wave_or_dpp(held_hi) << 32) | (uint64_t)wave_or_dpp(held_lo);
+		const bool deep = __any(deep_lane);
 		while (level >= 2) {
-			int next = 0;
-			bool inside = false;
+			bool sent = false;
 #pragma unroll
 			for (int k = 0; k < TZ; ++k) {
-				const bool send = ((valid >> k) & 1u) && lab[k] == level && ex[k] > 0;
-				if (!send && lab[k] < level && lab[k] > next && ((held >> k) & 1u)) {
-					next = lab[k];
-				}
+				const bool send = lab[k] == level && ex[k] > 0;
 				if (!__any(send)) {
 					continue;  // no voxel of this z-level sends: nothing to deliver either
 				}
@@ -297,14 +320,15 @@ __global__ __launch_bounds__(64 * kDischargeTilesPerBlock, 4) void k_grid_discha
 				int32_t f[6];
 #pragma unroll
 				for (int d = 0; d < 6; ++d) {
-					const int32_t fd = ((adm[k] >> d) & 1u) ? (e < r[d][k] ? e : r[d][k]) : 0;
+					const int32_t fd = ((adm[k] >> (8 * k + d)) & 1u) ? (e < r[d][k] ? e : r[d][k]) : 0;
 					r[d][k] -= fd;
 					e -= fd;
 					f[d] = fd;
 				}
 				// a sender's excess and every admissible residual are written back (the directions
-				// after the one that used up its excess unchanged)
-				dirty |= (send ? adm[k] : 0u) << (8 * k);
+				// after the one that used up its excess unchanged), with its z receivers' bits
+				dirty |= send ? adm[k] : 0u;
+				sent = true;
 				// deliveries: the receiver's excess and its residual back towards the sender
 				const int32_t from_xm = lane_xm<kTX>(f[3], 0, lx);  // sent +x by lane - 1
 				const int32_t from_xp = lane_xp<kTX>(f[0], 0, lx);  // sent -x by lane + 1
@@ -318,25 +342,37 @@ __global__ __launch_bounds__(64 * kDischargeTilesPerBlock, 4) void k_grid_discha
 				r[4][k] += from_yp;
 				// a receiver's excess and its four x and y residuals
 				dirty |= got > 0 ? (0x5bu << (8 * k)) : 0u;
-				inside |= got > 0;  // every push inside the tile is received by a lane or the column
 				// z: the column's own next voxels (pushes across the z faces are counted at the end)
 				if (k > 0) {
 					ex[k > 0 ? k - 1 : 0] += f[2];
 					r[5][k > 0 ? k - 1 : 0] += f[2];
-					dirty |= f[2] > 0 ? (1u << (8 * (k - 1) + 5)) | (1u << (8 * (k - 1) + 6)) : 0u;
-					inside |= f[2] > 0;
 				}
 				if (k < TZ - 1) {
 					ex[k < TZ - 1 ? k + 1 : 0] += f[5];
 					r[2][k < TZ - 1 ? k + 1 : 0] += f[5];
-					dirty |= f[5] > 0 ? (1u << (8 * (k + 1) + 2)) | (1u << (8 * (k + 1) + 6)) : 0u;
-					inside |= f[5] > 0;
 				}
 			}
-			if (__any(inside)) {
-				next = level - 1 > next ? level - 1 : next;
+			// after a step with senders, level - 1 is next (it may have received; when all the flow
+			// left the tile the step there finds only the voxels that held excess already), else
+			// the highest level below that held excess
+			if (sent) {
+				level = level - 1;
+				continue;
 			}
-			level = wave_max_nonneg(next);
+			const int j0 = top_level - level + 1;
+			const uint64_t below = j0 < 64 ? held_levels >> j0 : 0ull;
+			if (below) {
+				level = level - 1 - __builtin_ctzll(below);
+			} else if (deep) {
+				int next = 0;
+#pragma unroll
+				for (int k = 0; k < TZ; ++k) {
+					next = ((held >> k) & 1u) && lab[k] < level && lab[k] > next ? lab[k] : next;
+				}
+				level = wave_max_nonneg(next);
+			} else {
+				level = 0;
+			}
 		}
 		if (level == 1) {
 #pragma unroll
