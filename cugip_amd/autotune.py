@@ -70,15 +70,18 @@ def _solver(grids, conn, batch):
     return h, h.solve
 
 
-def tune_workload(name, repeats=3, log=print):
+def tune_workload(name, repeats=3, log=print, shapes=None):
+    """Times every (tile shape, phases, iterations) candidate on the workload's sample grids;
+    shapes (a list of (tx, ty, tz)) restricts the tile shapes to those of the connectivity's list."""
     conn, batch, max_voxels, build = WORKLOADS[name]
+    candidates = [s for s in SHAPES[conn] if shapes is None or tuple(s) in shapes]
     grids = build()
     voxels = sum(int(g[0].size) for g in grids)
     handle, solve = _solver(grids, conn, batch)
     results = []
     reference = None
     try:
-        for shape, phases, iters in itertools.product(SHAPES[conn], PHASES[conn], ITERATIONS):
+        for shape, phases, iters in itertools.product(candidates, PHASES[conn], ITERATIONS):
             handle.set_policy(tile_x=shape[0], tile_y=shape[1], tile_z=shape[2], discharges_per_relabel=phases,
                               max_tile_iterations=iters)
             flows = solve()
@@ -144,13 +147,17 @@ def main(argv=None):
     ap.add_argument("--out", default=None, help="write all timings as JSON")
     ap.add_argument("--write-header", action="store_true", help="rewrite csrc/tuned_policy.h")
     ap.add_argument("--from-json", default=None, help="take the timings of an earlier --out file")
+    ap.add_argument("--shapes", default=None, help="only these tile shapes, e.g. 16x4x4,32x2x4")
     args = ap.parse_args(argv)
+    shapes = None
+    if args.shapes:
+        shapes = {tuple(int(v) for v in s.split("x")) for s in args.shapes.split(",")}
     tuned = []
     if args.from_json:
         with open(args.from_json) as f:
             tuned = json.load(f)
     for name in ([] if args.from_json else args.workloads.split(",")):
-        tuned.append(tune_workload(name, args.repeats, log=lambda *a: print(*a, flush=True)))
+        tuned.append(tune_workload(name, args.repeats, log=lambda *a: print(*a, flush=True), shapes=shapes))
         b = tuned[-1]["best"]
         print("best %s: %s" % (name, json.dumps(b)), flush=True)
     if args.out:
