@@ -774,10 +774,13 @@ int64_t GridSolver::totals_from_readback(unsigned long long *visits)
 // Returns the number of pass launches; `also` receives the flag count of also_count (if given)
 // at the final check.
 template <typename Pass>
-int GridSolver::colour_sweeps(uint8_t *dirty, uint8_t *also_count, int &also, Pass &&pass, bool *stop, bool defer)
+int GridSolver::colour_sweeps(uint8_t *dirty, uint8_t *also_count, int &also, Pass &&pass, bool *stop, bool defer,
+                              int first_cycles)
 {
 	constexpr int kCyclesPerCheck = 2;
 	int launches = 0;
+	mSweepChecks = 0;
+	mSweepCycles = 0;
 	const ColourSet &cs = mSweep;
 	for (int c = 0; c < cs.colours; ++c) {
 		if (cs.size[c] == 0) {
@@ -798,7 +801,7 @@ int GridSolver::colour_sweeps(uint8_t *dirty, uint8_t *also_count, int &also, Pa
 		}
 	}
 	// the first sweep always leaves dirty tiles (the first colour ran before its neighbours), so the
-	// first cycles follow it without a check (one host wait fewer)
+	// first cycles (first_cycles of them) follow it without a check (one host wait fewer)
 	for (bool check = false;; check = true) {
 		if (check) {
 			int pending = 0;
@@ -819,13 +822,15 @@ int GridSolver::colour_sweeps(uint8_t *dirty, uint8_t *also_count, int &also, Pa
 				return launches;
 			}
 		}
-		for (int cycle = 0; cycle < kCyclesPerCheck; ++cycle) {
+		const int cycles = check ? kCyclesPerCheck : first_cycles;
+		for (int cycle = 0; cycle < cycles; ++cycle) {
 			for (int c = 0; c < cs.colours; ++c) {
 				launch_compact(cs, dirty, c, true);
 				pass(mList.get(), mListCount, cs.max_size, 0);
 				++launches;
 			}
 		}
+		mSweepCycles += cycles;
 	}
 }
 
@@ -937,7 +942,11 @@ int GridSolver::run_global_relabel(int32_t threshold, cgp_stats &stats, bool def
 	    [&](const int32_t *list, const int32_t *count, int grid, int first) {
 		    launch_bfs_pass(list, count, grid, threshold, first ? grid * tile_voxels : -1, mConn == 6 ? first : 0);
 	    },
-	    nullptr, defer);
+	    nullptr, defer, mRelabelCycles);
+	if (mSweepChecks > 1) {
+		// the sweeps ran past their first check: start the next relabel's first check that late
+		mRelabelCycles = std::min(mSweepCycles, (int)kMaxRelabelCycles);
+	}
 	mDirtyAClean = !mCheckDeferred;  // a deferred check may still find dirty tiles
 	return active;
 }
@@ -1317,6 +1326,9 @@ int64_t GridSolver::solve(const cgp_policy &policy, cgp_stats *stats_out, const 
 			active = counts[2];
 			mDirtyAClean = pending == 0;
 			mRelabelOneCheck = pending == 0;
+			if (pending > 0) {
+				mRelabelCycles = std::min(mRelabelCycles + 1, (int)kMaxRelabelCycles);
+			}
 			if (!reached) {
 				// (with nothing active after an exact relabel, the labels are exact too)
 				mClosedByCheck = true;

@@ -131,9 +131,14 @@ private:
 	bool mCheckDeferred = false;      // a count_two was queued without its host wait
 	bool mRelabelOneCheck = true;     // the last relabel ended at its first check (the deferral pays)
 	int mSweepChecks = 0;             // host checks of the last colour sweeps
+	int mSweepCycles = 0;             // colour cycles (after the first sweep) of the last colour sweeps
+	// cycles of a relabel's colour sweeps before their first check: what the last relabels needed
+	// (tiles two rows high carry labels across y in twice the passes of four-row tiles)
+	int mRelabelCycles = 2;
+	static constexpr int kMaxRelabelCycles = 6;
 	template <typename Pass>
 	int colour_sweeps(uint8_t *dirty, uint8_t *also_count, int &also, Pass &&pass, bool *stop = nullptr,
-	                  bool defer = false);
+	                  bool defer = false, int first_cycles = 2);
 	void run_discharge_phase(int colour, int max_iters);
 	// check: also report whether the closure reached a voxel with a residual sink link (the
 	// preflow is then not maximum and the closure is not S_min); it stops early when it does
