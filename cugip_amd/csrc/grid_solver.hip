@@ -271,6 +271,8 @@ void GridSolver::allocate_grids(int conn, const std::vector<InstanceDesc> &inst,
 	// read-backs: flows + visit counters, or the control block (pageable host memory in arena mode)
 	mHostScratch.allocate(kCtlTotals + 3 + count, mArena != nullptr);
 	++mGeneration;
+	mRelabelCycles = 2;
+	mRelabelFits = 0;
 }
 
 void GridSolver::set_batch(int conn, const GridSpec *specs, int count, bool device_pointers)
@@ -477,6 +479,8 @@ void GridSolver::configure_tiles(const Shape &shape)
 	mTilesReady = false;
 	++mGeneration;
 	mShape = shape;
+	mRelabelCycles = 2;
+	mRelabelFits = 0;
 	int64_t tiles = 0;
 	for (InstanceDesc &d : mInst) {
 		d.ntx = (d.nx + shape.tx - 1) / shape.tx;
@@ -943,12 +947,26 @@ int GridSolver::run_global_relabel(int32_t threshold, cgp_stats &stats, bool def
 		    launch_bfs_pass(list, count, grid, threshold, first ? grid * tile_voxels : -1, mConn == 6 ? first : 0);
 	    },
 	    nullptr, defer, mRelabelCycles);
-	if (mSweepChecks > 1) {
-		// the sweeps ran past their first check: start the next relabel's first check that late
-		mRelabelCycles = std::min(mSweepCycles, (int)kMaxRelabelCycles);
+	if (!mCheckDeferred) {
+		learn_relabel_cycles(mSweepChecks > 1 ? mSweepCycles : 0);
 	}
 	mDirtyAClean = !mCheckDeferred;  // a deferred check may still find dirty tiles
 	return active;
+}
+
+// Feedback from a relabel's checks: `needed` colour cycles when it ran past its first check (the
+// next relabels check that late), 0 when it ended at its first check.
+void GridSolver::learn_relabel_cycles(int needed)
+{
+	if (needed > 0) {
+		mRelabelCycles = std::min(needed, (int)kMaxRelabelCycles);
+		mRelabelFits = 0;
+		return;
+	}
+	if (mRelabelCycles > 2 && ++mRelabelFits >= 8) {
+		--mRelabelCycles;
+		mRelabelFits = 0;
+	}
 }
 
 void GridSolver::run_discharge_phase(int colour, int max_iters)
@@ -1326,9 +1344,7 @@ int64_t GridSolver::solve(const cgp_policy &policy, cgp_stats *stats_out, const 
 			active = counts[2];
 			mDirtyAClean = pending == 0;
 			mRelabelOneCheck = pending == 0;
-			if (pending > 0) {
-				mRelabelCycles = std::min(mRelabelCycles + 1, (int)kMaxRelabelCycles);
-			}
+			learn_relabel_cycles(pending > 0 ? mRelabelCycles + 1 : 0);
 			if (!reached) {
 				// (with nothing active after an exact relabel, the labels are exact too)
 				mClosedByCheck = true;

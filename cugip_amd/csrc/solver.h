@@ -133,9 +133,11 @@ private:
 	int mSweepChecks = 0;             // host checks of the last colour sweeps
 	int mSweepCycles = 0;             // colour cycles (after the first sweep) of the last colour sweeps
 	// cycles of a relabel's colour sweeps before their first check: what the last relabels needed
-	// (tiles two rows high carry labels across y in twice the passes of four-row tiles)
-	int mRelabelCycles = 2;
+	// (tiles two rows high carry labels across y in twice the passes of four-row tiles), one fewer
+	// after 8 relabels in a row ended at their first check; back to 2 for a new grid or tile shape
+	int mRelabelCycles = 2, mRelabelFits = 0;
 	static constexpr int kMaxRelabelCycles = 6;
+	void learn_relabel_cycles(int needed);
 	template <typename Pass>
 	int colour_sweeps(uint8_t *dirty, uint8_t *also_count, int &also, Pass &&pass, bool *stop = nullptr,
 	                  bool defer = false, int first_cycles = 2);

@@ -18,9 +18,9 @@ struct TunedPolicy {
 
 static const TunedPolicy kTunedPolicies[] = {
 	// conn batch max_voxels  tile        phases iters
-	{6, 0, 2097152LL, 16, 4, 4, 6, 8},  // grid6_small: 1.01 ms
-	{6, 0, INT64_MAX, 32, 2, 4, 8, 32},  // grid6_large: 3.32 ms
-	{6, 1, INT64_MAX, 16, 4, 4, 8, 32},  // batch6: 2.39 ms
+	{6, 0, 2097152LL, 16, 4, 4, 6, 8},  // grid6_small: 1.00 ms
+	{6, 0, INT64_MAX, 32, 2, 4, 8, 32},  // grid6_large: 3.15 ms
+	{6, 1, INT64_MAX, 32, 2, 4, 8, 8},  // batch6: 2.36 ms
 	{26, 0, 4194304LL, 32, 4, 4, 24, 2},  // grid26_small: 11.44 ms
 	{26, 0, INT64_MAX, 32, 4, 4, 24, 2},  // grid26: 15.96 ms
 	{26, 1, INT64_MAX, 16, 8, 4, 24, 1},  // batch26: 4.88 ms
