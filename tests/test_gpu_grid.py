@@ -190,6 +190,28 @@ def test_golden_multi_workgroup_lists(golden, conn, policy):
     assert digest(smin) == rec["smin_sha256"] and digest(tmin) == rec["tmin_sha256"]
 
 
+def test_golden_repeated_solves_across_tile_shapes(golden):
+    """Repeated solves of the 256^3 golden instance on one handle while the tile shape changes
+    (16x4x4, 32x2x4, 16x4x4): the relabel learns from each solve how many colour cycles come
+    before its first host check (two-row tiles need more) and starts over for a new shape; every
+    solve must still give the fixture's flow, S_min and T_min bit for bit."""
+    rec = [r for r in golden["instances"] if r["conn"] == 6 and r["dims"][0] >= 256][0]
+    nx, ny, nz = rec["dims"]
+    vol, cs, ct, cn = cugip_amd.make_instance(nx, ny, nz, rec["seed"], rec["instance"], 6)
+    with cugip_amd.GridGraph() as g:
+        g.set_grid(cs, ct, cn)
+        for shape in [(16, 4, 4), (32, 2, 4), (16, 4, 4)]:
+            g.set_policy(tile_x=shape[0], tile_y=shape[1], tile_z=shape[2])
+            for rep in range(3):
+                say(f"golden 256^3 tile {shape} solve {rep}")
+                flow = g.max_flow()
+                stats = g.stats.as_dict()
+                assert (stats["tile_x"], stats["tile_y"], stats["tile_z"]) == shape
+                assert flow == rec["flow"]
+                assert digest(g.source_set()) == rec["smin_sha256"]
+                assert digest(g.fill_segments(1, 0)) == rec["tmin_sha256"]
+
+
 def test_full_size_properties():
     """256^3: the S_min and T_min cuts both have capacity equal to the flow, S_min and T_min are
     disjoint, and repeated solves on one handle agree bit for bit."""
