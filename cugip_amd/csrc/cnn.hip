@@ -2,7 +2,8 @@
 // v_mfma_f32_16x16x16_bf16 (BF16 operands, FP32 accumulation), their backward passes, Adam.
 //
 // One workgroup (4 waves) owns an output brick of 16 x 4 x 4 voxels and stages its input brick
-// with a one-voxel halo (18 x 6 x 6 voxels, channel-last BF16) in LDS. A wave computes one row
+// with a one-voxel halo (18 x 6 x 6 voxels, BF16, one LDS plane per group of 4 channels) in LDS. A
+// wave computes one row
 // of 16 voxels at a time as D = A x B per tap: A = the 16 x 16 weight block of the tap (held in
 // registers for all taps), B = the 16 channels of the 16 voxels shifted by the tap offset (from
 // LDS), D = 16 output channels x 16 voxels. In the MFMA's lane layout, lane l holds voxel l % 16
@@ -131,6 +132,28 @@ __device__ inline void stage_brick(uint16_t *lds, const uint16_t *__restrict__ i
 	}
 }
 
+// Stage the input brick with its halo as one plane per group of 4 channels (8 bytes a voxel), so
+// that the 16 lanes of an MFMA B fragment, which read one channel group of 16 consecutive voxels,
+// read 128 contiguous bytes (channel-last, their 8-byte reads would stride 32 bytes and collide in
+// the LDS banks). Zero outside the volume.
+template <int CIN>
+__device__ inline void stage_brick_planar(uint16_t *lds, const uint16_t *__restrict__ in, int nx, int ny, int nz,
+                                          int bx, int by, int bz)
+{
+	constexpr int kParts = CIN / 4;
+	for (int e = threadIdx.x; e < HV * kParts; e += blockDim.x) {
+		const int hv = e / kParts, part = e % kParts;
+		const int hx = hv % HX, hy = (hv / HX) % HY, hz = hv / (HX * HY);
+		const int x = bx + hx - 1, y = by + hy - 1, z = bz + hz - 1;
+		uint2 val = make_uint2(0u, 0u);
+		if (x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz) {
+			const int64_t v = ((int64_t)z * ny + y) * nx + x;
+			val = *reinterpret_cast<const uint2 *>(in + v * CIN + part * 4);
+		}
+		*reinterpret_cast<uint2 *>(lds + (part * HV + hv) * 4) = val;
+	}
+}
+
 // Convolution layer. Forward (BWD false): out = ReLU(conv(in, W) + b) in bf16; with HEAD, also
 // logit = w_head . out + b_head (the sigmoid head fused into the last layer). Backward data (BWD
 // true, CIN 16): out = conv^T(g, W) masked by (mask_act > 0), the gradient at the pre-activation
@@ -143,7 +166,7 @@ __global__ __launch_bounds__(256) void k_conv(int nx, int ny, int nz, const uint
 {
 	__shared__ __attribute__((aligned(16))) uint16_t lds[HV * CIN];
 	const int bx = blockIdx.x * BX, by = blockIdx.y * BY, bz = blockIdx.z * BZ;
-	stage_brick<CIN>(lds, in, nx, ny, nz, bx, by, bz);
+	stage_brick_planar<CIN>(lds, in, nx, ny, nz, bx, by, bz);
 	constexpr int NM = CIN == 16 ? 27 : 7;
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 	const int col = lane & 15, grp = lane >> 4;
@@ -182,7 +205,7 @@ __global__ __launch_bounds__(256) void k_conv(int nx, int ny, int nz, const uint
 				c0 = 0;
 			}
 			const int hv = ((lz + 1 + tap_dz(t)) * HY + (ly + 1 + tap_dy(t))) * HX + (col + 1 + tap_dx(t));
-			const short4_t b = *reinterpret_cast<const short4_t *>(lds + hv * CIN + c0);
+			const short4_t b = *reinterpret_cast<const short4_t *>(lds + ((c0 / 4) * HV + hv) * 4);
 			acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[m], b, acc, 0, 0, 0);
 		}
 		const int x = bx + col, y = by + ly, z = bz + lz;
