@@ -1962,9 +1962,12 @@ __global__ __launch_bounds__(256) void k_closure_reset(uint8_t *__restrict__ s, 
 // workgroup adds its counts into acc[0], acc[1] and takes a ticket in acc[2]; the last one stores
 // the totals in out[0], out[1] and clears acc for the next launch (acc starts at zero).
 __global__ __launch_bounds__(1024) void k_count_flags2(const uint8_t *__restrict__ a, const uint8_t *__restrict__ b,
-                                                       int n, int32_t *__restrict__ out, int32_t *__restrict__ acc)
+                                                       int n, int32_t *__restrict__ out, int32_t *__restrict__ acc,
+                                                       const unsigned long long *__restrict__ slots, int64_t nsum,
+                                                       unsigned long long *__restrict__ total)
 {
 	__shared__ int sums[2][1024 / 64];
+	__shared__ int last;
 	const int i = ((int)blockIdx.x * 1024 + (int)threadIdx.x) * 4;
 	int ca = 0, cb = 0;
 	if (i < n) {
@@ -2001,11 +2004,28 @@ __global__ __launch_bounds__(1024) void k_count_flags2(const uint8_t *__restrict
 			atomicAdd(&acc[1], tb);
 		}
 		__threadfence();
+		last = 0;
 		if (atomicAdd(&acc[2], 1) == (int)gridDim.x - 1) {
 			__threadfence();
 			out[0] = atomicExch(&acc[0], 0);
 			out[1] = atomicExch(&acc[1], 0);
 			atomicExch(&acc[2], 0);
+			last = 1;
+		}
+	}
+	if (slots) {
+		// the last block also sums the slotted visit counters and flows (k_sum_slots' work; earlier
+		// launches wrote them)
+		__syncthreads();
+		if (last) {
+			for (int64_t i = threadIdx.x; i < nsum; i += blockDim.x) {
+				unsigned long long sum = 0;
+#pragma unroll
+				for (int j = 0; j < kSlots; ++j) {
+					sum += slots[i * kSlotWords + j * kSlotStride];
+				}
+				total[i] = sum;
+			}
 		}
 	}
 }

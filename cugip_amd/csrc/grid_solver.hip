@@ -709,17 +709,22 @@ void GridSolver::count_two(uint8_t *a, uint8_t *b, int &na, int &nb, bool defer)
 			                   mCount.get() + 2, (const uint8_t *)nullptr, (const SolveControl *)nullptr, 0u);
 		});
 	} else {
+		// with the totals prefetched, the count's last workgroup also sums the slots
 		launch(kCompact, 0, 0.0, [&] {
 			const int blocks = (mTiles + kCompactTilesPerGroup - 1) / kCompactTilesPerGroup;
 			hipLaunchKernelGGL(k_count_flags2, dim3(blocks), dim3(1024), 0, mStream, (const uint8_t *)a,
-			                   (const uint8_t *)b, mTiles, mCount.get() + 1, mCursors.get() + kCountAcc);
+			                   (const uint8_t *)b, mTiles, mCount.get() + 1, mCursors.get() + kCountAcc,
+			                   mPrefetchTotals ? (const unsigned long long *)mSlots.get() : nullptr, (int64_t)(3 + n),
+			                   mCounterTotal.get());
 		});
 	}
 	size_t bytes = 4 * sizeof(int32_t);
 	if (mPrefetchTotals) {
-		hipLaunchKernelGGL(k_sum_slots, dim3(blocks_for((int64_t)(3 + n), 256)), dim3(256), 0, mStream, mSlots.get(),
-		                   (int64_t)(3 + n), mCounterTotal.get());
-		CGP_LAUNCH_CHECK();
+		if (mTiles <= kCompactBlockTiles) {
+			hipLaunchKernelGGL(k_sum_slots, dim3(blocks_for((int64_t)(3 + n), 256)), dim3(256), 0, mStream,
+			                   mSlots.get(), (int64_t)(3 + n), mCounterTotal.get());
+			CGP_LAUNCH_CHECK();
+		}
 		bytes = (kReadbackHead + n) * sizeof(unsigned long long);
 	}
 	if (defer) {
