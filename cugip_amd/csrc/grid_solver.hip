@@ -259,16 +259,18 @@ void GridSolver::allocate_grids(int conn, const std::vector<InstanceDesc> &inst,
 	mSlots.allocate((3 + count) * kSlotWords, mArena);
 	mCounters.borrow(mSlots.get(), 3 * kSlotWords);
 	mFlow.borrow(mSlots.get() + 3 * kSlotWords, count * kSlotWords);
-	mStatus.allocate(1, mArena);
 	// one region read back whole by the closure's checks: the counts (int32 [0] list count, [1] [2]
-	// count_two, [3] closure reached t), then the 3 visit-counter totals and the per-grid flow totals
+	// count_two, [3] closure reached t), the init status (int32 [4]), then the 3 visit-counter totals
+	// and the per-grid flow totals; every count read brings the status along
 	mReadback.allocate(kReadbackHead + count, mArena);
 	mCount.borrow(reinterpret_cast<int32_t *>(mReadback.get()), 4);
-	mCounterTotal.borrow(mReadback.get() + 2, 3);
+	mStatus.borrow(reinterpret_cast<int32_t *>(mReadback.get()) + 4, 1);
+	mCounterTotal.borrow(mReadback.get() + 3, 3);
 	mFlowTotal.borrow(mReadback.get() + kReadbackHead, count);
 	// control block of the device-driven solve, then its visit-counter and per-grid flow totals
 	mCtl.allocate(kCtlTotals + 3 + count, mArena);
 	// read-backs: flows + visit counters, or the control block (pageable host memory in arena mode)
+	static_assert(kCtlTotals >= 8, "the host scratch also holds the deferred counts after the read-back");
 	mHostScratch.allocate(kCtlTotals + 3 + count, mArena != nullptr);
 	++mGeneration;
 	mRelabelCycles = 2;
@@ -698,10 +700,6 @@ void GridSolver::count_two(uint8_t *a, uint8_t *b, int &na, int &nb, bool defer)
 {
 	int32_t *pinned = reinterpret_cast<int32_t *>(mHostScratch.get());
 	const size_t n = mInst.size();
-	int32_t *status_copy = pinned + 2 * (kReadbackHead + n);
-	if (mStatusPending) {
-		CGP_HIP(hipMemcpyAsync(status_copy, mStatus.get(), sizeof(int32_t), hipMemcpyDeviceToHost, mStream));
-	}
 	if (mTiles <= kCompactBlockTiles) {
 		launch(kCompact, 0, 0.0, [&] {
 			hipLaunchKernelGGL(k_compact_block<1>, dim3(1), dim3(kCompactThreads), 0, mStream, a, mTiles,
@@ -718,7 +716,7 @@ void GridSolver::count_two(uint8_t *a, uint8_t *b, int &na, int &nb, bool defer)
 			                   mCounterTotal.get());
 		});
 	}
-	size_t bytes = 4 * sizeof(int32_t);
+	size_t bytes = 6 * sizeof(int32_t);  // the counts and the init status
 	if (mPrefetchTotals) {
 		if (mTiles <= kCompactBlockTiles) {
 			hipLaunchKernelGGL(k_sum_slots, dim3(blocks_for((int64_t)(3 + n), 256)), dim3(256), 0, mStream,
@@ -753,7 +751,7 @@ void GridSolver::count_two_wait(int &na, int &nb, bool has_b)
 	mCheckDeferred = false;
 	mTotalsRead = mPrefetchTotals;
 	if (mStatusPending) {
-		apply_status(pinned[2 * (kReadbackHead + mInst.size())]);
+		apply_status(pinned[4]);
 	}
 	na = pinned[1];
 	nb = has_b ? pinned[2] : 0;
@@ -765,7 +763,7 @@ int64_t GridSolver::totals_from_readback(unsigned long long *visits)
 {
 	const unsigned long long *words = reinterpret_cast<const unsigned long long *>(mHostScratch.get());
 	for (int k = 0; k < 3; ++k) {
-		visits[k] = words[2 + k];
+		visits[k] = words[3 + k];
 	}
 	int64_t total = 0;
 	for (size_t i = 0; i < mInst.size(); ++i) {

@@ -228,7 +228,8 @@ private:
 	const int32_t *mListCount = nullptr;    // count (or class cursors) of the last compacted list
 	int mDischargeStride = 0;               // 0: the last list is contiguous; else its class stride
 	DeviceBuffer<unsigned long long> mReadback;  // counts, visit-counter and flow totals (one read)
-	static constexpr int kReadbackHead = 5;      // 2 words of counts, 3 of visit counters
+	// 2 words of counts (int32 [0..3]), the init status (int32 [4]), 3 words of visit counters
+	static constexpr int kReadbackHead = 6;
 	bool mPrefetchTotals = false;  // closure checks also read the flow and visit-counter totals
 	bool mTotalsRead = false;      // the last check did (and nothing ran since)
 	int64_t totals_from_readback(unsigned long long *visits);
