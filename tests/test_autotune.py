@@ -48,3 +48,23 @@ def test_candidates_cover_compiled_shapes():
     for conn, shapes in autotune.SHAPES.items():
         for tx, ty, tz in shapes:
             assert "F(%d, %d, %d, %d)" % (conn, tx, ty, tz) in src
+
+
+def test_shapes_filter_restricts_candidates(monkeypatch):
+    """--shapes keeps only the listed tile shapes of the workload's connectivity (the one-wave
+    retune times 2 shapes × phases × iterations instead of every compiled shape)."""
+    seen = []
+
+    class FakeHandle:
+        def set_policy(self, tile_x, tile_y, tile_z, discharges_per_relabel, max_tile_iterations):
+            seen.append((tile_x, tile_y, tile_z))
+
+        def close(self):
+            pass
+
+    monkeypatch.setitem(autotune.WORKLOADS, "fake", (6, False, None, lambda: []))
+    monkeypatch.setattr(autotune, "_solver", lambda grids, conn, batch: (FakeHandle(), lambda: [1]))
+    res = autotune.tune_workload("fake", repeats=1, log=lambda *a: None, shapes={(16, 4, 4), (32, 2, 4)})
+    assert set(seen) == {(16, 4, 4), (32, 2, 4)}
+    assert len(res["candidates"]) == 2 * len(autotune.PHASES[6]) * len(autotune.ITERATIONS)
+    assert {tuple(c["tile"]) for c in res["candidates"]} == {(16, 4, 4), (32, 2, 4)}
