@@ -7,7 +7,9 @@
 // of 16 voxels at a time as D = A x B per tap: A = the 16 x 16 weight block of the tap (held in
 // registers for all taps), B = the 16 channels of the 16 voxels shifted by the tap offset (from
 // LDS), D = 16 output channels x 16 voxels. In the MFMA's lane layout, lane l holds voxel l % 16
-// and channels 4 * (l / 16) .. + 3 of D, so each lane stores 8 contiguous bytes.
+// and channels 4 * (l / 16) .. + 3 of D, so each lane stores 8 contiguous bytes. Inference
+// (k_conv_fwd_z) walks several bricks along z per workgroup and keeps the input planes in an LDS
+// ring, staging only the planes each brick adds.
 #include <cmath>
 #include <cstring>
 
@@ -247,6 +249,122 @@ __global__ __launch_bounds__(256) void k_conv(int nx, int ny, int nz, const uint
 			*reinterpret_cast<ushort4 *>(out + v * 16 + 4 * grp) = q;
 			if (HEAD && grp == 0) {
 				logits[v] = head_part + head[kCnnChannels];
+			}
+		}
+	}
+}
+
+// Forward layer for inference, marching along z: workgroup (bx, by, bz) computes kZBricks
+// consecutive 16 x 4 x 4 output bricks in z. The input planes stay in an LDS ring of 8 z-planes
+// (one plane per 4-channel group as in k_conv), so each brick stages only the 4 planes it adds
+// instead of its whole 6-plane halo box. Same arithmetic and output as k_conv<CIN, false, HEAD>.
+constexpr int kZBricks = 8;
+constexpr int kZSlots = 8;
+
+template <int CIN>
+__device__ inline void stage_planes(uint16_t *lds, const uint16_t *__restrict__ in, int nx, int ny, int nz, int bx,
+                                    int by, int p0, int np)
+{
+	constexpr int kParts = CIN / 4;
+	constexpr int PL = HX * HY;
+	for (int e = threadIdx.x; e < np * PL * kParts; e += blockDim.x) {
+		const int part = e % kParts, pv = (e / kParts) % PL, pi = e / (kParts * PL);
+		const int hx = pv % HX, hy = pv / HX;
+		const int x = bx + hx - 1, y = by + hy - 1, z = p0 + pi;
+		uint2 val = make_uint2(0u, 0u);
+		if (x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz) {
+			const int64_t v = ((int64_t)z * ny + y) * nx + x;
+			val = *reinterpret_cast<const uint2 *>(in + v * CIN + part * 4);
+		}
+		const int slot = (z + kZSlots) % kZSlots;  // z >= -1
+		*reinterpret_cast<uint2 *>(lds + ((part * kZSlots + slot) * PL + pv) * 4) = val;
+	}
+}
+
+template <int CIN, bool HEAD>
+__global__ __launch_bounds__(256) void k_conv_fwd_z(int nx, int ny, int nz, const uint16_t *__restrict__ in,
+                                                   const uint16_t *__restrict__ pack, const float *__restrict__ bias,
+                                                   uint16_t *__restrict__ out, const float *__restrict__ head,
+                                                   float *__restrict__ logits)
+{
+	constexpr int PL = HX * HY;
+	__shared__ __attribute__((aligned(16))) uint16_t lds[kZSlots * PL * CIN];
+	const int bx = blockIdx.x * BX, by = blockIdx.y * BY, bz0 = blockIdx.z * (kZBricks * BZ);
+	constexpr int NM = CIN == 16 ? 27 : 7;
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const int col = lane & 15, grp = lane >> 4;
+	short4_t a[NM];
+#pragma unroll
+	for (int m = 0; m < NM; ++m) {
+		a[m] = *reinterpret_cast<const short4_t *>(pack + (m * 64 + lane) * 4);
+	}
+	float bias4[4], hw4[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+	for (int j = 0; j < 4; ++j) {
+		bias4[j] = bias[4 * grp + j];
+		if (HEAD) {
+			hw4[j] = head[4 * grp + j];
+		}
+	}
+	stage_planes<CIN>(lds, in, nx, ny, nz, bx, by, bz0 - 1, BZ + 2);
+	for (int zb = 0; zb < kZBricks; ++zb) {
+		const int bz = bz0 + zb * BZ;
+		if (bz >= nz) {
+			break;  // uniform across the workgroup
+		}
+		if (zb > 0) {
+			__syncthreads();  // the previous brick's reads of the slots about to be refilled are done
+			stage_planes<CIN>(lds, in, nx, ny, nz, bx, by, bz + 1, BZ);
+		}
+		__syncthreads();
+		const int lz = wave;
+#pragma unroll 1
+		for (int ly = 0; ly < BY; ++ly) {
+			float4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+			for (int m = 0; m < NM; ++m) {
+				int t, c0;
+				if (CIN == 16) {
+					t = m;
+					c0 = 4 * grp;
+				} else {
+					t = 4 * m + grp;
+					t = t < kCnnTaps ? t : kCnnTaps - 1;  // padded taps have zero weights
+					c0 = 0;
+				}
+				const int slot = (bz + lz + tap_dz(t) + kZSlots) % kZSlots;
+				const int pv = (ly + 1 + tap_dy(t)) * HX + (col + 1 + tap_dx(t));
+				const short4_t b = *reinterpret_cast<const short4_t *>(lds + (((c0 / 4) * kZSlots + slot) * PL + pv) * 4);
+				acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[m], b, acc, 0, 0, 0);
+			}
+			const int x = bx + col, y = by + ly, z = bz + lz;
+			const bool valid = x < nx && y < ny && z < nz;
+			const int64_t v = ((int64_t)z * ny + y) * nx + x;
+			ushort4 q;
+			uint16_t r[4];
+			float head_part = 0.0f;
+#pragma unroll
+			for (int j = 0; j < 4; ++j) {
+				float val = acc[j] + bias4[j];
+				val = val > 0.0f ? val : 0.0f;
+				r[j] = f2bf(val);
+				if (HEAD) {
+					head_part += hw4[j] * bf2f(r[j]);
+				}
+			}
+			q.x = r[0];
+			q.y = r[1];
+			q.z = r[2];
+			q.w = r[3];
+			if (HEAD) {
+				head_part += __shfl_xor(head_part, 16, 64);
+				head_part += __shfl_xor(head_part, 32, 64);
+			}
+			if (valid) {
+				*reinterpret_cast<ushort4 *>(out + v * 16 + 4 * grp) = q;
+				if (HEAD && grp == 0) {
+					logits[v] = head_part + head[kCnnChannels];
+				}
 			}
 		}
 	}
@@ -561,19 +679,19 @@ void CnnModel::forward(GridSolver &grid, const uint8_t *volume, bool device_volu
 		vol = mVolume.get();
 	}
 	hipLaunchKernelGGL(k_cnn_input, dim3(blocks_of(mVox)), dim3(256), 0, mStream, mVox, vol, d.cs, d.ct, mX0.get());
-	const dim3 grid3((mNx + BX - 1) / BX, (mNy + BY - 1) / BY, (mNz + BZ - 1) / BZ);
+	// inference layers march along z, kZBricks bricks a workgroup
+	const dim3 gridz((mNx + BX - 1) / BX, (mNy + BY - 1) / BY, (mNz + BZ * kZBricks - 1) / (BZ * kZBricks));
 	const float *p = mParams.get();
-	hipLaunchKernelGGL((k_conv<4, false, false>), grid3, dim3(256), 0, mStream, mNx, mNy, mNz, mX0.get(),
-	                   mPackFwd.get() + pack_fwd_off(0), p + off_b(0), (const uint16_t *)nullptr, mAct[0].get(),
-	                   (const float *)nullptr, (float *)nullptr);
+	hipLaunchKernelGGL((k_conv_fwd_z<4, false>), gridz, dim3(256), 0, mStream, mNx, mNy, mNz, mX0.get(),
+	                   mPackFwd.get() + pack_fwd_off(0), p + off_b(0), mAct[0].get(), (const float *)nullptr,
+	                   (float *)nullptr);
 	for (int layer = 1; layer <= 2; ++layer) {
-		hipLaunchKernelGGL((k_conv<16, false, false>), grid3, dim3(256), 0, mStream, mNx, mNy, mNz,
+		hipLaunchKernelGGL((k_conv_fwd_z<16, false>), gridz, dim3(256), 0, mStream, mNx, mNy, mNz,
 		                   mAct[layer - 1].get(), mPackFwd.get() + pack_fwd_off(layer), p + off_b(layer),
-		                   (const uint16_t *)nullptr, mAct[layer].get(), (const float *)nullptr, (float *)nullptr);
+		                   mAct[layer].get(), (const float *)nullptr, (float *)nullptr);
 	}
-	hipLaunchKernelGGL((k_conv<16, false, true>), grid3, dim3(256), 0, mStream, mNx, mNy, mNz, mAct[2].get(),
-	                   mPackFwd.get() + pack_fwd_off(3), p + off_b(3), (const uint16_t *)nullptr, mAct[3].get(),
-	                   p + kOffHeadW, mLogit.get());
+	hipLaunchKernelGGL((k_conv_fwd_z<16, true>), gridz, dim3(256), 0, mStream, mNx, mNy, mNz, mAct[2].get(),
+	                   mPackFwd.get() + pack_fwd_off(3), p + off_b(3), mAct[3].get(), p + kOffHeadW, mLogit.get());
 	hipLaunchKernelGGL(k_threshold, dim3(blocks_of(mVox)), dim3(256), 0, mStream, mVox, mLogit.get(), mSide.get());
 	CGP_LAUNCH_CHECK();
 	CGP_HIP(hipStreamSynchronize(mStream));
