@@ -261,25 +261,54 @@ __global__ __launch_bounds__(256) void k_conv(int nx, int ny, int nz, const uint
 constexpr int kZBricks = 8;
 constexpr int kZSlots = 8;
 
+// Per-thread staging map of one input z-plane of the brick box (18 x 6 voxels, CIN / 4 channel
+// groups): thread t stages entries t, t + 256, ...; their slice offsets in global memory and their
+// LDS offsets (without the ring slot) are fixed for the workgroup, so staging a plane is a load and
+// a store per entry.
 template <int CIN>
-__device__ inline void stage_planes(uint16_t *lds, const uint16_t *__restrict__ in, int nx, int ny, int nz, int bx,
-                                    int by, int p0, int np)
-{
-	constexpr int kParts = CIN / 4;
-	constexpr int PL = HX * HY;
-	for (int e = threadIdx.x; e < np * PL * kParts; e += blockDim.x) {
-		const int part = e % kParts, pv = (e / kParts) % PL, pi = e / (kParts * PL);
-		const int hx = pv % HX, hy = pv / HX;
-		const int x = bx + hx - 1, y = by + hy - 1, z = p0 + pi;
-		uint2 val = make_uint2(0u, 0u);
-		if (x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz) {
-			const int64_t v = ((int64_t)z * ny + y) * nx + x;
-			val = *reinterpret_cast<const uint2 *>(in + v * CIN + part * 4);
+struct PlaneMap {
+	static constexpr int kParts = CIN / 4;
+	static constexpr int PL = HX * HY;
+	static constexpr int kPerPlane = PL * kParts;
+	static constexpr int kIters = (kPerPlane + 255) / 256;
+	uint32_t goff[kIters];  // uint16 offset of the entry in a z-slice of `in`
+	uint32_t loff[kIters];  // uint16 offset of the entry in LDS, ring slot 0
+	uint32_t inside = 0;    // bit i: entry i exists and its (x, y) lies in the volume
+	uint32_t exists = 0;    // bit i: entry i exists
+
+	__device__ void init(int nx, int ny, int bx, int by)
+	{
+#pragma unroll
+		for (int i = 0; i < kIters; ++i) {
+			const int e = (int)threadIdx.x + i * 256;
+			const int part = e % kParts, pv = e / kParts;
+			const int hx = pv % HX, hy = pv / HX;
+			const int x = bx + hx - 1, y = by + hy - 1;
+			const bool ex = e < kPerPlane;
+			exists |= ex ? (1u << i) : 0u;
+			inside |= (ex && x >= 0 && x < nx && y >= 0 && y < ny) ? (1u << i) : 0u;
+			goff[i] = (inside >> i) & 1u ? (uint32_t)((y * nx + x) * CIN + part * 4) : 0u;
+			loff[i] = (uint32_t)((part * kZSlots * PL + pv) * 4);
 		}
-		const int slot = (z + kZSlots) % kZSlots;  // z >= -1
-		*reinterpret_cast<uint2 *>(lds + ((part * kZSlots + slot) * PL + pv) * 4) = val;
 	}
-}
+
+	__device__ void stage(uint16_t *lds, const uint16_t *__restrict__ in, int nx, int ny, int nz, int z) const
+	{
+		const bool zin = z >= 0 && z < nz;
+		const uint16_t *slice = in + (zin ? (int64_t)z * nx * ny * CIN : 0);
+		const uint32_t soff = ((uint32_t)(z + kZSlots) & (uint32_t)(kZSlots - 1)) * PL * 4;
+#pragma unroll
+		for (int i = 0; i < kIters; ++i) {
+			if ((exists >> i) & 1u) {
+				uint2 val = make_uint2(0u, 0u);
+				if (zin && ((inside >> i) & 1u)) {
+					val = *reinterpret_cast<const uint2 *>(slice + goff[i]);
+				}
+				*reinterpret_cast<uint2 *>(lds + loff[i] + soff) = val;
+			}
+		}
+	}
+};
 
 template <int CIN, bool HEAD>
 __global__ __launch_bounds__(256) void k_conv_fwd_z(int nx, int ny, int nz, const uint16_t *__restrict__ in,
@@ -287,6 +316,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd_z(int nx, int ny, int nz, cons
                                                    uint16_t *__restrict__ out, const float *__restrict__ head,
                                                    float *__restrict__ logits)
 {
+	static_assert(kZSlots == 8, "ring slots are taken modulo 8");
 	constexpr int PL = HX * HY;
 	__shared__ __attribute__((aligned(16))) uint16_t lds[kZSlots * PL * CIN];
 	const int bx = blockIdx.x * BX, by = blockIdx.y * BY, bz0 = blockIdx.z * (kZBricks * BZ);
@@ -306,7 +336,15 @@ __global__ __launch_bounds__(256) void k_conv_fwd_z(int nx, int ny, int nz, cons
 			hw4[j] = head[4 * grp + j];
 		}
 	}
-	stage_planes<CIN>(lds, in, nx, ny, nz, bx, by, bz0 - 1, BZ + 2);
+	PlaneMap<CIN> map;
+	map.init(nx, ny, bx, by);
+#pragma unroll 1
+	for (int p = -1; p <= BZ; ++p) {
+		map.stage(lds, in, nx, ny, nz, bz0 + p);
+	}
+	// the lane's LDS base: its channel group's planes, its voxel column (x + 1 for the halo)
+	const uint32_t lbase = (uint32_t)((((CIN == 16 ? grp : 0) * kZSlots * PL) + col + 1) * 4);
+	const int lz = wave;
 	for (int zb = 0; zb < kZBricks; ++zb) {
 		const int bz = bz0 + zb * BZ;
 		if (bz >= nz) {
@@ -314,27 +352,36 @@ __global__ __launch_bounds__(256) void k_conv_fwd_z(int nx, int ny, int nz, cons
 		}
 		if (zb > 0) {
 			__syncthreads();  // the previous brick's reads of the slots about to be refilled are done
-			stage_planes<CIN>(lds, in, nx, ny, nz, bx, by, bz + 1, BZ);
+#pragma unroll 1
+			for (int p = 1; p <= BZ; ++p) {
+				map.stage(lds, in, nx, ny, nz, bz + p);
+			}
 		}
 		__syncthreads();
-		const int lz = wave;
+		// ring slot offsets of the planes z - 1, z, z + 1 of this wave's output slice
+		uint32_t sdz[3];
+#pragma unroll
+		for (int d = 0; d < 3; ++d) {
+			sdz[d] = ((uint32_t)(bz + lz + d - 1 + kZSlots) & (uint32_t)(kZSlots - 1)) * PL * 4;
+		}
 #pragma unroll 1
 		for (int ly = 0; ly < BY; ++ly) {
 			float4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
 			for (int m = 0; m < NM; ++m) {
-				int t, c0;
-				if (CIN == 16) {
-					t = m;
-					c0 = 4 * grp;
+				short4_t b;
+				if constexpr (CIN == 16) {
+					const int t = m;  // compile-time offsets: the reads take immediate offsets
+					const uint32_t off = sdz[tap_dz(t) + 1] +
+					                     (uint32_t)((((ly + 1 + tap_dy(t)) * HX) + tap_dx(t)) * 4);
+					b = *reinterpret_cast<const short4_t *>(lds + lbase + off);
 				} else {
-					t = 4 * m + grp;
+					int t = 4 * m + grp;
 					t = t < kCnnTaps ? t : kCnnTaps - 1;  // padded taps have zero weights
-					c0 = 0;
+					const uint32_t off = sdz[tap_dz(t) + 1] +
+					                     (uint32_t)((((ly + 1 + tap_dy(t)) * HX) + tap_dx(t)) * 4);
+					b = *reinterpret_cast<const short4_t *>(lds + lbase + off);
 				}
-				const int slot = (bz + lz + tap_dz(t) + kZSlots) % kZSlots;
-				const int pv = (ly + 1 + tap_dy(t)) * HX + (col + 1 + tap_dx(t));
-				const short4_t b = *reinterpret_cast<const short4_t *>(lds + (((c0 / 4) * kZSlots + slot) * PL + pv) * 4);
 				acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[m], b, acc, 0, 0, 0);
 			}
 			const int x = bx + col, y = by + ly, z = bz + lz;
