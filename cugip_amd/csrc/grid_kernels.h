@@ -61,13 +61,89 @@ __device__ inline int discharge_tile(const int32_t *list, const int32_t *count, 
 // kInitThreads consecutive xy indices (x fastest, running on across rows, so no lane idles on a
 // partial tile), z into chunks of kInitDepth; workgroup b takes one strip × one chunk (strip-major
 // within its chunk; block_inst / inst_first_block give the grid of b in a batch), thread t the
-// column of its xy index, whose +z neighbours are its own next voxels. All loads of a voxel are
-// issued before its arithmetic and stores.
+// column of its xy index, whose +z neighbours are its own next voxels. 6C: all loads of a voxel are
+// issued before its arithmetic and stores; 26C: one residual plane at a time over the column.
 constexpr int kInitThreads = 256, kInitDepth = 8;
 
 __host__ __device__ inline int64_t init_blocks(int64_t nx, int64_t ny, int64_t nz)
 {
 	return ((nx * ny + kInitThreads - 1) / kInitThreads) * ((nz + kInitDepth - 1) / kInitDepth);
+}
+
+// The column work of k_grid_init one direction at a time: every voxel of the column for residual
+// plane d before plane d + 1, so that only one plane's address is live. Voxel by voxel, all 39
+// plane addresses of a 26-connected grid would stay live across the column, more than the scalar
+// registers hold. State bits and inflow sums stay in registers until the last plane.
+template <int kConn>
+__device__ inline void init_column_by_plane(const GridView &g, int x, int y, int oz, uint32_t xy, uint32_t sxy,
+                                            long long &cancelled, bool &negative, bool &overflow, bool &wide)
+{
+	using D = Dirs<kConn>;
+	constexpr int K = D::K, H = D::H, Z = kInitDepth;
+	uint32_t state[Z];
+	long long incoming[Z];
+#pragma unroll
+	for (int k = 0; k < Z; ++k) {
+		state[k] = 0;
+		incoming[k] = 0;
+		if (oz + k >= g.nz) {
+			continue;
+		}
+		const int32_t cs = tile_at(g.cs, g, oz, k, xy, sxy), ct = tile_at(g.ct, g, oz, k, xy, sxy);
+		negative |= (cs < 0) | (ct < 0);
+		const int32_t m = cs < ct ? cs : ct;
+		cancelled += m;
+		incoming[k] = cs - m;
+		state[k] = (cs - m > 0 ? kStateExcess : 0u) | (ct - m > 0 ? kStateSink : 0u);
+		tile_at(g.e, g, oz, k, xy, sxy) = cs - m;
+		tile_at(g.rt, g, oz, k, xy, sxy) = ct - m;
+	}
+#pragma unroll 1
+	for (int d = 0; d < K; ++d) {
+		const int dx = D::dx(d), dy = D::dy(d), dz = D::dz(d);
+		const bool in_xy = x + dx >= 0 && x + dx < g.nx && y + dy >= 0 && y + dy < g.ny;
+		// edge pair h: stored at v for the negative-half direction d < H, at the neighbour
+		// u = v + offset(d) for its mirror (the pair's capacity towards v is then u's own)
+		const bool half = d < H;
+		const int h = half ? d : d - H;
+		const int32_t *pn = g.cn + (int64_t)h * g.nvox;
+		const int32_t *pb = g.cnb ? g.cnb + (int64_t)h * g.nvox : nullptr;
+		const uint32_t uxy = half ? xy : slice_xy(g, x + dx, y + dy);
+		const int sk = half ? 0 : dz;
+		int32_t *pr = g.r + (int64_t)d * g.nvox;
+		int32_t out[Z], in[Z];
+#pragma unroll
+		for (int k = 0; k < Z; ++k) {
+			out[k] = in[k] = 0;
+			const int z = oz + k;
+			if (z < g.nz && in_xy && z + dz >= 0 && z + dz < g.nz) {
+				const int32_t c = tile_at(pn, g, oz, k + sk, uxy, sxy);
+				const int32_t b = pb ? tile_at(pb, g, oz, k + sk, uxy, sxy) : c;
+				out[k] = half ? c : b;
+				in[k] = half ? b : c;
+			}
+		}
+#pragma unroll
+		for (int k = 0; k < Z; ++k) {
+			if (oz + k >= g.nz) {
+				continue;
+			}
+			negative |= (out[k] < 0) | (in[k] < 0);
+			overflow |= ((long long)out[k] + (long long)in[k]) > 0x7fffffffLL;
+			wide |= ((long long)out[k] + (long long)in[k]) > 0x7fffLL;
+			incoming[k] += in[k];
+			tile_at(pr, g, oz, k, xy, sxy) = out[k];
+			state[k] |= out[k] > 0 ? (1u << d) : 0u;
+		}
+	}
+#pragma unroll
+	for (int k = 0; k < Z; ++k) {
+		if (oz + k >= g.nz) {
+			continue;
+		}
+		overflow |= incoming[k] > 0x7fffffffLL;
+		tile_at(g.omask, g, oz, k, xy, sxy) = state[k];
+	}
 }
 
 template <int kConn>
@@ -96,9 +172,13 @@ __global__ __launch_bounds__(kInitThreads) void k_grid_init(GridView gv, const i
 	const int oz = chunk * kInitDepth;
 	long long cancelled = 0;
 	bool negative = false, overflow = false, wide = false;
-	// the whole column unrolled for 6C; 26C keeps 2 voxels in flight (52 capacities each)
-	constexpr int kUnroll = kConn == 6 ? kInitDepth : 2;
-	if (xy < sxy) {
+	// 6C: the whole column unrolled, voxel by voxel; 26C: plane by plane
+	constexpr int kUnroll = kInitDepth;
+	if constexpr (kConn == 26) {
+		if (xy < sxy) {
+			init_column_by_plane<kConn>(g, x, y, oz, xy, sxy, cancelled, negative, overflow, wide);
+		}
+	} else if (xy < sxy) {
 #pragma unroll kUnroll
 		for (int k = 0; k < kInitDepth; ++k) {
 			const int z = oz + k;
