@@ -63,25 +63,34 @@ __device__ inline int discharge_tile(const int32_t *list, const int32_t *count, 
 // within its chunk; block_inst / inst_first_block give the grid of b in a batch), thread t the
 // column of its xy index, whose +z neighbours are its own next voxels. 6C: all loads of a voxel are
 // issued before its arithmetic and stores; 26C: one residual plane at a time over the column.
-constexpr int kInitThreads = 256, kInitDepth = 8;
+// 26C instead cuts the xy plane into kInitTileX x kInitTileY rectangles, so that the columns whose
+// capacities a column's mirror residuals read (its 8 xy neighbours) mostly belong to the same
+// workgroup.
+constexpr int kInitThreads = 256, kInitDepth = 8, kInitTileX = 64, kInitTileY = kInitThreads / kInitTileX;
 
-__host__ __device__ inline int64_t init_blocks(int64_t nx, int64_t ny, int64_t nz)
+__host__ __device__ inline int64_t init_blocks(int64_t nx, int64_t ny, int64_t nz, int conn)
 {
-	return ((nx * ny + kInitThreads - 1) / kInitThreads) * ((nz + kInitDepth - 1) / kInitDepth);
+	const int64_t per_chunk = conn == 26 ? ((nx + kInitTileX - 1) / kInitTileX) * ((ny + kInitTileY - 1) / kInitTileY)
+	                                     : (nx * ny + kInitThreads - 1) / kInitThreads;
+	return per_chunk * ((nz + kInitDepth - 1) / kInitDepth);
 }
 
-// The column work of k_grid_init one direction at a time: every voxel of the column for residual
-// plane d before plane d + 1, so that only one plane's address is live. Voxel by voxel, all 39
-// plane addresses of a 26-connected grid would stay live across the column, more than the scalar
-// registers hold. State bits and inflow sums stay in registers until the last plane.
-template <int kConn>
+// The column work of k_grid_init one capacity plane at a time: plane h gives residual plane h of
+// every voxel of the column (the pair's capacity stored at the voxel) and its mirror plane h + H
+// (the capacity stored at the neighbour v - offset(h)), both loaded together so that the
+// neighbour's column, which a nearby thread of the workgroup loads for its own plane h, is read
+// from the cache. Only one plane's address is live: voxel by voxel, all 39 plane addresses of a
+// 26-connected grid would stay live across the column, more than the scalar registers hold.
+// State bits and inflow sums stay in registers until the last plane.
+template <int kConn, bool kBack>
 __device__ inline void init_column_by_plane(const GridView &g, int x, int y, int oz, uint32_t xy, uint32_t sxy,
                                             long long &cancelled, bool &negative, bool &overflow, bool &wide)
 {
 	using D = Dirs<kConn>;
-	constexpr int K = D::K, H = D::H, Z = kInitDepth;
-	uint32_t state[Z];
-	long long incoming[Z];
+	constexpr int H = D::H, Z = kInitDepth;
+	// inflow sums saturate at 2^31 (each term is below 2^31, so the 32-bit sum cannot wrap): only
+	// whether one exceeds int32 matters; likewise a capacity pair's sum of two values below 2^31
+	uint32_t state[Z], incoming[Z];
 #pragma unroll
 	for (int k = 0; k < Z; ++k) {
 		state[k] = 0;
@@ -93,47 +102,65 @@ __device__ inline void init_column_by_plane(const GridView &g, int x, int y, int
 		negative |= (cs < 0) | (ct < 0);
 		const int32_t m = cs < ct ? cs : ct;
 		cancelled += m;
-		incoming[k] = cs - m;
+		incoming[k] = (uint32_t)(cs - m);
 		state[k] = (cs - m > 0 ? kStateExcess : 0u) | (ct - m > 0 ? kStateSink : 0u);
 		tile_at(g.e, g, oz, k, xy, sxy) = cs - m;
 		tile_at(g.rt, g, oz, k, xy, sxy) = ct - m;
 	}
 #pragma unroll 1
-	for (int d = 0; d < K; ++d) {
-		const int dx = D::dx(d), dy = D::dy(d), dz = D::dz(d);
-		const bool in_xy = x + dx >= 0 && x + dx < g.nx && y + dy >= 0 && y + dy < g.ny;
-		// edge pair h: stored at v for the negative-half direction d < H, at the neighbour
-		// u = v + offset(d) for its mirror (the pair's capacity towards v is then u's own)
-		const bool half = d < H;
-		const int h = half ? d : d - H;
+	for (int h = 0; h < H; ++h) {
+		// direction h leads to v + offset(h); its mirror h + H to u = v - offset(h), whose own
+		// plane-h capacity pair is the edge (u, v)
+		const int dx = D::dx(h), dy = D::dy(h), dz = D::dz(h);
+		const bool own_xy = x + dx >= 0 && x + dx < g.nx && y + dy >= 0 && y + dy < g.ny;
+		const bool mir_xy = x - dx >= 0 && x - dx < g.nx && y - dy >= 0 && y - dy < g.ny;
 		const int32_t *pn = g.cn + (int64_t)h * g.nvox;
-		const int32_t *pb = g.cnb ? g.cnb + (int64_t)h * g.nvox : nullptr;
-		const uint32_t uxy = half ? xy : slice_xy(g, x + dx, y + dy);
-		const int sk = half ? 0 : dz;
-		int32_t *pr = g.r + (int64_t)d * g.nvox;
-		int32_t out[Z], in[Z];
+		const int32_t *pb = kBack ? g.cnb + (int64_t)h * g.nvox : nullptr;
+		// every lane loads (from its own column where u lies outside the grid in x or y) and keeps
+		// the value only for an edge that exists: no divergent loads; the z tests are uniform
+		const uint32_t uxy = mir_xy ? slice_xy(g, x - dx, y - dy) : xy;
+		// the column's offset made opaque to the compiler inside each plane's step, so that its
+		// per-slice offsets are not hoisted out of the loop as 64-bit values (two registers each
+		// across all planes) but formed next to each load and store as 32-bit offsets
+		uint32_t vxy = xy;
+		asm volatile("" : "+v"(vxy));
+		int32_t out[Z], in[Z], mout[Z], min_[Z];
 #pragma unroll
 		for (int k = 0; k < Z; ++k) {
-			out[k] = in[k] = 0;
+			out[k] = in[k] = mout[k] = min_[k] = 0;
 			const int z = oz + k;
-			if (z < g.nz && in_xy && z + dz >= 0 && z + dz < g.nz) {
-				const int32_t c = tile_at(pn, g, oz, k + sk, uxy, sxy);
-				const int32_t b = pb ? tile_at(pb, g, oz, k + sk, uxy, sxy) : c;
-				out[k] = half ? c : b;
-				in[k] = half ? b : c;
+			if (z >= g.nz) {
+				continue;
 			}
+			const bool own = own_xy && (unsigned)(z + dz) < (unsigned)g.nz;
+			const int32_t c = tile_at(pn, g, oz, k, vxy, sxy);
+			const int32_t b = kBack ? tile_at(pb, g, oz, k, vxy, sxy) : c;
+			out[k] = own ? c : 0;
+			in[k] = own ? b : 0;
+			// u's slice, or the voxel's own one where u lies below or above the grid
+			const bool mz = (unsigned)(z - dz) < (unsigned)g.nz;
+			const int64_t ms = slice_start(g, mz ? z - dz : z);
+			const int32_t mc = at(pn + ms, uxy);
+			const int32_t mb = kBack ? at(pb + ms, uxy) : mc;
+			min_[k] = mir_xy && mz ? mc : 0;
+			mout[k] = mir_xy && mz ? mb : 0;
 		}
+		int32_t *pr = g.r + (int64_t)h * g.nvox, *pm = g.r + (int64_t)(h + H) * g.nvox;
 #pragma unroll
 		for (int k = 0; k < Z; ++k) {
 			if (oz + k >= g.nz) {
 				continue;
 			}
-			negative |= (out[k] < 0) | (in[k] < 0);
-			overflow |= ((long long)out[k] + (long long)in[k]) > 0x7fffffffLL;
-			wide |= ((long long)out[k] + (long long)in[k]) > 0x7fffLL;
-			incoming[k] += in[k];
-			tile_at(pr, g, oz, k, xy, sxy) = out[k];
-			state[k] |= out[k] > 0 ? (1u << d) : 0u;
+			negative |= (out[k] | in[k] | mout[k] | min_[k]) < 0;
+			const uint32_t pair = (uint32_t)out[k] + (uint32_t)in[k];
+			const uint32_t mpair = (uint32_t)mout[k] + (uint32_t)min_[k];
+			overflow |= (pair | mpair) > 0x7fffffffu;
+			wide |= (pair | mpair) > 0x7fffu;
+			incoming[k] = min(incoming[k] + (uint32_t)in[k], 0x80000000u);
+			incoming[k] = min(incoming[k] + (uint32_t)min_[k], 0x80000000u);
+			tile_at(pr, g, oz, k, vxy, sxy) = out[k];
+			tile_at(pm, g, oz, k, vxy, sxy) = mout[k];
+			state[k] |= (out[k] > 0 ? (1u << h) : 0u) | (mout[k] > 0 ? (1u << (h + H)) : 0u);
 		}
 	}
 #pragma unroll
@@ -141,7 +168,7 @@ __device__ inline void init_column_by_plane(const GridView &g, int x, int y, int
 		if (oz + k >= g.nz) {
 			continue;
 		}
-		overflow |= incoming[k] > 0x7fffffffLL;
+		overflow |= incoming[k] > 0x7fffffffu;
 		tile_at(g.omask, g, oz, k, xy, sxy) = state[k];
 	}
 }
@@ -165,62 +192,76 @@ __global__ __launch_bounds__(kInitThreads) void k_grid_init(GridView gv, const i
 		g = instance_at(gv, i);
 	}
 	const uint32_t sxy = (uint32_t)(g.nx * g.ny);
-	const int strips = (int)((sxy + kInitThreads - 1) / kInitThreads);
-	const int chunk = lb / strips, strip = lb - chunk * strips;
-	const uint32_t xy = (uint32_t)strip * kInitThreads + threadIdx.x;
-	const int y = (int)(xy / (uint32_t)g.nx), x = (int)(xy - (uint32_t)y * (uint32_t)g.nx);
-	const int oz = chunk * kInitDepth;
 	long long cancelled = 0;
 	bool negative = false, overflow = false, wide = false;
 	// 6C: the whole column unrolled, voxel by voxel; 26C: plane by plane
 	constexpr int kUnroll = kInitDepth;
 	if constexpr (kConn == 26) {
-		if (xy < sxy) {
-			init_column_by_plane<kConn>(g, x, y, oz, xy, sxy, cancelled, negative, overflow, wide);
-		}
-	} else if (xy < sxy) {
-#pragma unroll kUnroll
-		for (int k = 0; k < kInitDepth; ++k) {
-			const int z = oz + k;
-			if (z >= g.nz) {
-				continue;
+		const int tiles_x = (g.nx + kInitTileX - 1) / kInitTileX;
+		const int tiles = tiles_x * ((g.ny + kInitTileY - 1) / kInitTileY);
+		const int chunk = lb / tiles, t = lb - chunk * tiles;
+		const int ty = t / tiles_x, tx = t - ty * tiles_x;
+		const int x = tx * kInitTileX + (int)(threadIdx.x % kInitTileX);
+		const int y = ty * kInitTileY + (int)(threadIdx.x / kInitTileX);
+		if (x < g.nx && y < g.ny) {
+			if (g.cnb) {
+				init_column_by_plane<kConn, true>(g, x, y, chunk * kInitDepth, slice_xy(g, x, y), sxy, cancelled,
+				                                  negative, overflow, wide);
+			} else {
+				init_column_by_plane<kConn, false>(g, x, y, chunk * kInitDepth, slice_xy(g, x, y), sxy, cancelled,
+				                                   negative, overflow, wide);
 			}
-			int32_t out[K], in[K];
-			const int32_t cs = tile_at(g.cs, g, oz, k, xy, sxy), ct = tile_at(g.ct, g, oz, k, xy, sxy);
-#pragma unroll
-			for (int d = 0; d < K; ++d) {
-				out[d] = in[d] = 0;
-				if (!inside(g, x + D::dx(d), y + D::dy(d), z + D::dz(d))) {
+		}
+	} else {
+		const int strips = (int)((sxy + kInitThreads - 1) / kInitThreads);
+		const int chunk = lb / strips, strip = lb - chunk * strips;
+		const uint32_t xy = (uint32_t)strip * kInitThreads + threadIdx.x;
+		const int y = (int)(xy / (uint32_t)g.nx), x = (int)(xy - (uint32_t)y * (uint32_t)g.nx);
+		const int oz = chunk * kInitDepth;
+		if (xy < sxy) {
+#pragma unroll kUnroll
+			for (int k = 0; k < kInitDepth; ++k) {
+				const int z = oz + k;
+				if (z >= g.nz) {
 					continue;
 				}
-				if (d < H) {
-					out[d] = tile_at(g.cn + (int64_t)d * g.nvox, g, oz, k, xy, sxy);
-					in[d] = g.cnb ? tile_at(g.cnb + (int64_t)d * g.nvox, g, oz, k, xy, sxy) : out[d];
-				} else {
-					const uint32_t uxy = slice_xy(g, x + D::dx(d), y + D::dy(d));
-					const int uk = k + D::dz(d);
-					in[d] = tile_at(g.cn + (int64_t)(d - H) * g.nvox, g, oz, uk, uxy, sxy);
-					out[d] = g.cnb ? tile_at(g.cnb + (int64_t)(d - H) * g.nvox, g, oz, uk, uxy, sxy) : in[d];
-				}
-			}
-			negative |= (cs < 0) | (ct < 0);
-			const int32_t m = cs < ct ? cs : ct;
-			cancelled += m;
-			long long incoming = cs - m;
-			uint32_t state = (cs - m > 0 ? kStateExcess : 0u) | (ct - m > 0 ? kStateSink : 0u);
+				int32_t out[K], in[K];
+				const int32_t cs = tile_at(g.cs, g, oz, k, xy, sxy), ct = tile_at(g.ct, g, oz, k, xy, sxy);
 #pragma unroll
-			for (int d = 0; d < K; ++d) {
-				negative |= (out[d] < 0) | (in[d] < 0);
-				overflow |= ((long long)out[d] + (long long)in[d]) > 0x7fffffffLL;
-				wide |= ((long long)out[d] + (long long)in[d]) > 0x7fffLL;
-				incoming += in[d];
-				tile_at(g.r + (int64_t)d * g.nvox, g, oz, k, xy, sxy) = out[d];
-				state |= out[d] > 0 ? (1u << d) : 0u;
+				for (int d = 0; d < K; ++d) {
+					out[d] = in[d] = 0;
+					if (!inside(g, x + D::dx(d), y + D::dy(d), z + D::dz(d))) {
+						continue;
+					}
+					if (d < H) {
+						out[d] = tile_at(g.cn + (int64_t)d * g.nvox, g, oz, k, xy, sxy);
+						in[d] = g.cnb ? tile_at(g.cnb + (int64_t)d * g.nvox, g, oz, k, xy, sxy) : out[d];
+					} else {
+						const uint32_t uxy = slice_xy(g, x + D::dx(d), y + D::dy(d));
+						const int uk = k + D::dz(d);
+						in[d] = tile_at(g.cn + (int64_t)(d - H) * g.nvox, g, oz, uk, uxy, sxy);
+						out[d] = g.cnb ? tile_at(g.cnb + (int64_t)(d - H) * g.nvox, g, oz, uk, uxy, sxy) : in[d];
+					}
+				}
+				negative |= (cs < 0) | (ct < 0);
+				const int32_t m = cs < ct ? cs : ct;
+				cancelled += m;
+				long long incoming = cs - m;
+				uint32_t state = (cs - m > 0 ? kStateExcess : 0u) | (ct - m > 0 ? kStateSink : 0u);
+#pragma unroll
+				for (int d = 0; d < K; ++d) {
+					negative |= (out[d] < 0) | (in[d] < 0);
+					overflow |= ((long long)out[d] + (long long)in[d]) > 0x7fffffffLL;
+					wide |= ((long long)out[d] + (long long)in[d]) > 0x7fffLL;
+					incoming += in[d];
+					tile_at(g.r + (int64_t)d * g.nvox, g, oz, k, xy, sxy) = out[d];
+					state |= out[d] > 0 ? (1u << d) : 0u;
+				}
+				overflow |= incoming > 0x7fffffffLL;
+				tile_at(g.omask, g, oz, k, xy, sxy) = state;
+				tile_at(g.e, g, oz, k, xy, sxy) = cs - m;
+				tile_at(g.rt, g, oz, k, xy, sxy) = ct - m;
 			}
-			overflow |= incoming > 0x7fffffffLL;
-			tile_at(g.omask, g, oz, k, xy, sxy) = state;
-			tile_at(g.e, g, oz, k, xy, sxy) = cs - m;
-			tile_at(g.rt, g, oz, k, xy, sxy) = ct - m;
 		}
 	}
 	const int bits = (negative ? 1 : 0) | (overflow ? 2 : 0) | (wide ? 4 : 0);

@@ -537,7 +537,7 @@ void GridSolver::configure_tiles(const Shape &shape)
 		int64_t blocks = 0;
 		for (size_t i = 0; i < mInst.size(); ++i) {
 			first[i] = (int32_t)blocks;
-			blocks += init_blocks(mInst[i].nx, mInst[i].ny, mInst[i].nz);
+			blocks += init_blocks(mInst[i].nx, mInst[i].ny, mInst[i].nz, mConn);
 		}
 		mInitBlocks = (int)blocks;
 		if (mInst.size() > 1) {
