@@ -75,6 +75,18 @@ __host__ __device__ inline int64_t init_blocks(int64_t nx, int64_t ny, int64_t n
 	return per_chunk * ((nz + kInitDepth - 1) / kInitDepth);
 }
 
+// Column (x, y) and z-chunk of thread threadIdx.x of init workgroup lb in the xy-tile layout.
+__device__ inline void init_tile_coords(const GridView &g, int lb, int &x, int &y, int &chunk)
+{
+	const int tiles_x = (g.nx + kInitTileX - 1) / kInitTileX;
+	const int tiles = tiles_x * ((g.ny + kInitTileY - 1) / kInitTileY);
+	chunk = lb / tiles;
+	const int t = lb - chunk * tiles;
+	const int ty = t / tiles_x, tx = t - ty * tiles_x;
+	x = tx * kInitTileX + (int)(threadIdx.x % kInitTileX);
+	y = ty * kInitTileY + (int)(threadIdx.x / kInitTileX);
+}
+
 // The column work of k_grid_init one capacity plane at a time: plane h gives residual plane h of
 // every voxel of the column (the pair's capacity stored at the voxel) and its mirror plane h + H
 // (the capacity stored at the neighbour v - offset(h)), both loaded together so that the
@@ -197,12 +209,8 @@ __global__ __launch_bounds__(kInitThreads) void k_grid_init(GridView gv, const i
 	// 6C: the whole column unrolled, voxel by voxel; 26C: plane by plane
 	constexpr int kUnroll = kInitDepth;
 	if constexpr (kConn == 26) {
-		const int tiles_x = (g.nx + kInitTileX - 1) / kInitTileX;
-		const int tiles = tiles_x * ((g.ny + kInitTileY - 1) / kInitTileY);
-		const int chunk = lb / tiles, t = lb - chunk * tiles;
-		const int ty = t / tiles_x, tx = t - ty * tiles_x;
-		const int x = tx * kInitTileX + (int)(threadIdx.x % kInitTileX);
-		const int y = ty * kInitTileY + (int)(threadIdx.x / kInitTileX);
+		int x, y, chunk;
+		init_tile_coords(g, lb, x, y, chunk);
 		if (x < g.nx && y < g.ny) {
 			if (g.cnb) {
 				init_column_by_plane<kConn, true>(g, x, y, chunk * kInitDepth, slice_xy(g, x, y), sxy, cancelled,
