@@ -19,7 +19,7 @@ import time
 import cugip_amd
 
 SHAPES = {6: [(32, 8, 4), (32, 8, 8), (64, 4, 4), (32, 4, 4), (64, 4, 8), (16, 8, 4), (16, 4, 4), (32, 2, 4)],
-          26: [(32, 4, 4), (16, 8, 4), (16, 4, 4)]}
+          26: [(32, 4, 4), (16, 8, 4), (16, 4, 4), (32, 4, 2)]}
 PHASES = {6: [2, 4, 6, 8, 12, 16], 26: [8, 16, 24]}
 ITERATIONS = [1, 2, 4, 8, 16, 32]
 

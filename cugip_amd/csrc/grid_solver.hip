@@ -33,7 +33,8 @@ struct ShapeTag {
 	F(6, 32, 2, 4)         \
 	F(26, 32, 4, 4)        \
 	F(26, 16, 8, 4)        \
-	F(26, 16, 4, 4)
+	F(26, 16, 4, 4)        \
+	F(26, 32, 4, 2)
 
 template <typename Fn>
 void dispatch_shape(int conn, int tx, int ty, int tz, Fn &&fn)

@@ -21,9 +21,9 @@ static const TunedPolicy kTunedPolicies[] = {
 	{6, 0, 2097152LL, 16, 4, 4, 6, 8},  // grid6_small: 1.00 ms
 	{6, 0, INT64_MAX, 32, 2, 4, 8, 32},  // grid6_large: 3.15 ms
 	{6, 1, INT64_MAX, 32, 2, 4, 8, 8},  // batch6: 2.36 ms
-	{26, 0, 4194304LL, 32, 4, 4, 24, 2},  // grid26_small: 11.44 ms
-	{26, 0, INT64_MAX, 32, 4, 4, 24, 2},  // grid26: 15.96 ms
-	{26, 1, INT64_MAX, 16, 8, 4, 24, 1},  // batch26: 4.88 ms
+	{26, 0, 4194304LL, 32, 4, 2, 24, 2},  // grid26_small: 6.88 ms
+	{26, 0, INT64_MAX, 32, 4, 2, 24, 2},  // grid26: 10.69 ms
+	{26, 1, INT64_MAX, 32, 4, 2, 24, 2},  // batch26: 3.01 ms
 };
 
 inline TunedPolicy tuned_policy(int conn, bool batch, int64_t voxels_per_grid)

@@ -43,6 +43,7 @@ CASES = [
     ((24, 20, 16), 26, 1, 0, {}),
     ((40, 33, 17), 26, 2, 0, dict(tile_x=16, tile_y=8, tile_z=4)),
     ((35, 18, 21), 26, 3, 4, dict(tile_x=16, tile_y=4, tile_z=4)),
+    ((41, 23, 19), 26, 5, 1, dict(tile_x=32, tile_y=4, tile_z=2)),
     ((48, 40, 32), 26, 0, 0, dict(discharges_per_relabel=3)),
 ]
 
@@ -168,6 +169,7 @@ def test_golden_fixtures(golden):
 
 @pytest.mark.parametrize("conn,policy", [
     (26, dict(tile_x=16, tile_y=4, tile_z=4)),
+    (26, dict(tile_x=32, tile_y=4, tile_z=2)),
     (26, dict(tile_x=16, tile_y=4, tile_z=4, discharges_per_relabel=5)),
     (6, dict(engine=cugip_amd.ENGINE_GRAPH)),
     (6, dict(tile_x=32, tile_y=2, tile_z=4, discharges_per_relabel=3, max_tile_iterations=4)),
